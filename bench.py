@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark of the CSEM forward hot path on one MI355X.
+
+Workload (the largest single-GPU configuration the headline metric is quoted
+on): p = 2 Nedelec elements on a 40^3 Kuhn grid (384,000 tetrahedra,
+2,480,240 DOFs, 106,797,280 CSR non-zeros), layered marine model with seeded
+log-normal noise (sigma_log = 0.1), x-directed HED at (0, 0, -950) m, 1 Hz.
+
+One step = numeric assembly (element geometry + fused atomic-free CSR
+assembly) + point-dipole source + PEC boundary condition + preconditioner
+set-up + Krylov solve to a true relative residual <= 1e-8, with every input
+already resident in HBM.  The headline value is the mixed-precision step
+time (complex64 Krylov with FP64 residual replacement); the FP64 step time is
+reported beside it.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+METRIC = "p=2 CSEM solve time @1e-8 relres (FP64 vs mixed); batched RHS/s; adjoint grad/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmv_bytes(nnz, n, bv):
+    """Algorithmic bytes of one CSR SpMV: values + int32 columns + row pointers + x, y."""
+    return nnz * (bv + 4) + 4 * (n + 1) + 2 * n * bv
+
+
+def cpu_baseline(w, plan_rowptr, plan_cols, values_c128, b_c128, iters_fp64, budget_s):
+    """Oracle restatement timed on the host (single thread) on a bounded sample."""
+    from threadpoolctl import threadpool_limits
+    import scipy.sparse as sp
+    from oracle import fem, krylov, topology as otopo
+
+    T = w.topo
+    with threadpool_limits(1):
+        # (a) element matrices + assembly: a sample of elements, extrapolated linearly
+        ns = 3000
+        sel = np.arange(ns)
+        eo, fo = otopo.orientations(T.elems_nodes[sel], T.elems_edges[sel], T.edges_nodes,
+                                    T.elem_faces_edges[sel])
+        t0 = time.perf_counter()
+        M, K = fem.all_element_matrices(T.coords[sel], eo, fo, w.sigma[sel], T.p)
+        fem.assemble(T.dofs[sel], M, K, w.frequency, T.ndof)
+        t_asm = (time.perf_counter() - t0) * T.n_elems / ns
+        # (b) COCG + Jacobi iterations with scipy CSR SpMV on the full system
+        A = sp.csr_matrix((values_c128, plan_cols, plan_rowptr), shape=(T.ndof, T.ndof))
+        dinv = 1.0 / A.diagonal()
+        k = 3
+        while True:
+            t0 = time.perf_counter()
+            krylov.cocg(A, b_c128, dinv, tol=1e-300, maxit=k)
+            dt = time.perf_counter() - t0
+            if dt > 0.3 * budget_s or k >= 400:
+                break
+            k = max(k + 1, int(k * min(8.0, 0.35 * budget_s / max(dt, 1e-3))))
+        t_it = dt / k
+    total = t_asm + t_it * iters_fp64
+    return dict(value=total, unit="s", cores=1, kind="port",
+                sample=(f"oracle (numpy/scipy) on 1 host thread: element matrices+assembly of "
+                        f"{ns} of {T.n_elems} elements extrapolated linearly ({t_asm:.1f} s), plus "
+                        f"{k} COCG+Jacobi iterations with scipy CSR SpMV on the full system "
+                        f"({t_it*1e3:.1f} ms/iter) x {iters_fp64} iterations (the GPU FP64 count)"),
+                assembly_s=t_asm, per_iteration_s=t_it)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of host work for the CPU baseline sample (0 = skip)")
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the multi-GPU sharding of the "
+                          "path is described in DESIGN.md but not implemented; run --gpus 1"}))
+        sys.exit(2)
+
+    import torch
+    from petgem_amd import core
+    from petgem_amd.synthetic import CONFIGS, Workload
+
+    cfg = CONFIGS[args.config]
+    t0 = time.perf_counter()
+    w = Workload(**cfg, seed=2026)
+    T = w.topo
+    log(f"workload {args.config}: {w.describe()} built in {time.perf_counter()-t0:.1f}s")
+    dev = torch.device("cuda")
+    t0 = time.perf_counter()
+    mesh = core.DeviceMesh.from_topology(T, dev)
+    asm = core.Assembler(mesh)
+    plan = asm.plan
+    sigma = torch.from_numpy(w.sigma).to(dev)
+    ent_bc, dof_bc = mesh.entity_mask(T.boundary_dofs)
+    omega = core.omega_of(w.frequency)
+    moment = core.dipole_moment(w.source)
+    src_el = int(core.locate_points(mesh, np.array(w.source["position"])[None]).cpu()[0])
+    values = torch.empty((plan.nnz, 2), dtype=torch.float64, device=dev)
+    b = torch.zeros((plan.ndof, 2), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    log(f"setup (upload + assembly plan) {time.perf_counter()-t0:.1f}s; nnz={plan.nnz}")
+
+    def assemble():
+        asm.geometry(sigma)
+        asm.assemble(omega, out=values)
+        b.zero_()
+        core.source_vector(mesh, src_el, w.source["position"], moment, omega, b=b)
+        core.apply_dirichlet(plan, values, dof_bc, b)
+
+    assemble()
+    solvers = {prec: core.KrylovSolver(plan, values, precision=prec, rtol=args.rtol)
+               for prec in ("fp64", "mixed")}
+    results = {}
+    for prec in ("fp64", "mixed"):
+        S = solvers[prec]
+        times, infos = [], []
+        for it in range(args.warmup + args.steps):
+            x = torch.zeros_like(b)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            assemble()
+            S.refresh()
+            x, info = S.solve(b, x)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if info["status"] != 0 or info["relres"] > args.rtol:
+                raise RuntimeError(f"{prec} solve did not converge: {info}")
+            if it >= args.warmup:
+                times.append(dt)
+                infos.append(info)
+            log(f"{prec} step {it}: {dt*1e3:.1f} ms, {info}")
+        results[prec] = dict(t=float(np.median(times)), times=times, info=infos[-1], x=x)
+
+    # assembly alone (HIP events on the current stream)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(5):
+        assemble()
+    ev1.record()
+    torch.cuda.synchronize()
+    t_asm_ms = ev0.elapsed_time(ev1) / 5
+
+    # receivers (mixed solution) and agreement with FP64
+    rx_el = core.locate_points(mesh, w.receivers)
+    f64 = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, results["fp64"]["x"], omega))
+    f32 = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, results["mixed"]["x"], omega))
+    field_diff = float(np.abs(f64 - f32).max() / np.abs(f64).max())
+
+    # roofline of the dominant kernel: the complex64 SpMV of the mixed solve
+    prof = solvers["mixed"].profile(reps=20)
+    prof64 = solvers["fp64"].profile(reps=20)
+    n, nnz = plan.ndof, plan.nnz
+    by32 = spmv_bytes(nnz, n, 8)
+    by64 = spmv_bytes(nnz, n, 16)
+    ach32 = by32 / (prof["ms32"] * 1e-3) / 1e9
+    ach64 = by64 / (prof64["ms64"] * 1e-3) / 1e9
+    log(f"SpMV c64 {prof['ms32']:.3f} ms ({ach32:.0f} GB/s, G={prof['g32']}), "
+        f"c128 {prof64['ms64']:.3f} ms ({ach64:.0f} GB/s, G={prof64['g64']}); assembly {t_asm_ms:.2f} ms")
+
+    cpu = None
+    if args.cpu_budget > 0:
+        log("cpu baseline ...")
+        vals_h = core.as_complex(values)
+        b_h = core.as_complex(b)
+        cpu = cpu_baseline(w, plan.h_rowptr, plan.h_cols, vals_h, b_h,
+                           results["fp64"]["info"]["iterations"], args.cpu_budget)
+        log(f"cpu baseline {cpu['value']:.1f} s")
+
+    t_mixed, t_fp64 = results["mixed"]["t"], results["fp64"]["t"]
+    line = {
+        "metric": METRIC,
+        "value": round(t_mixed, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64 Krylov/SpMV + complex128 residual, solution, assembly",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{args.config}: p={T.p} Nedelec, {w.n}^3 Kuhn hexes x 6 = {T.n_elems} "
+                         f"tets, {T.ndof} DOFs, {nnz} nnz, layered marine model"
+                         f"{' + log-normal noise 0.1' if cfg.get('noise') else ''}, "
+                         f"HED at (0,0,-950) m, {w.frequency} Hz, rtol {args.rtol}"),
+            "solver": "COCG + Jacobi, true-residual stop",
+        },
+        "fp64_s": round(t_fp64, 6),
+        "mixed_s": round(t_mixed, 6),
+        "mixed_speedup": round(t_fp64 / t_mixed, 3),
+        "iterations": {"fp64": results["fp64"]["info"]["iterations"],
+                       "mixed": results["mixed"]["info"]["iterations"],
+                       "mixed_refinements": results["mixed"]["info"]["refinements"]},
+        "relres": {"fp64": results["fp64"]["info"]["relres"],
+                   "mixed": results["mixed"]["info"]["relres"]},
+        "receiver_field_diff_mixed_vs_fp64": field_diff,
+        "assembly_ms": round(t_asm_ms, 3),
+        "roofline": {
+            "bound": "hbm", "kernel": f"k_spmv<{prof['g32']},1,float2,float2> (complex64 SpMV + p^T q)",
+            "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes": by32, "avg_ms": round(prof["ms32"], 5),
+            "fp64_spmv": {"achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                          "avg_ms": round(prof64["ms64"], 5), "algorithmic_bytes": by64},
+        },
+        "cpu_baseline": None if cpu is None else {
+            k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "host": platform.processor() or platform.machine(),
+    }
+    if cpu is not None:
+        line["cpu_baseline"]["speedup_vs_gpu_fp64"] = round(cpu["value"] / t_fp64, 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+/*
+ * petgem_amd -- C ABI of the MI355X (gfx950) CSEM forward hot path.
+ *
+ * Every entry point returns an int status (PG_OK == 0; otherwise a PG_ERR_*
+ * code, with a message from pg_last_error()).  Device pointers are plain
+ * pointers into memory the caller owns (the Python host side allocates them
+ * as torch tensors); `stream` is a hipStream_t passed as void* (NULL = the
+ * default stream).  Complex numbers are interleaved (re, im) pairs of
+ * double (complex128) or float (complex64).  The library never frees caller
+ * memory; objects it creates (tables, plans, solvers) are released with the
+ * matching *_destroy call.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository, ocastilloreyes/petgem 1.0).
+ */
+#ifndef PETGEM_AMD_H
+#define PETGEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_OK 0
+#define PG_ERR_ARG 1         /* invalid argument or size */
+#define PG_ERR_HIP 2         /* HIP runtime error */
+#define PG_ERR_UNSUPPORTED 3 /* order / option not supported */
+#define PG_ERR_NOT_FOUND 4   /* point outside the mesh */
+#define PG_ERR_BREAKDOWN 5   /* Krylov breakdown (zero or non-finite pivot) */
+
+#define PG_ABI_VERSION 1
+
+int pg_abi_version(void);
+const char *pg_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Reference-element tables for uniform order p (1..3), resident on the
+ * current device: per-orientation basis contractions over the order-2p
+ * quadrature.  Replaces the per-element quadrature loop of
+ * computeElementalMatrices (petgem/hvfem.py:250-310) and the basis
+ * evaluation of shape3DETet (petgem/hvfem.py:319-464).
+ * ---------------------------------------------------------------------- */
+int pg_tables_create(int p, void **tables);
+int pg_tables_destroy(void *tables);
+/* Host copy of the (V x V x 12) table, V = n_variants(p); for tests. */
+int pg_tables_host(int p, double *out, int64_t out_len);
+
+/* Bytes of per-element workspace pg_element_geometry fills (E elements). */
+size_t pg_element_workspace_bytes(int64_t n_elems);
+
+/* Per-element Jacobian coefficients, conductivity tensor and packed
+ * orientation.  Inputs are the scratch arrays the reference Solver reads per
+ * element (petgem/solver.py:193-227): coords (E,4,3) f64 [nodes.dat],
+ * elems_nodes (E,4) [meshConnectivity.dat], elems_edges (E,6) [edges.dat],
+ * edges_nodes (E,6,2) [edgesNodes.dat], faces_edges (E,4,3)
+ * [facesEdges.dat], sigma (E,2) [conductivityModel.dat].
+ * Replaces computeJacobian (petgem/hvfem.py:101-119) and
+ * computeElementOrientation (petgem/hvfem.py:122-220). */
+int pg_element_geometry(int p, int64_t n_elems, const double *coords,
+                        const int32_t *elems_nodes, const int32_t *elems_edges,
+                        const int32_t *edges_nodes, const int32_t *faces_edges,
+                        const double *sigma, void *workspace, void *stream);
+
+/* Orientation codes alone: edge flags (E,6) and face codes (E,4).
+ * Replaces computeElementOrientation (petgem/hvfem.py:122-220). */
+int pg_element_orientation(int64_t n_elems, const int32_t *elems_nodes,
+                           const int32_t *elems_edges, const int32_t *edges_nodes,
+                           const int32_t *faces_edges, int32_t *edge_orient,
+                           int32_t *face_orient, void *stream);
+
+/* Element mass and curl-curl matrices M, K, each (E,n,n) f64 row-major,
+ * from a filled geometry workspace.  Replaces computeElementalMatrices
+ * (petgem/hvfem.py:223-316). */
+int pg_element_matrices(const void *tables, int64_t n_elems, const void *workspace,
+                        double *M, double *K, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Assembly plan (host-side, integer only): CSR pattern of the union of
+ * element DOF cliques with sorted columns, and, for every block of the CSR
+ * between two mesh entities (edge/face/volume), the list of contributing
+ * elements in ascending element order.  Replaces the preallocation and
+ * insertion pattern of createParallelMatrix + MatSetValues
+ * (petgem/parallel.py:150-177, petgem/solver.py:186-235).
+ * elems_edges (E,6), elems_faces (E,4) are host arrays.
+ * ---------------------------------------------------------------------- */
+int pg_plan_build(int p, int64_t n_elems, int64_t n_edges, int64_t n_faces,
+                  const int32_t *elems_edges, const int32_t *elems_faces, void **plan);
+/* info[0..5] = ndof, nnz, npairs, ncontrib, n_entities, max entity DOFs */
+int pg_plan_info(const void *plan, int64_t *info);
+/* Copy the plan into caller host arrays: rowptr (ndof+1), cols (nnz),
+ * pair_slot0 (npairs), pair_rowlen (npairs), pair_cbeg (npairs+1),
+ * pair_ab (2*npairs), contrib (ncontrib, packed e<<8 | la<<4 | lb). */
+int pg_plan_export(const void *plan, int32_t *rowptr, int32_t *cols, int32_t *pair_slot0,
+                   int32_t *pair_rowlen, int32_t *pair_cbeg, int32_t *pair_ab,
+                   uint32_t *contrib);
+int pg_plan_destroy(void *plan);
+
+/* Device view of an exported plan (all device pointers). */
+typedef struct pg_plan_dev {
+  int32_t p;
+  int64_t n_edges, n_faces, n_elems;
+  int64_t ndof, nnz, npairs;
+  const int32_t *rowptr, *cols;
+  const int32_t *pair_slot0, *pair_rowlen, *pair_cbeg, *pair_ab;
+  const uint32_t *contrib;
+} pg_plan_dev;
+
+/* Numeric assembly, atomic-free: CSR values (nnz complex128) of
+ * A = sum_e (K_e - i*omega*mu0*M_e), each slot summed over its contributing
+ * elements in ascending element order, element matrices recomputed from the
+ * tables (never stored).  entity_bc (nullable, one byte per entity) folds the
+ * PEC condition in (zero rows/columns, unit diagonal).  Replaces the element
+ * loop and assembly of Solver.assembly (petgem/solver.py:191-235). */
+int pg_assemble(const void *tables, const pg_plan_dev *plan, const void *workspace,
+                double omega, const uint8_t *entity_bc, double *values, void *stream);
+
+/* PEC boundary condition on an assembled system: rows and columns of DOFs
+ * with dof_bc != 0 zeroed, unit diagonal, b = 0 there (b nullable).
+ * Replaces A.zeroRowsColumns + b.setValues (petgem/solver.py:552-577). */
+int pg_apply_dirichlet(int64_t ndof, const int32_t *rowptr, const int32_t *cols,
+                       const uint8_t *dof_bc, double *values, double *b, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Points: location and basis evaluation
+ * ---------------------------------------------------------------------- */
+/* Lowest-index element whose barycentric coordinates are within
+ * [-tol, 1+tol] for each point, -1 if none.  Replaces
+ * Delaunay.find_simplex(bruteforce=True, tol) as used in
+ * petgem/preprocessing.py:414-420 and petgem/postprocessing.py:532-539. */
+int pg_locate_points(int64_t n_elems, const double *coords, int64_t n_points,
+                     const double *points, double tol, int32_t *element, void *stream);
+
+/* Point-dipole source b[dofs_e] += i*omega*mu0 * (I*L*dir) . N_e(x_s) for one
+ * element.  dir = (cos az cos dip, sin az cos dip, sin dip) * current*length.
+ * Replaces the CSEM source block of Solver.assembly (petgem/solver.py:247-316)
+ * with computeSourceVectorRotation, tetrahedronXYZToXiEtaZeta and
+ * computeBasisFunctions (petgem/hvfem.py:2303-2553). */
+int pg_source_vector(int p, int32_t element, const double *coords, const int32_t *elems_nodes,
+                     const int32_t *elems_edges, const int32_t *edges_nodes,
+                     const int32_t *faces_edges, const int32_t *dofs, const double *position,
+                     const double *moment, double omega, double *b, void *stream);
+
+/* Receiver fields (n_points,6) complex128: E = sum x_j N_j, H = sum x_j
+ * curl N_j / (i*omega*mu0), for points already located in `element`.
+ * Replaces fieldInterpolator (petgem/postprocessing.py:479-616). */
+int pg_receiver_fields(int p, int64_t n_points, const double *points, const int32_t *element,
+                       const double *coords, const int32_t *elems_nodes,
+                       const int32_t *elems_edges, const int32_t *edges_nodes,
+                       const int32_t *faces_edges, const int32_t *dofs, const double *x,
+                       double omega, double *fields, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Sparse kernels and Krylov solve.  Replace MatMult and KSPSolve
+ * (petgem/solver.py:584-590; PETSc KSP/PC, third-party).
+ * ---------------------------------------------------------------------- */
+/* y = A x, complex128 CSR */
+int pg_spmv_z(int64_t nrows, const int32_t *rowptr, const int32_t *cols, const double *values,
+              const double *x, double *y, void *stream);
+/* y = A x, complex64 CSR */
+int pg_spmv_c(int64_t nrows, const int32_t *rowptr, const int32_t *cols, const float *values,
+              const float *x, float *y, void *stream);
+/* values64 -> values32 (round to nearest) */
+int pg_demote_z2c(int64_t n, const double *src, float *dst, void *stream);
+
+#define PG_KSP_COCG 0
+#define PG_KSP_GMRES 1
+#define PG_PC_NONE 0
+#define PG_PC_JACOBI 1
+#define PG_PREC_FP64 0
+#define PG_PREC_MIXED 1
+
+typedef struct pg_solve_opts {
+  int32_t ksp;        /* PG_KSP_* */
+  int32_t pc;         /* PG_PC_* */
+  int32_t precision;  /* PG_PREC_* */
+  int32_t restart;    /* GMRES restart length */
+  int32_t maxit;      /* max total Krylov iterations */
+  int32_t check_every;/* iterations between host convergence checks */
+  double rtol;        /* on the true residual ||b - A x|| / ||b|| */
+  double inner_rtol;  /* mixed precision: reliable-update threshold */
+} pg_solve_opts;
+
+typedef struct pg_solve_info {
+  int32_t iterations;       /* Krylov iterations (inner iterations for mixed) */
+  int32_t refinements;      /* FP64 residual replacements (mixed) */
+  int32_t status;           /* PG_OK, PG_ERR_BREAKDOWN, or -1 = not converged */
+  int32_t pad;
+  double relres;            /* true FP64 ||b - A x|| / ||b|| at exit */
+} pg_solve_info;
+
+/* Create a solver for an n x n complex symmetric CSR matrix on the device
+ * (values complex128; a complex64 copy is made for PG_PREC_MIXED). */
+int pg_solver_create(int64_t n, int64_t nnz, const int32_t *rowptr, const int32_t *cols,
+                     const double *values, const pg_solve_opts *opts, void **solver,
+                     void *stream);
+/* Rebuild what the solver derives from the matrix values (Jacobi diagonal,
+ * complex64 copy) after the caller has reassembled them in place. */
+int pg_solver_refresh(void *solver, void *stream);
+/* Solve A x = b (x holds the initial guess on entry). */
+int pg_solver_solve(void *solver, const double *b, double *x, pg_solve_info *info, void *stream);
+int pg_solver_destroy(void *solver);
+/* Average duration (ms) of the solver's own SpMV kernel over `reps` launches
+ * on `stream`, timed with HIP events: complex128 (ms64) and, for mixed
+ * precision, complex64 (ms32); g64/g32 = lanes per row chosen at creation. */
+int pg_solver_profile(void *solver, int reps, double *ms64, double *ms32, int32_t *g64,
+                      int32_t *g32, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PETGEM_AMD_H */

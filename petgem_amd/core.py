@@ -1,0 +1,315 @@
+"""Device-resident CSEM system: mesh arrays, assembly, source, BC, solve, receivers.
+
+Thin host orchestration over the C ABI; every array that the hot path touches
+lives in HBM as a torch tensor on the current HIP device, and all arithmetic
+happens in the HIP kernels of ``libpetgem_amd.so``.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .mesh import dof_counts
+
+MU0 = 4.0 * math.pi * 1e-7
+
+
+def _dev(device):
+    return torch.device(device if device is not None else "cuda")
+
+
+def _i32(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(dev)
+
+
+def _f64(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+
+def omega_of(frequency):
+    """Angular frequency exactly as the reference forms it (``solver.py:174``)."""
+    return float(frequency) * 2.0 * math.pi
+
+
+def dipole_moment(source):
+    """I*L times the rotated unit vector (``solver.py:253-268``, ``hvfem.py:2303-2344``)."""
+    az = math.radians(source.get("azimuth", 0.0))
+    dip = math.radians(source.get("dip", 0.0))
+    a = az
+    b = dip
+    rot = np.array([math.cos(a) * math.cos(b), math.sin(a) * math.cos(b), math.sin(b)])
+    il = float(source.get("current", 1.0)) * float(source.get("length", 1.0))
+    return il * rot
+
+
+class DeviceMesh:
+    """Per-element arrays in the reference's scratch layout, resident on the device."""
+
+    def __init__(self, p, coords, elems_nodes, elems_edges, edges_nodes, faces_edges, dofs,
+                 n_edges, n_faces, elems_faces, device=None):
+        dev = _dev(device)
+        self.device = dev
+        self.p = int(p)
+        self.n_elems = int(np.asarray(elems_nodes).shape[0])
+        self.n_edges = int(n_edges)
+        self.n_faces = int(n_faces)
+        de, df, dv, n = dof_counts(self.p)
+        self.n_dof_elem = n
+        self.ndof = self.n_edges * de + self.n_faces * df + self.n_elems * dv
+        self.coords = _f64(np.asarray(coords).reshape(self.n_elems, 12), dev)
+        self.elems_nodes = _i32(elems_nodes, dev)
+        self.elems_edges = _i32(elems_edges, dev)
+        self.edges_nodes = _i32(np.asarray(edges_nodes).reshape(self.n_elems, 12), dev)
+        self.faces_edges = _i32(np.asarray(faces_edges).reshape(self.n_elems, 12), dev)
+        self.dofs = _i32(dofs, dev)
+        # host copies needed by the (host-side, integer) assembly plan
+        self.h_elems_edges = np.ascontiguousarray(elems_edges, dtype=np.int32)
+        self.h_elems_faces = np.ascontiguousarray(elems_faces, dtype=np.int32)
+
+    @classmethod
+    def from_topology(cls, topo, device=None):
+        return cls(topo.p, topo.coords, topo.elems_nodes, topo.elems_edges,
+                   topo.elem_edges_nodes, topo.elem_faces_edges, topo.dofs, topo.n_edges,
+                   topo.n_faces, topo.elems_faces, device)
+
+    def entity_mask(self, boundary_dofs):
+        """(per-entity, per-DOF) uint8 boundary masks from a list of boundary DOFs."""
+        de, df, dv, _ = dof_counts(self.p)
+        dof_bc = np.zeros(self.ndof, dtype=np.uint8)
+        dof_bc[np.asarray(boundary_dofs, dtype=np.int64)] = 1
+        ent = np.zeros(self.n_edges + self.n_faces + self.n_elems, dtype=np.uint8)
+        ent[: self.n_edges] = dof_bc[0: self.n_edges * de: de]
+        if df:
+            f0 = self.n_edges * de
+            ent[self.n_edges: self.n_edges + self.n_faces] = dof_bc[f0: f0 + self.n_faces * df: df]
+        return (torch.from_numpy(ent).to(self.device), torch.from_numpy(dof_bc).to(self.device))
+
+
+class Tables:
+    """Device-resident reference-element tables for order p."""
+
+    def __init__(self, p):
+        L = nat.lib()
+        h = ctypes.c_void_p()
+        nat.check(L.pg_tables_create(int(p), ctypes.byref(h)))
+        self.handle = h
+        self.p = p
+
+    def __del__(self):
+        try:
+            if self.handle:
+                nat.lib().pg_tables_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class Plan:
+    """Assembly plan (CSR pattern + per-block contribution lists) on the device."""
+
+    def __init__(self, mesh):
+        L = nat.lib()
+        h = ctypes.c_void_p()
+        nat.check(L.pg_plan_build(mesh.p, mesh.n_elems, mesh.n_edges, mesh.n_faces,
+                                  nat.ptr(mesh.h_elems_edges), nat.ptr(mesh.h_elems_faces),
+                                  ctypes.byref(h)))
+        try:
+            info = np.zeros(6, dtype=np.int64)
+            nat.check(L.pg_plan_info(h, nat.ptr(info)))
+            self.ndof, self.nnz, self.npairs, self.ncontrib, self.n_entities, self.maxd = (
+                int(v) for v in info)
+            rowptr = np.empty(self.ndof + 1, dtype=np.int32)
+            cols = np.empty(self.nnz, dtype=np.int32)
+            slot0 = np.empty(self.npairs, dtype=np.int32)
+            rowlen = np.empty(self.npairs, dtype=np.int32)
+            cbeg = np.empty(self.npairs + 1, dtype=np.int32)
+            ab = np.empty(2 * self.npairs, dtype=np.int32)
+            contrib = np.empty(self.ncontrib, dtype=np.uint32)
+            nat.check(L.pg_plan_export(h, nat.ptr(rowptr), nat.ptr(cols), nat.ptr(slot0),
+                                       nat.ptr(rowlen), nat.ptr(cbeg), nat.ptr(ab),
+                                       nat.ptr(contrib)))
+        finally:
+            L.pg_plan_destroy(h)
+        if self.ndof != mesh.ndof:
+            raise RuntimeError(f"plan ndof {self.ndof} != mesh ndof {mesh.ndof}")
+        dev = mesh.device
+        self.h_rowptr, self.h_cols = rowptr, cols
+        self.rowptr = torch.from_numpy(rowptr).to(dev)
+        self.cols = torch.from_numpy(cols).to(dev)
+        self.slot0 = torch.from_numpy(slot0).to(dev)
+        self.rowlen = torch.from_numpy(rowlen).to(dev)
+        self.cbeg = torch.from_numpy(cbeg).to(dev)
+        self.ab = torch.from_numpy(ab).to(dev)
+        self.contrib = torch.from_numpy(contrib.view(np.int32)).to(dev)
+        self.dev_struct = nat.PlanDev(
+            mesh.p, mesh.n_edges, mesh.n_faces, mesh.n_elems, self.ndof, self.nnz, self.npairs,
+            self.rowptr.data_ptr(), self.cols.data_ptr(), self.slot0.data_ptr(),
+            self.rowlen.data_ptr(), self.cbeg.data_ptr(), self.ab.data_ptr(),
+            self.contrib.data_ptr())
+
+
+class Assembler:
+    """Element geometry + fused atomic-free assembly for one mesh and order."""
+
+    def __init__(self, mesh, tables=None, plan=None):
+        self.mesh = mesh
+        self.tables = tables if tables is not None else Tables(mesh.p)
+        self.plan = plan if plan is not None else Plan(mesh)
+        nbytes = int(nat.lib().pg_element_workspace_bytes(mesh.n_elems))
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=mesh.device)
+        self._sigma = None
+
+    def geometry(self, sigma, stream=None):
+        """Fill the per-element workspace (Jacobian coefficients, sigma, orientation)."""
+        m = self.mesh
+        sig = sigma if torch.is_tensor(sigma) else _f64(sigma, m.device)
+        self._sigma = sig
+        nat.check(nat.lib().pg_element_geometry(
+            m.p, m.n_elems, nat.ptr(m.coords), nat.ptr(m.elems_nodes), nat.ptr(m.elems_edges),
+            nat.ptr(m.edges_nodes), nat.ptr(m.faces_edges), nat.ptr(sig), nat.ptr(self.workspace),
+            nat.stream_handle(stream)))
+
+    def element_matrices(self, stream=None):
+        m = self.mesh
+        n = m.n_dof_elem
+        M = torch.empty((m.n_elems, n, n), dtype=torch.float64, device=m.device)
+        K = torch.empty_like(M)
+        nat.check(nat.lib().pg_element_matrices(self.tables.handle, m.n_elems,
+                                                nat.ptr(self.workspace), nat.ptr(M), nat.ptr(K),
+                                                nat.stream_handle(stream)))
+        return M, K
+
+    def assemble(self, omega, ent_bc=None, out=None, stream=None):
+        """CSR values (nnz,2) float64 = complex128 interleaved."""
+        if out is None:
+            out = torch.empty((self.plan.nnz, 2), dtype=torch.float64, device=self.mesh.device)
+        nat.check(nat.lib().pg_assemble(self.tables.handle, ctypes.byref(self.plan.dev_struct),
+                                        nat.ptr(self.workspace), float(omega), nat.ptr(ent_bc),
+                                        nat.ptr(out), nat.stream_handle(stream)))
+        return out
+
+
+def apply_dirichlet(plan, values, dof_bc, b=None, stream=None):
+    nat.check(nat.lib().pg_apply_dirichlet(plan.ndof, nat.ptr(plan.rowptr), nat.ptr(plan.cols),
+                                           nat.ptr(dof_bc), nat.ptr(values), nat.ptr(b),
+                                           nat.stream_handle(stream)))
+
+
+def locate_points(mesh, points, tol=1e-12, stream=None):
+    pts = _f64(np.atleast_2d(points), mesh.device)
+    out = torch.empty(pts.shape[0], dtype=torch.int32, device=mesh.device)
+    nat.check(nat.lib().pg_locate_points(mesh.n_elems, nat.ptr(mesh.coords), pts.shape[0],
+                                         nat.ptr(pts), float(tol), nat.ptr(out),
+                                         nat.stream_handle(stream)))
+    return out
+
+
+def source_vector(mesh, element, position, moment, omega, b=None, stream=None):
+    """Add the point-dipole contribution to b (complex128 as (N,2) float64)."""
+    element = int(element)
+    if not 0 <= element < mesh.n_elems:
+        raise nat.PgError(nat.PG_ERR_NOT_FOUND,
+                          f"source element {element} not in the mesh (source outside the domain?)")
+    if b is None:
+        b = torch.zeros((mesh.ndof, 2), dtype=torch.float64, device=mesh.device)
+    elif tuple(b.shape) != (mesh.ndof, 2) or b.dtype != torch.float64:
+        raise ValueError("b must be a (ndof,2) float64 tensor")
+    pos = _f64(np.asarray(position, dtype=np.float64), mesh.device)
+    mom = _f64(np.asarray(moment, dtype=np.float64), mesh.device)
+    nat.check(nat.lib().pg_source_vector(mesh.p, int(element), nat.ptr(mesh.coords),
+                                         nat.ptr(mesh.elems_nodes), nat.ptr(mesh.elems_edges),
+                                         nat.ptr(mesh.edges_nodes), nat.ptr(mesh.faces_edges),
+                                         nat.ptr(mesh.dofs), nat.ptr(pos), nat.ptr(mom),
+                                         float(omega), nat.ptr(b), nat.stream_handle(stream)))
+    return b
+
+
+def receiver_fields(mesh, points, elements, x, omega, stream=None):
+    pts = _f64(np.atleast_2d(points), mesh.device)
+    el = elements if torch.is_tensor(elements) else _i32(elements, mesh.device)
+    el_h = el.detach().cpu().numpy()
+    if el_h.shape[0] != pts.shape[0] or (el_h >= mesh.n_elems).any() or el.dtype != torch.int32:
+        raise ValueError("receiver elements must be int32, one per point, < n_elems (-1 = none)")
+    if tuple(x.shape) != (mesh.ndof, 2) or x.dtype != torch.float64:
+        raise ValueError("x must be a (ndof,2) float64 tensor")
+    out = torch.empty((pts.shape[0], 6, 2), dtype=torch.float64, device=mesh.device)
+    nat.check(nat.lib().pg_receiver_fields(mesh.p, pts.shape[0], nat.ptr(pts), nat.ptr(el),
+                                           nat.ptr(mesh.coords), nat.ptr(mesh.elems_nodes),
+                                           nat.ptr(mesh.elems_edges), nat.ptr(mesh.edges_nodes),
+                                           nat.ptr(mesh.faces_edges), nat.ptr(mesh.dofs),
+                                           nat.ptr(x), float(omega), nat.ptr(out),
+                                           nat.stream_handle(stream)))
+    return out
+
+
+_KSP = {"cocg": nat.PG_KSP_COCG, "gmres": nat.PG_KSP_GMRES}
+_PC = {"none": nat.PG_PC_NONE, "jacobi": nat.PG_PC_JACOBI}
+_PREC = {"fp64": nat.PG_PREC_FP64, "mixed": nat.PG_PREC_MIXED}
+
+
+class KrylovSolver:
+    """Device Krylov solver for a complex symmetric CSR matrix (values complex128)."""
+
+    def __init__(self, plan, values, ksp="cocg", pc="jacobi", precision="fp64", rtol=1e-8,
+                 maxit=100000, check_every=25, inner_rtol=1e-3, restart=30, stream=None):
+        self.plan = plan
+        self.values = values
+        self.opts = nat.SolveOpts(_KSP[ksp], _PC[pc], _PREC[precision], int(restart), int(maxit),
+                                  int(check_every), float(rtol), float(inner_rtol))
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pg_solver_create(plan.ndof, plan.nnz, nat.ptr(plan.rowptr),
+                                             nat.ptr(plan.cols), nat.ptr(values),
+                                             ctypes.byref(self.opts), ctypes.byref(h),
+                                             nat.stream_handle(stream)))
+        self.handle = h
+
+    def refresh(self, stream=None):
+        """Re-derive the preconditioner (and complex64 copy) after reassembly in place."""
+        nat.check(nat.lib().pg_solver_refresh(self.handle, nat.stream_handle(stream)))
+
+    def profile(self, reps=20, stream=None):
+        """Average device time (ms) of this solver's SpMV kernel(s), HIP events on `stream`."""
+        ms64, ms32 = ctypes.c_double(), ctypes.c_double()
+        g64, g32 = ctypes.c_int32(), ctypes.c_int32()
+        nat.check(nat.lib().pg_solver_profile(self.handle, int(reps), ctypes.byref(ms64),
+                                              ctypes.byref(ms32), ctypes.byref(g64),
+                                              ctypes.byref(g32), nat.stream_handle(stream)))
+        return dict(ms64=ms64.value, ms32=ms32.value, g64=g64.value, g32=g32.value)
+
+    def solve(self, b, x=None, stream=None):
+        if x is None:
+            x = torch.zeros_like(b)
+        info = nat.SolveInfo()
+        nat.check(nat.lib().pg_solver_solve(self.handle, nat.ptr(b), nat.ptr(x),
+                                            ctypes.byref(info), nat.stream_handle(stream)))
+        return x, dict(iterations=info.iterations, refinements=info.refinements,
+                       status=info.status, relres=info.relres)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                nat.lib().pg_solver_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def spmv(plan, values, x, stream=None):
+    y = torch.empty_like(x)
+    nat.check(nat.lib().pg_spmv_z(plan.ndof, nat.ptr(plan.rowptr), nat.ptr(plan.cols),
+                                  nat.ptr(values), nat.ptr(x), nat.ptr(y),
+                                  nat.stream_handle(stream)))
+    return y
+
+
+def as_complex(t):
+    """(…,2) float64 tensor -> numpy complex128."""
+    a = t.detach().cpu().numpy()
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def from_complex(z, device):
+    z = np.asarray(z, dtype=np.complex128)
+    return torch.from_numpy(np.stack([z.real, z.imag], axis=-1).copy()).to(device)

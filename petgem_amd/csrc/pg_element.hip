@@ -1,0 +1,662 @@
+// Element geometry, element matrices, atomic-free CSR assembly, PEC boundary
+// condition, point location, point-dipole source and receiver interpolation
+// for gfx950.
+//
+// Element matrices use the reference-tensor form: with J the Jacobian
+// (rows x_k - x_0, petgem/hvfem.py:112-114), G = J^-T diag(sh,sh,sv) J^-1 and
+// H = J J^T / det J,
+//     M_ij = det J * sum_{a<=b} G'_ab T^M_ab[v_i][v_j]
+//     K_ij =         sum_{a<=b} H'_ab T^K_ab[v_i][v_j]
+// where v_i is the orientation "variant" of local DOF i and T are quadrature
+// contractions of the reference basis computed once per order (tables.cpp
+// part below).  This equals the per-quadrature-point loop of
+// computeElementalMatrices (petgem/hvfem.py:258-314) up to summation order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pg_basis.h"
+#include "pg_common.h"
+
+namespace pg {
+
+constexpr double kMu0 = 4.0 * 3.14159265358979323846 * 1e-7;  // petgem/solver.py:175
+
+// -------------------------------------------------------------------------
+// reference tables
+// -------------------------------------------------------------------------
+static void build_table(int p, std::vector<double>& T) {
+  const int V = n_variants(p);
+  T.assign((size_t)V * V * 12, 0.0);
+  std::vector<V3> S(V), C(V);
+  const int nq = quad_npoints(p);
+  for (int q = 0; q < nq; ++q) {
+    double X[3], w;
+    quad_point(p, q, X, &w);
+    double lam[4];
+    V3 dlam[4];
+    affine(X, lam, dlam);
+    for (int e = 0; e < 6; ++e)
+      for (int o = 0; o < 2; ++o) {
+        const int v = entity_variant_base(e, o, p);
+        edge_functions(lam, dlam, e, o, p, &S[v], &C[v]);
+      }
+    if (p >= 2)
+      for (int f = 0; f < 4; ++f)
+        for (int o = 0; o < 6; ++o) {
+          const int v = entity_variant_base(6 + f, o, p);
+          face_functions(lam, dlam, f, o, p, &S[v], &C[v]);
+        }
+    if (p >= 3) {
+      const int v = entity_variant_base(10, 0, p);
+      bubble_functions(lam, dlam, p, &S[v], &C[v]);
+    }
+    for (int a = 0; a < V; ++a)
+      for (int b = 0; b < V; ++b) {
+        double* t = &T[((size_t)a * V + b) * 12];
+        const V3 s = S[a], u = S[b], c = C[a], d = C[b];
+        t[0] += w * (s.x * u.x);
+        t[1] += w * (s.y * u.y);
+        t[2] += w * (s.z * u.z);
+        t[3] += w * (s.x * u.y + s.y * u.x);
+        t[4] += w * (s.x * u.z + s.z * u.x);
+        t[5] += w * (s.y * u.z + s.z * u.y);
+        t[6] += w * (c.x * d.x);
+        t[7] += w * (c.y * d.y);
+        t[8] += w * (c.z * d.z);
+        t[9] += w * (c.x * d.y + c.y * d.x);
+        t[10] += w * (c.x * d.z + c.z * d.x);
+        t[11] += w * (c.y * d.z + c.z * d.y);
+      }
+  }
+}
+
+struct Tables {
+  int p;
+  int V;
+  int device;
+  double* d_T;
+};
+
+// -------------------------------------------------------------------------
+// geometry + orientation (one thread per element)
+// -------------------------------------------------------------------------
+// workspace layout: coef[E][12] (det*G' then H'/det), then orient[E] (int32)
+__host__ __device__ inline double* ws_coef(void* ws) { return reinterpret_cast<double*>(ws); }
+__host__ __device__ inline int32_t* ws_orient(void* ws, int64_t E) {
+  return reinterpret_cast<int32_t*>(reinterpret_cast<double*>(ws) + 12 * E);
+}
+
+__device__ inline int element_orientation(const int32_t* en, const int32_t* ee, const int32_t* edn,
+                                          const int32_t* fe, int* eo, int* fo) {
+  int packed = 0;
+  for (int i = 0; i < 6; ++i) {
+    const int a = en[edge_vert(i, 0)], b = en[edge_vert(i, 1)];
+    const int flag = (a == edn[2 * i + 1] && b == edn[2 * i + 0]) ? 1 : 0;
+    eo[i] = flag;
+    packed |= flag << i;
+  }
+  for (int f = 0; f < 4; ++f) {
+    const int g0 = ee[face_edge(f, 0)], g1 = ee[face_edge(f, 1)];
+    int k1 = 0, k2 = 0;
+    for (int k = 0; k < 3; ++k) {
+      if (g0 == fe[3 * f + k]) k1 = k + 1;
+      if (g1 == fe[3 * f + k]) k2 = k + 1;
+    }
+    const int sel = k1 * 10 + k2;
+    int code = 0;
+    if (sel == 12) code = 0;
+    else if (sel == 31) code = 1;
+    else if (sel == 23) code = 2;
+    else if (sel == 32) code = 3;
+    else if (sel == 13) code = 4;
+    else if (sel == 21) code = 5;
+    fo[f] = code;
+    packed |= code << (6 + 3 * f);
+  }
+  return packed;
+}
+
+__device__ inline void jacobian(const double* c, double J[3][3], double iJ[3][3], double* det) {
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) J[r][k] = c[3 * (r + 1) + k] - c[k];
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double d = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double id = 1.0 / d;
+  iJ[0][0] = c00 * id;
+  iJ[1][0] = c01 * id;
+  iJ[2][0] = c02 * id;
+  iJ[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+  iJ[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+  iJ[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+  iJ[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+  iJ[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+  iJ[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+  *det = d;
+}
+
+__global__ void k_geometry(int64_t E, const double* __restrict__ coords,
+                           const int32_t* __restrict__ en, const int32_t* __restrict__ ee,
+                           const int32_t* __restrict__ edn, const int32_t* __restrict__ fe,
+                           const double* __restrict__ sigma, double* __restrict__ coef,
+                           int32_t* __restrict__ orient) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double J[3][3], iJ[3][3], det;
+  jacobian(coords + 12 * e, J, iJ, &det);
+  const double s[3] = {sigma[2 * e], sigma[2 * e], sigma[2 * e + 1]};
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+  double* out = coef + 12 * e;
+  const double idet = 1.0 / det;
+  for (int c = 0; c < 6; ++c) {
+    const int a = ia[c], b = ib[c];
+    double g = 0.0, h = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      g += s[k] * iJ[k][a] * iJ[k][b];
+      h += J[a][k] * J[b][k];
+    }
+    out[c] = det * g;
+    out[6 + c] = h * idet;
+  }
+  int eo[6], fo[4];
+  orient[e] = element_orientation(en + 4 * e, ee + 6 * e, edn + 12 * e, fe + 12 * e, eo, fo);
+}
+
+__global__ void k_orientation(int64_t E, const int32_t* __restrict__ en,
+                              const int32_t* __restrict__ ee, const int32_t* __restrict__ edn,
+                              const int32_t* __restrict__ fe, int32_t* __restrict__ eo_out,
+                              int32_t* __restrict__ fo_out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int eo[6], fo[4];
+  element_orientation(en + 4 * e, ee + 6 * e, edn + 12 * e, fe + 12 * e, eo, fo);
+  for (int i = 0; i < 6; ++i) eo_out[6 * e + i] = eo[i];
+  for (int i = 0; i < 4; ++i) fo_out[4 * e + i] = fo[i];
+}
+
+// -------------------------------------------------------------------------
+// element matrices (one thread per (element, i, j))
+// -------------------------------------------------------------------------
+template <int P>
+__device__ inline void dof_entity(int i, int& la, int& k) {
+  constexpr int ne = 6 * P, nf = P * (P - 1);
+  if (i < ne) {
+    la = i / P;
+    k = i % P;
+    return;
+  }
+  i -= ne;
+  if (nf > 0 && i < 4 * nf) {
+    la = 6 + i / (nf > 0 ? nf : 1);
+    k = i % (nf > 0 ? nf : 1);
+    return;
+  }
+  i -= 4 * nf;
+  la = 10;
+  k = i;
+}
+
+__device__ inline void contract(const double* __restrict__ t, const double* __restrict__ cf,
+                                double* m, double* kk) {
+  double a = 0.0, b = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    a += cf[c] * t[c];
+    b += cf[6 + c] * t[6 + c];
+  }
+  *m = a;
+  *kk = b;
+}
+
+template <int P>
+__global__ void k_element_matrices(int64_t E, const double* __restrict__ T,
+                                   const double* __restrict__ coef,
+                                   const int32_t* __restrict__ orient, double* __restrict__ M,
+                                   double* __restrict__ K) {
+  constexpr int n = P * (P + 2) * (P + 3) / 2;
+  constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * n * n) return;
+  const int64_t e = t / (n * n);
+  const int ij = (int)(t - e * n * n);
+  const int i = ij / n, j = ij % n;
+  const int pk = orient[e];
+  int la, ka, lb, kb;
+  dof_entity<P>(i, la, ka);
+  dof_entity<P>(j, lb, kb);
+  const int va = entity_variant_base(la, orient_of(pk, la), P) + ka;
+  const int vb = entity_variant_base(lb, orient_of(pk, lb), P) + kb;
+  double m, kk;
+  contract(T + ((size_t)va * V + vb) * 12, coef + 12 * e, &m, &kk);
+  M[t] = m;
+  K[t] = kk;
+}
+
+// -------------------------------------------------------------------------
+// fused assembly: one thread per (entity pair, row k of the pair's block)
+// -------------------------------------------------------------------------
+template <int P>
+__device__ inline int entity_ndof_global(int64_t g, int64_t nE, int64_t nF) {
+  if (g < nE) return P;
+  if (g < nE + nF) return P * (P - 1);
+  return P * (P - 1) * (P - 2) / 2;
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_assemble(
+    int64_t npairs, int64_t nE, int64_t nF, const double* __restrict__ T,
+    const int32_t* __restrict__ pair_slot0, const int32_t* __restrict__ pair_rowlen,
+    const int32_t* __restrict__ pair_cbeg, const int32_t* __restrict__ pair_ab,
+    const uint32_t* __restrict__ contrib, const double* __restrict__ coef,
+    const int32_t* __restrict__ orient, double omega_mu, const uint8_t* __restrict__ ent_bc,
+    double2* __restrict__ values) {
+  constexpr int MAXD = P == 1 ? 1 : P * (P - 1);
+  constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t pair = t / MAXD;
+  const int k = (int)(t - pair * MAXD);
+  if (pair >= npairs) return;
+  const int64_t A = pair_ab[2 * pair], B = pair_ab[2 * pair + 1];
+  const int dA = entity_ndof_global<P>(A, nE, nF);
+  const int dB = entity_ndof_global<P>(B, nE, nF);
+  if (k >= dA) return;
+  double re[MAXD], im[MAXD];
+#pragma unroll
+  for (int l = 0; l < MAXD; ++l) re[l] = im[l] = 0.0;
+  const int c0 = pair_cbeg[pair], c1 = pair_cbeg[pair + 1];
+  for (int c = c0; c < c1; ++c) {
+    const uint32_t u = contrib[c];
+    const int64_t e = u >> 8;
+    const int la = (u >> 4) & 15, lb = u & 15;
+    const int pk = orient[e];
+    const double* cf = coef + 12 * e;
+    const int va = entity_variant_base(la, orient_of(pk, la), P) + k;
+    const int vb = entity_variant_base(lb, orient_of(pk, lb), P);
+    const double* trow = T + ((size_t)va * V + vb) * 12;
+#pragma unroll
+    for (int l = 0; l < MAXD; ++l) {
+      if (l < dB) {
+        double m, kk;
+        contract(trow + 12 * l, cf, &m, &kk);
+        re[l] += kk;
+        im[l] += -(omega_mu * m);
+      }
+    }
+  }
+  if (ent_bc != nullptr && (ent_bc[A] | ent_bc[B])) {
+#pragma unroll
+    for (int l = 0; l < MAXD; ++l) {
+      re[l] = (A == B && k == l && ent_bc[A]) ? 1.0 : 0.0;
+      im[l] = 0.0;
+    }
+  }
+  const int64_t s0 = (int64_t)pair_slot0[pair] + (int64_t)k * pair_rowlen[pair];
+#pragma unroll
+  for (int l = 0; l < MAXD; ++l)
+    if (l < dB) values[s0 + l] = make_double2(re[l], im[l]);
+}
+
+// -------------------------------------------------------------------------
+// PEC boundary condition on an assembled CSR (one thread per row)
+// -------------------------------------------------------------------------
+__global__ void k_dirichlet(int64_t n, const int32_t* __restrict__ rowptr,
+                            const int32_t* __restrict__ cols, const uint8_t* __restrict__ bc,
+                            double2* __restrict__ vals, double2* __restrict__ b) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const bool br = bc[r] != 0;
+  for (int s = rowptr[r]; s < rowptr[r + 1]; ++s) {
+    const int c = cols[s];
+    if (br) vals[s] = make_double2(c == r ? 1.0 : 0.0, 0.0);
+    else if (bc[c]) vals[s] = make_double2(0.0, 0.0);
+  }
+  if (br && b != nullptr) b[r] = make_double2(0.0, 0.0);
+}
+
+// -------------------------------------------------------------------------
+// point location: lowest element index whose barycentric coordinates are
+// within [-tol, 1+tol]
+// -------------------------------------------------------------------------
+__device__ inline bool barycentric_inside(const double* c, const double* x, double tol) {
+  // columns of T are v_i - v_3 (i = 0..2); solve T lam = x - v_3
+  double T[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) T[k][i] = c[3 * i + k] - c[9 + k];
+  const double r[3] = {x[0] - c[9], x[1] - c[10], x[2] - c[11]};
+  const double c00 = T[1][1] * T[2][2] - T[1][2] * T[2][1];
+  const double c01 = T[1][2] * T[2][0] - T[1][0] * T[2][2];
+  const double c02 = T[1][0] * T[2][1] - T[1][1] * T[2][0];
+  const double d = T[0][0] * c00 + T[0][1] * c01 + T[0][2] * c02;
+  if (!(d != 0.0) || !isfinite(d)) return false;
+  const double id = 1.0 / d;
+  double inv[3][3];
+  inv[0][0] = c00 * id;
+  inv[1][0] = c01 * id;
+  inv[2][0] = c02 * id;
+  inv[0][1] = (T[0][2] * T[2][1] - T[0][1] * T[2][2]) * id;
+  inv[1][1] = (T[0][0] * T[2][2] - T[0][2] * T[2][0]) * id;
+  inv[2][1] = (T[0][1] * T[2][0] - T[0][0] * T[2][1]) * id;
+  inv[0][2] = (T[0][1] * T[1][2] - T[0][2] * T[1][1]) * id;
+  inv[1][2] = (T[0][2] * T[1][0] - T[0][0] * T[1][2]) * id;
+  inv[2][2] = (T[0][0] * T[1][1] - T[0][1] * T[1][0]) * id;
+  double last = 1.0;
+  for (int i = 0; i < 3; ++i) {
+    const double l = inv[i][0] * r[0] + inv[i][1] * r[1] + inv[i][2] * r[2];
+    if (!(l >= -tol && l <= 1.0 + tol)) return false;
+    last -= l;
+  }
+  return last >= -tol && last <= 1.0 + tol;
+}
+
+__global__ void k_locate(int64_t E, const double* __restrict__ coords, int64_t npts,
+                         const double* __restrict__ pts, double tol, int32_t* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const double* c = coords + 12 * e;
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = fmin(fmin(c[k], c[3 + k]), fmin(c[6 + k], c[9 + k]));
+    hi[k] = fmax(fmax(c[k], c[3 + k]), fmax(c[6 + k], c[9 + k]));
+  }
+  for (int64_t q = 0; q < npts; ++q) {
+    const double* x = pts + 3 * q;
+    const double sl = 1e-6 * (hi[0] - lo[0] + hi[1] - lo[1] + hi[2] - lo[2]);
+    if (x[0] < lo[0] - sl || x[0] > hi[0] + sl || x[1] < lo[1] - sl || x[1] > hi[1] + sl ||
+        x[2] < lo[2] - sl || x[2] > hi[2] + sl)
+      continue;
+    if (barycentric_inside(c, x, tol)) atomicMin(&out[q], (int32_t)e);
+  }
+}
+
+__global__ void k_fill_i32(int64_t n, int32_t* p, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_notfound(int64_t n, int32_t* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && p[i] == 0x7fffffff) p[i] = -1;
+}
+
+// -------------------------------------------------------------------------
+// basis at a physical point of one element (source and receivers)
+// -------------------------------------------------------------------------
+template <int P>
+__device__ inline void physical_basis(const double* c, const int32_t* en, const int32_t* ee,
+                                      const int32_t* edn, const int32_t* fe, const double* x,
+                                      V3* N, V3* Cu) {
+  constexpr int n = P * (P + 2) * (P + 3) / 2;
+  double J[3][3], iJ[3][3], det;
+  jacobian(c, J, iJ, &det);
+  // reference coordinates: xi = J^-T (x - x0)
+  const double r[3] = {x[0] - c[0], x[1] - c[1], x[2] - c[2]};
+  double X[3];
+  for (int a = 0; a < 3; ++a) X[a] = r[0] * iJ[0][a] + r[1] * iJ[1][a] + r[2] * iJ[2][a];
+  int eo[6], fo[4];
+  element_orientation(en, ee, edn, fe, eo, fo);
+  V3 S[n], C[n];
+  element_functions(X, eo, fo, P, S, C);
+  for (int i = 0; i < n; ++i) {
+    const V3 s = S[i], u = C[i];
+    N[i] = v3(iJ[0][0] * s.x + iJ[0][1] * s.y + iJ[0][2] * s.z,
+              iJ[1][0] * s.x + iJ[1][1] * s.y + iJ[1][2] * s.z,
+              iJ[2][0] * s.x + iJ[2][1] * s.y + iJ[2][2] * s.z);
+    Cu[i] = v3((J[0][0] * u.x + J[1][0] * u.y + J[2][0] * u.z) / det,
+               (J[0][1] * u.x + J[1][1] * u.y + J[2][1] * u.z) / det,
+               (J[0][2] * u.x + J[1][2] * u.y + J[2][2] * u.z) / det);
+  }
+}
+
+template <int P>
+__global__ void k_source(int32_t e, const double* __restrict__ coords,
+                         const int32_t* __restrict__ en, const int32_t* __restrict__ ee,
+                         const int32_t* __restrict__ edn, const int32_t* __restrict__ fe,
+                         const int32_t* __restrict__ dofs, const double* __restrict__ pos,
+                         const double* __restrict__ moment, double omega_mu,
+                         double2* __restrict__ b) {
+  constexpr int n = P * (P + 2) * (P + 3) / 2;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  V3 N[n], Cu[n];
+  const int64_t E = e;
+  physical_basis<P>(coords + 12 * E, en + 4 * E, ee + 6 * E, edn + 12 * E, fe + 12 * E, pos, N,
+                    Cu);
+  for (int i = 0; i < n; ++i) {
+    const double v = moment[0] * N[i].x + moment[1] * N[i].y + moment[2] * N[i].z;
+    const int g = dofs[E * n + i];
+    double2 bi = b[g];
+    bi.y += omega_mu * v;
+    b[g] = bi;
+  }
+}
+
+template <int P>
+__global__ void k_receivers(int64_t npts, const double* __restrict__ pts,
+                            const int32_t* __restrict__ elem, const double* __restrict__ coords,
+                            const int32_t* __restrict__ en, const int32_t* __restrict__ ee,
+                            const int32_t* __restrict__ edn, const int32_t* __restrict__ fe,
+                            const int32_t* __restrict__ dofs, const double2* __restrict__ x,
+                            double omega_mu, double2* __restrict__ out) {
+  constexpr int n = P * (P + 2) * (P + 3) / 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npts) return;
+  const int64_t E = elem[q];
+  double2* o = out + 6 * q;
+  if (E < 0) {
+    for (int k = 0; k < 6; ++k) o[k] = make_double2(0.0, 0.0);
+    return;
+  }
+  V3 N[n], Cu[n];
+  physical_basis<P>(coords + 12 * E, en + 4 * E, ee + 6 * E, edn + 12 * E, fe + 12 * E,
+                    pts + 3 * q, N, Cu);
+  double2 f[6];
+  for (int k = 0; k < 6; ++k) f[k] = make_double2(0.0, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const double2 xi = x[dofs[E * n + i]];
+    const double nv[3] = {N[i].x, N[i].y, N[i].z};
+    const double cv[3] = {Cu[i].x, Cu[i].y, Cu[i].z};
+    for (int k = 0; k < 3; ++k) {
+      f[k].x += xi.x * nv[k];
+      f[k].y += xi.y * nv[k];
+      f[3 + k].x += xi.x * cv[k];
+      f[3 + k].y += xi.y * cv[k];
+    }
+  }
+  // H = curl E / (i omega mu0): (a + ib) / (i w) = (b - i a) / w
+  for (int k = 0; k < 3; ++k) {
+    o[k] = f[k];
+    o[3 + k] = make_double2(f[3 + k].y / omega_mu, -f[3 + k].x / omega_mu);
+  }
+}
+
+static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace pg
+
+using namespace pg;
+
+// =========================================================================
+// C ABI
+// =========================================================================
+extern "C" {
+
+int pg_tables_host(int p, double* out, int64_t out_len) {
+  if (p < 1 || p > kMaxP) PG_FAIL(PG_ERR_UNSUPPORTED, "order p=%d not supported (1..3)", p);
+  std::vector<double> T;
+  build_table(p, T);
+  if (out_len < (int64_t)T.size()) PG_FAIL(PG_ERR_ARG, "table needs %zu doubles", T.size());
+  std::memcpy(out, T.data(), T.size() * sizeof(double));
+  return PG_OK;
+}
+
+int pg_tables_create(int p, void** tables) {
+  if (!tables) PG_FAIL(PG_ERR_ARG, "tables is NULL");
+  if (p < 1 || p > kMaxP) PG_FAIL(PG_ERR_UNSUPPORTED, "order p=%d not supported (1..3)", p);
+  std::vector<double> T;
+  build_table(p, T);
+  Tables* t = new Tables();
+  t->p = p;
+  t->V = n_variants(p);
+  hipGetDevice(&t->device);
+  hipError_t err = hipMalloc(&t->d_T, T.size() * sizeof(double));
+  if (err != hipSuccess) {
+    delete t;
+    PG_FAIL(PG_ERR_HIP, "hipMalloc(tables): %s", hipGetErrorString(err));
+  }
+  err = hipMemcpy(t->d_T, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    hipFree(t->d_T);
+    delete t;
+    PG_FAIL(PG_ERR_HIP, "hipMemcpy(tables): %s", hipGetErrorString(err));
+  }
+  *tables = t;
+  return PG_OK;
+}
+
+int pg_tables_destroy(void* tables) {
+  if (!tables) return PG_OK;
+  Tables* t = static_cast<Tables*>(tables);
+  hipFree(t->d_T);
+  delete t;
+  return PG_OK;
+}
+
+size_t pg_element_workspace_bytes(int64_t n_elems) {
+  return (size_t)n_elems * (12 * sizeof(double) + sizeof(int32_t));
+}
+
+int pg_element_geometry(int p, int64_t E, const double* coords, const int32_t* en,
+                        const int32_t* ee, const int32_t* edn, const int32_t* fe,
+                        const double* sigma, void* ws, void* stream) {
+  if (p < 1 || p > kMaxP) PG_FAIL(PG_ERR_UNSUPPORTED, "order p=%d not supported", p);
+  if (E <= 0) PG_FAIL(PG_ERR_ARG, "n_elems must be positive");
+  if (!coords || !en || !ee || !edn || !fe || !sigma || !ws) PG_FAIL(PG_ERR_ARG, "NULL input");
+  hipLaunchKernelGGL(k_geometry, dim3(nblocks(E, 256)), dim3(256), 0, as_stream(stream), E,
+                     coords, en, ee, edn, fe, sigma, ws_coef(ws), ws_orient(ws, E));
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_element_orientation(int64_t E, const int32_t* en, const int32_t* ee, const int32_t* edn,
+                           const int32_t* fe, int32_t* eo, int32_t* fo, void* stream) {
+  if (E <= 0 || !en || !ee || !edn || !fe || !eo || !fo) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  hipLaunchKernelGGL(k_orientation, dim3(nblocks(E, 256)), dim3(256), 0, as_stream(stream), E, en,
+                     ee, edn, fe, eo, fo);
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_element_matrices(const void* tables, int64_t E, const void* ws, double* M, double* K,
+                        void* stream) {
+  if (!tables || !ws || !M || !K || E <= 0) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  const Tables* t = static_cast<const Tables*>(tables);
+  const int n = ndof_element(t->p);
+  const int64_t total = E * n * n;
+  void* w = const_cast<void*>(ws);
+  const dim3 g(nblocks(total, 256)), b(256);
+  hipStream_t s = as_stream(stream);
+  switch (t->p) {
+    case 1: hipLaunchKernelGGL(k_element_matrices<1>, g, b, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), M, K); break;
+    case 2: hipLaunchKernelGGL(k_element_matrices<2>, g, b, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), M, K); break;
+    case 3: hipLaunchKernelGGL(k_element_matrices<3>, g, b, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), M, K); break;
+  }
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_assemble(const void* tables, const pg_plan_dev* pl, const void* ws, double omega,
+                const uint8_t* ent_bc, double* values, void* stream) {
+  if (!tables || !pl || !ws || !values) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  const Tables* t = static_cast<const Tables*>(tables);
+  if (pl->p != t->p) PG_FAIL(PG_ERR_ARG, "plan order %d != tables order %d", pl->p, t->p);
+  if (pl->n_elems >= (1LL << 24)) PG_FAIL(PG_ERR_UNSUPPORTED, "more than 2^24 elements");
+  const int maxd = t->p == 1 ? 1 : t->p * (t->p - 1);
+  const int64_t threads = pl->npairs * maxd;
+  void* w = const_cast<void*>(ws);
+  const double om = omega * kMu0;
+  const dim3 g(nblocks(threads, 256)), b(256);
+  hipStream_t s = as_stream(stream);
+  double2* v = reinterpret_cast<double2*>(values);
+#define PG_ASM(P)                                                                          \
+  hipLaunchKernelGGL(k_assemble<P>, g, b, 0, s, pl->npairs, pl->n_edges, pl->n_faces, t->d_T, \
+                     pl->pair_slot0, pl->pair_rowlen, pl->pair_cbeg, pl->pair_ab, pl->contrib, \
+                     ws_coef(w), ws_orient(w, pl->n_elems), om, ent_bc, v)
+  switch (t->p) {
+    case 1: PG_ASM(1); break;
+    case 2: PG_ASM(2); break;
+    case 3: PG_ASM(3); break;
+  }
+#undef PG_ASM
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_apply_dirichlet(int64_t n, const int32_t* rowptr, const int32_t* cols, const uint8_t* bc,
+                       double* values, double* b, void* stream) {
+  if (n <= 0 || !rowptr || !cols || !bc || !values) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  hipLaunchKernelGGL(k_dirichlet, dim3(nblocks(n, 256)), dim3(256), 0, as_stream(stream), n,
+                     rowptr, cols, bc, reinterpret_cast<double2*>(values),
+                     reinterpret_cast<double2*>(b));
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_locate_points(int64_t E, const double* coords, int64_t npts, const double* pts, double tol,
+                     int32_t* out, void* stream) {
+  if (E <= 0 || npts <= 0 || !coords || !pts || !out) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(npts, 256)), dim3(256), 0, s, npts, out,
+                     (int32_t)0x7fffffff);
+  hipLaunchKernelGGL(k_locate, dim3(nblocks(E, 256)), dim3(256), 0, s, E, coords, npts, pts, tol,
+                     out);
+  hipLaunchKernelGGL(k_notfound, dim3(nblocks(npts, 256)), dim3(256), 0, s, npts, out);
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_source_vector(int p, int32_t e, const double* coords, const int32_t* en, const int32_t* ee,
+                     const int32_t* edn, const int32_t* fe, const int32_t* dofs,
+                     const double* pos, const double* moment, double omega, double* b,
+                     void* stream) {
+  if (p < 1 || p > kMaxP) PG_FAIL(PG_ERR_UNSUPPORTED, "order p=%d not supported", p);
+  if (e < 0) PG_FAIL(PG_ERR_NOT_FOUND, "source element not found (outside the mesh)");
+  if (!coords || !en || !ee || !edn || !fe || !dofs || !pos || !moment || !b)
+    PG_FAIL(PG_ERR_ARG, "NULL input");
+  const double om = omega * kMu0;
+  hipStream_t s = as_stream(stream);
+  double2* bb = reinterpret_cast<double2*>(b);
+  switch (p) {
+    case 1: hipLaunchKernelGGL(k_source<1>, dim3(1), dim3(64), 0, s, e, coords, en, ee, edn, fe, dofs, pos, moment, om, bb); break;
+    case 2: hipLaunchKernelGGL(k_source<2>, dim3(1), dim3(64), 0, s, e, coords, en, ee, edn, fe, dofs, pos, moment, om, bb); break;
+    case 3: hipLaunchKernelGGL(k_source<3>, dim3(1), dim3(64), 0, s, e, coords, en, ee, edn, fe, dofs, pos, moment, om, bb); break;
+  }
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_receiver_fields(int p, int64_t npts, const double* pts, const int32_t* elem,
+                       const double* coords, const int32_t* en, const int32_t* ee,
+                       const int32_t* edn, const int32_t* fe, const int32_t* dofs,
+                       const double* x, double omega, double* fields, void* stream) {
+  if (p < 1 || p > kMaxP) PG_FAIL(PG_ERR_UNSUPPORTED, "order p=%d not supported", p);
+  if (npts <= 0 || !pts || !elem || !coords || !en || !ee || !edn || !fe || !dofs || !x ||
+      !fields)
+    PG_FAIL(PG_ERR_ARG, "bad arguments");
+  const double om = omega * kMu0;
+  hipStream_t s = as_stream(stream);
+  const dim3 g(nblocks(npts, 64)), b(64);
+  const double2* xx = reinterpret_cast<const double2*>(x);
+  double2* f = reinterpret_cast<double2*>(fields);
+  switch (p) {
+    case 1: hipLaunchKernelGGL(k_receivers<1>, g, b, 0, s, npts, pts, elem, coords, en, ee, edn, fe, dofs, xx, om, f); break;
+    case 2: hipLaunchKernelGGL(k_receivers<2>, g, b, 0, s, npts, pts, elem, coords, en, ee, edn, fe, dofs, xx, om, f); break;
+    case 3: hipLaunchKernelGGL(k_receivers<3>, g, b, 0, s, npts, pts, elem, coords, en, ee, edn, fe, dofs, xx, om, f); break;
+  }
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,148 @@
+"""The C-ABI library on the CPU: it loads, exports every declared entry point,
+and its host-only parts (assembly plan, reference tables, argument errors)
+match the oracle exactly.  No GPU work is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from petgem_amd import _native as nat
+from petgem_amd.mesh import Topology
+from petgem_amd.synthetic import Workload
+from oracle import basis as B, fem, quadrature as Q
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "petgem_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = nat.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), s
+        assert s in nat.SIGNATURES, f"{s} has no ctypes signature"
+    assert L.pg_abi_version() == 1
+
+
+def _plan(T):
+    L = nat.lib()
+    h = ctypes.c_void_p()
+    ee = np.ascontiguousarray(T.elems_edges, np.int32)
+    ef = np.ascontiguousarray(T.elems_faces, np.int32)
+    nat.check(L.pg_plan_build(T.p, T.n_elems, T.n_edges, T.n_faces, nat.ptr(ee), nat.ptr(ef),
+                              ctypes.byref(h)))
+    info = np.zeros(6, np.int64)
+    nat.check(L.pg_plan_info(h, nat.ptr(info)))
+    nd, nnz, npairs, nc = (int(v) for v in info[:4])
+    out = dict(rowptr=np.empty(nd + 1, np.int32), cols=np.empty(nnz, np.int32),
+               slot0=np.empty(npairs, np.int32), rowlen=np.empty(npairs, np.int32),
+               cbeg=np.empty(npairs + 1, np.int32), ab=np.empty(2 * npairs, np.int32),
+               contrib=np.empty(nc, np.uint32))
+    nat.check(L.pg_plan_export(h, *[nat.ptr(out[k]) for k in
+                                    ("rowptr", "cols", "slot0", "rowlen", "cbeg", "ab", "contrib")]))
+    L.pg_plan_destroy(h)
+    return info, out
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_plan_pattern_bit_exact(test_mesh, p):
+    pts, en, _ = test_mesh
+    T = Topology(pts, en, p)
+    info, pl = _plan(T)
+    indptr, cols = fem.csr_pattern(T.dofs, T.ndof)
+    assert info[0] == T.ndof
+    assert np.array_equal(pl["rowptr"], indptr) and np.array_equal(pl["cols"], cols)
+
+
+def test_plan_contributions_cover_every_element_entry():
+    """Every (element, i, j) lands in exactly one slot, listed in ascending element order."""
+    w = Workload(3, 2)
+    T = w.topo
+    info, pl = _plan(T)
+    n = T.n_dof_elem
+    p = T.p
+    ab = pl["ab"].reshape(-1, 2)
+    count = np.zeros(pl["cols"].size, dtype=np.int64)
+    loc_off = lambda la: la * p if la < 6 else 6 * p + (la - 6) * p * (p - 1)  # noqa: E731
+    loc_n = lambda la: p if la < 6 else p * (p - 1)  # noqa: E731
+    for k in range(ab.shape[0]):
+        c0, c1 = pl["cbeg"][k], pl["cbeg"][k + 1]
+        es = pl["contrib"][c0:c1] >> 8
+        assert np.all(np.diff(es.astype(np.int64)) > 0)
+        for u in pl["contrib"][c0:c1]:
+            e, la, lb = int(u >> 8), int((u >> 4) & 15), int(u & 15)
+            for a in range(loc_n(la)):
+                for b in range(loc_n(lb)):
+                    r, c = T.dofs[e, loc_off(la) + a], T.dofs[e, loc_off(lb) + b]
+                    s = pl["slot0"][k] + a * pl["rowlen"][k] + b
+                    assert pl["cols"][s] == c and pl["rowptr"][r] <= s < pl["rowptr"][r + 1]
+                    count[s] += 1
+    assert count.sum() == T.n_elems * n * n
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_reference_tables_match_oracle_contraction(p):
+    V = B.ndof_element(p) and (12 * p + 24 * p * (p - 1) + p * (p - 1) * (p - 2) // 2)
+    T = np.zeros(V * V * 12)
+    nat.check(nat.lib().pg_tables_host(p, nat.ptr(T), T.size))
+    T = T.reshape(V, V, 12)
+    pts, w = Q.rule_for_order(p)
+    S = np.zeros((len(w), V, 3))
+    C = np.zeros_like(S)
+    for e in range(6):
+        for o in range(2):
+            v = (e * 2 + o) * p
+            S[:, v:v + p], C[:, v:v + p] = B.edge_functions(pts, e, o, p)
+    if p >= 2:
+        for f in range(4):
+            for o in range(6):
+                v = 12 * p + (f * 6 + o) * p * (p - 1)
+                S[:, v:v + p * (p - 1)], C[:, v:v + p * (p - 1)] = B.face_functions(pts, f, o, p)
+    if p >= 3:
+        S[:, 12 * p + 24 * p * (p - 1):], C[:, 12 * p + 24 * p * (p - 1):] = B.bubble_functions(pts, p)
+    R = np.einsum("q,qia,qjb->ijab", w, S, S)
+    RK = np.einsum("q,qia,qjb->ijab", w, C, C)
+    combos = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    for c, (a, b) in enumerate(combos):
+        refm = R[..., a, b] if a == b else R[..., a, b] + R[..., b, a]
+        refk = RK[..., a, b] if a == b else RK[..., a, b] + RK[..., b, a]
+        assert np.abs(T[..., c] - refm).max() <= 1e-14
+        assert np.abs(T[..., 6 + c] - refk).max() <= 1e-14
+
+
+def test_errors_are_reported():
+    L = nat.lib()
+    h = ctypes.c_void_p()
+    ee = np.zeros((1, 6), np.int32)
+    rc = L.pg_plan_build(4, 1, 6, 4, nat.ptr(ee), nat.ptr(ee), ctypes.byref(h))
+    assert rc == nat.PG_ERR_UNSUPPORTED and b"order" in L.pg_last_error()
+    with pytest.raises(nat.PgError):
+        nat.check(L.pg_plan_build(2, 1, 6, 4, nat.ptr(np.full((1, 6), 9, np.int32)),
+                                  nat.ptr(np.zeros((1, 4), np.int32)), ctypes.byref(h)))
+
+
+def test_config_sizes_match_baseline_table():
+    """Exact sizes of the benchmark workloads (N, nnz, boundary DOFs)."""
+    for n, p, ndof, nnz, nb in [(20, 1, 59660, 938060, 7200), (40, 2, 2480240, 106797280, 96000),
+                                (30, 2, 1053180, 45133560, 54000)]:
+        w = Workload(n, p)
+        info, _ = (None, None)
+        L = nat.lib()
+        h = ctypes.c_void_p()
+        T = w.topo
+        ee = np.ascontiguousarray(T.elems_edges, np.int32)
+        ef = np.ascontiguousarray(T.elems_faces, np.int32)
+        nat.check(L.pg_plan_build(p, T.n_elems, T.n_edges, T.n_faces, nat.ptr(ee), nat.ptr(ef),
+                                  ctypes.byref(h)))
+        inf = np.zeros(6, np.int64)
+        L.pg_plan_info(h, nat.ptr(inf))
+        L.pg_plan_destroy(h)
+        assert (T.ndof, int(inf[1]), T.boundary_dofs.size) == (ndof, nnz, nb)
