@@ -116,8 +116,16 @@ typedef struct pg_plan_dev {
 int pg_assemble(const void *tables, const pg_plan_dev *plan, const void *workspace,
                 double omega, const uint8_t *entity_bc, double *values, void *stream);
 
+/* Stiffness and mass parts separately, K (nnz f64) and M (nnz f64), so that
+ * A(omega) = K - i*omega*mu0*M can be formed for any frequency (batched
+ * multi-frequency solves).  Same order of summation as pg_assemble;
+ * entity_bc folds the PEC rows/columns in (K: 0 / unit diagonal, M: 0). */
+int pg_assemble_km(const void *tables, const pg_plan_dev *plan, const void *workspace,
+                   const uint8_t *entity_bc, double *K, double *M, void *stream);
+
 /* PEC boundary condition on an assembled system: rows and columns of DOFs
- * with dof_bc != 0 zeroed, unit diagonal, b = 0 there (b nullable).
+ * with dof_bc != 0 zeroed, unit diagonal, b = 0 there (values or b may be
+ * NULL, not both).
  * Replaces A.zeroRowsColumns + b.setValues (petgem/solver.py:552-577). */
 int pg_apply_dirichlet(int64_t ndof, const int32_t *rowptr, const int32_t *cols,
                        const uint8_t *dof_bc, double *values, double *b, void *stream);
@@ -150,6 +158,23 @@ int pg_receiver_fields(int p, int64_t n_points, const double *points, const int3
                        const int32_t *elems_edges, const int32_t *edges_nodes,
                        const int32_t *faces_edges, const int32_t *dofs, const double *x,
                        double omega, double *fields, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Adjoint gradient of a receiver misfit with respect to per-element
+ * log-conductivity (no counterpart in the reference; BASELINE extension).
+ * ---------------------------------------------------------------------- */
+/* Adjoint source f[dofs_e] += sum_c coef[r][c] N_c(x_r) for located receivers
+ * (coef: (n_points,3) complex128 = weight * conj(E_pred - E_obs)). */
+int pg_adjoint_rhs(int p, int64_t n_points, const double *points, const int32_t *element,
+                   const double *coords, const int32_t *elems_nodes, const int32_t *elems_edges,
+                   const int32_t *edges_nodes, const int32_t *faces_edges, const int32_t *dofs,
+                   const double *coef, double *f, void *stream);
+/* g[e] += sum_r -omega_r*mu0 * Im(lam_e^T M_e(sigma) x_e), with M_e
+ * recomputed from the tables and a filled geometry workspace; x, lam are
+ * (nrhs, ndof) complex128, omega (nrhs) host array. */
+int pg_adjoint_gradient(const void *tables, int64_t n_elems, const void *workspace,
+                        const int32_t *dofs, int nrhs, int64_t ndof, const double *x,
+                        const double *lam, const double *omega, double *g, void *stream);
 
 /* ------------------------------------------------------------------------
  * Sparse kernels and Krylov solve.  Replace MatMult and KSPSolve
@@ -206,6 +231,23 @@ int pg_solver_destroy(void *solver);
  * precision, complex64 (ms32); g64/g32 = lanes per row chosen at creation. */
 int pg_solver_profile(void *solver, int reps, double *ms64, double *ms32, int32_t *g64,
                       int32_t *g32, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Batched multi-RHS solve on one mesh (many sources and frequencies in one
+ * launch; BASELINE extension, no reference counterpart -- the reference
+ * creates one KSP per right-hand side, petgem/solver.py:584-590).
+ * A_r = K - i*omega_r*mu0*M from pg_assemble_km; vectors are row-major
+ * (ndof, nrhs) complex128.  nrhs: power of two < 64, or 64/128/192/256.
+ * ---------------------------------------------------------------------- */
+int pg_batch_create(int64_t n, int64_t nnz, const int32_t *rowptr, const int32_t *cols,
+                    const double *K, const double *M, int nrhs, const double *omega,
+                    double rtol, int maxit, void **batch, void *stream);
+/* Independent COCG + Jacobi per column, true-residual check at the end;
+ * per-column iterations, relres and status (0 ok, -1 not converged,
+ * PG_ERR_BREAKDOWN) into host arrays (each nullable). */
+int pg_batch_solve(void *batch, const double *b, double *x, int32_t *iterations, double *relres,
+                   int32_t *status, void *stream);
+int pg_batch_destroy(void *batch);
 
 #ifdef __cplusplus
 }

@@ -53,12 +53,17 @@ SIGNATURES = {
     "pg_plan_export": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "pg_plan_destroy": (ctypes.c_int, [_P]),
     "pg_assemble": (ctypes.c_int, [_P, ctypes.POINTER(PlanDev), _P, _D, _P, _P, _P]),
+    "pg_assemble_km": (ctypes.c_int, [_P, ctypes.POINTER(PlanDev), _P, _P, _P, _P, _P]),
     "pg_apply_dirichlet": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     "pg_locate_points": (ctypes.c_int, [_I64, _P, _I64, _P, _D, _P, _P]),
     "pg_source_vector": (ctypes.c_int, [ctypes.c_int, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _D,
                                         _P, _P]),
     "pg_receiver_fields": (ctypes.c_int, [ctypes.c_int, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                           _D, _P, _P]),
+    "pg_adjoint_rhs": (ctypes.c_int, [ctypes.c_int, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                      _P]),
+    "pg_adjoint_gradient": (ctypes.c_int, [_P, _I64, _P, _P, ctypes.c_int, _I64, _P, _P, _P, _P,
+                                           _P]),
     "pg_spmv_z": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     "pg_spmv_c": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     "pg_demote_z2c": (ctypes.c_int, [_I64, _P, _P, _P]),
@@ -67,6 +72,10 @@ SIGNATURES = {
     "pg_solver_solve": (ctypes.c_int, [_P, _P, _P, ctypes.POINTER(SolveInfo), _P]),
     "pg_solver_destroy": (ctypes.c_int, [_P]),
     "pg_solver_refresh": (ctypes.c_int, [_P, _P]),
+    "pg_batch_create": (ctypes.c_int, [_I64, _I64, _P, _P, _P, _P, ctypes.c_int, _P, _D,
+                                       ctypes.c_int, ctypes.POINTER(_P), _P]),
+    "pg_batch_solve": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "pg_batch_destroy": (ctypes.c_int, [_P]),
     "pg_solver_profile": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P]),
 }
 

@@ -1,0 +1,167 @@
+"""Adjoint gradient of a receiver misfit w.r.t. per-element log-conductivity.
+
+No counterpart in the reference (an extension of its forward path).  For
+sources s and frequencies f, with A_f x_sf = b_sf the PEC-constrained system
+of the reference's discretisation and E_sf = R x_sf the receiver fields
+(``postprocessing.py:566-614``), the misfit is
+
+    Phi = 1/2 sum_{s,f,r,c} w_{sfrc} |E_sfrc - E^obs_sfrc|^2,  w = 1/|E^obs|^2,
+
+and, since A_f = A_f^T (complex symmetric), one adjoint solve per (s,f) with
+the same operator gives
+
+    lam_sf = A_f^-1 R^T (w conj(E_sf - E^obs_sf)),
+    dPhi/dlog(sigma_e) = sum_{s,f} -omega_f mu0 Im(lam_e^T M_e(sigma) x_e),
+
+because dA_e/dlog(sigma_e) = -i omega mu0 M_e(sigma) (M_e is linear in
+sigma; horizontal and vertical conductivity scale together).  The element
+mass matrices are recomputed on the device inside the contraction kernel,
+never stored.  Boundary rows/columns are not differentiated (the PEC rows
+are fixed), which the device code realises by zeroing the adjoint source on
+boundary DOFs.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from . import core
+
+
+class InversionProblem:
+    """Forward fields, misfit and adjoint gradient for many sources and frequencies."""
+
+    def __init__(self, topo, source_positions, frequencies, receivers, observed=None,
+                 rtol=1e-10, precision="fp64", moment=(1.0, 0.0, 0.0), device=None):
+        self.topo = topo
+        self.mesh = core.DeviceMesh.from_topology(topo, device)
+        self.asm = core.Assembler(self.mesh)
+        self.ent_bc, self.dof_bc = self.mesh.entity_mask(topo.boundary_dofs)
+        self.sources = np.atleast_2d(np.asarray(source_positions, dtype=np.float64))
+        self.freqs = [float(f) for f in frequencies]
+        self.rx = np.atleast_2d(np.asarray(receivers, dtype=np.float64))
+        self.moment = np.asarray(moment, dtype=np.float64)
+        self.rtol, self.precision = rtol, precision
+        self.src_el = core.locate_points(self.mesh, self.sources).cpu().numpy()
+        if (self.src_el < 0).any():
+            raise ValueError("a source lies outside the mesh")
+        self.rx_el = core.locate_points(self.mesh, self.rx)
+        if (self.rx_el.cpu().numpy() < 0).any():
+            raise ValueError("a receiver lies outside the mesh")
+        self.rx_dev = torch.from_numpy(self.rx).to(self.mesh.device)
+        nnz = self.asm.plan.nnz
+        self.values = [torch.empty((nnz, 2), dtype=torch.float64, device=self.mesh.device)
+                       for _ in self.freqs]
+        self.solvers = [None] * len(self.freqs)
+        self.observed = None if observed is None else np.asarray(observed)
+        self.x = None
+        self.stats = dict(solves=0, iterations=0)
+
+    # ---------------------------------------------------------------- forward
+    def _set_model(self, log_sigma):
+        ls = log_sigma.detach() if torch.is_tensor(log_sigma) else torch.from_numpy(
+            np.asarray(log_sigma, dtype=np.float64))
+        sig = torch.exp(ls.to(self.mesh.device, torch.float64))
+        self.asm.geometry(torch.stack([sig, sig], dim=1).contiguous())
+        for fi, f in enumerate(self.freqs):
+            self.asm.assemble(core.omega_of(f), ent_bc=self.ent_bc, out=self.values[fi])
+            if self.solvers[fi] is None:
+                self.solvers[fi] = core.KrylovSolver(self.asm.plan, self.values[fi],
+                                                     precision=self.precision, rtol=self.rtol)
+            else:
+                self.solvers[fi].refresh()
+
+    def _solve(self, fi, b):
+        x, info = self.solvers[fi].solve(b)
+        if info["status"] != 0:
+            raise RuntimeError(f"Krylov solve failed: {info}")
+        self.stats["solves"] += 1
+        self.stats["iterations"] += info["iterations"]
+        return x
+
+    def _mask(self, v):
+        nat.check(nat.lib().pg_apply_dirichlet(self.mesh.ndof, nat.ptr(self.asm.plan.rowptr),
+                                               nat.ptr(self.asm.plan.cols), nat.ptr(self.dof_bc),
+                                               None, nat.ptr(v), nat.stream_handle()))
+
+    def forward(self, log_sigma):
+        """Receiver E fields (S, F, R, 3) complex128 and the solutions kept on the device."""
+        self._set_model(log_sigma)
+        S, F = self.sources.shape[0], len(self.freqs)
+        self.x = [[None] * F for _ in range(S)]
+        out = np.zeros((S, F, self.rx.shape[0], 3), dtype=np.complex128)
+        for fi, f in enumerate(self.freqs):
+            om = core.omega_of(f)
+            for si in range(S):
+                b = core.source_vector(self.mesh, int(self.src_el[si]), self.sources[si],
+                                       self.moment, om)
+                self._mask(b)
+                x = self._solve(fi, b)
+                self.x[si][fi] = x
+                fld = core.receiver_fields(self.mesh, self.rx, self.rx_el, x, om)
+                out[si, fi] = core.as_complex(fld)[:, 0:3]
+        return out
+
+    def misfit(self, fields):
+        d = fields - self.observed
+        w = 1.0 / np.maximum(np.abs(self.observed) ** 2, 1e-300)
+        return 0.5 * float(np.sum(w * np.abs(d) ** 2))
+
+    # --------------------------------------------------------------- gradient
+    def misfit_and_gradient(self, log_sigma):
+        if self.observed is None:
+            raise ValueError("observed data not set")
+        fields = self.forward(log_sigma)
+        phi = self.misfit(fields)
+        m = self.mesh
+        g = torch.zeros(m.n_elems, dtype=torch.float64, device=m.device)
+        w = 1.0 / np.maximum(np.abs(self.observed) ** 2, 1e-300)
+        L = nat.lib()
+        for fi, f in enumerate(self.freqs):
+            om = core.omega_of(f)
+            for si in range(self.sources.shape[0]):
+                coef = w[si, fi] * np.conj(fields[si, fi] - self.observed[si, fi])
+                cdev = core.from_complex(coef, m.device)
+                rhs = torch.zeros((m.ndof, 2), dtype=torch.float64, device=m.device)
+                nat.check(L.pg_adjoint_rhs(m.p, self.rx.shape[0], nat.ptr(self.rx_dev),
+                                           nat.ptr(self.rx_el), nat.ptr(m.coords),
+                                           nat.ptr(m.elems_nodes), nat.ptr(m.elems_edges),
+                                           nat.ptr(m.edges_nodes), nat.ptr(m.faces_edges),
+                                           nat.ptr(m.dofs), nat.ptr(cdev), nat.ptr(rhs),
+                                           nat.stream_handle()))
+                self._mask(rhs)
+                lam = self._solve(fi, rhs)
+                omega = np.array([om], dtype=np.float64)
+                nat.check(L.pg_adjoint_gradient(self.asm.tables.handle, m.n_elems,
+                                                nat.ptr(self.asm.workspace), nat.ptr(m.dofs), 1,
+                                                m.ndof, nat.ptr(self.x[si][fi]), nat.ptr(lam),
+                                                nat.ptr(omega), nat.ptr(g), nat.stream_handle()))
+        return phi, g
+
+
+class LogConductivityMisfit(torch.autograd.Function):
+    """Phi(log sigma) with the adjoint gradient as its backward."""
+
+    @staticmethod
+    def forward(ctx, log_sigma, problem):
+        phi, g = problem.misfit_and_gradient(log_sigma)
+        ctx.save_for_backward(g)
+        return torch.tensor(phi, dtype=torch.float64, device=log_sigma.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return grad_out * g.to(grad_out.device), None
+
+
+def misfit(log_sigma, problem):
+    return LogConductivityMisfit.apply(log_sigma, problem)
+
+
+def synthetic_observations(problem, log_sigma):
+    """Fields of a model, to be used as observed data."""
+    return problem.forward(log_sigma)
+
+
+MU0 = 4.0 * math.pi * 1e-7

@@ -191,6 +191,52 @@ class Assembler:
         return out
 
 
+def assemble_km(assembler, ent_bc=None, stream=None):
+    """Stiffness and mass parts of the assembled system, each (nnz,) float64."""
+    pl = assembler.plan
+    K = torch.empty(pl.nnz, dtype=torch.float64, device=assembler.mesh.device)
+    M = torch.empty_like(K)
+    nat.check(nat.lib().pg_assemble_km(assembler.tables.handle, ctypes.byref(pl.dev_struct),
+                                       nat.ptr(assembler.workspace), nat.ptr(ent_bc), nat.ptr(K),
+                                       nat.ptr(M), nat.stream_handle(stream)))
+    return K, M
+
+
+class BatchSolver:
+    """Many right-hand sides (each with its own frequency) in one batched COCG."""
+
+    def __init__(self, plan, K, M, frequencies, rtol=1e-8, maxit=100000, stream=None):
+        self.plan, self.K, self.M = plan, K, M
+        self.omega = np.array([omega_of(f) for f in frequencies], dtype=np.float64)
+        self.nrhs = self.omega.size
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pg_batch_create(plan.ndof, plan.nnz, nat.ptr(plan.rowptr),
+                                            nat.ptr(plan.cols), nat.ptr(K), nat.ptr(M),
+                                            self.nrhs, nat.ptr(self.omega), float(rtol), int(maxit),
+                                            ctypes.byref(h), nat.stream_handle(stream)))
+        self.handle = h
+
+    def solve(self, B, stream=None):
+        """B: (ndof, nrhs, 2) float64 (complex interleaved, row-major); returns X, info."""
+        if tuple(B.shape) != (self.plan.ndof, self.nrhs, 2) or not B.is_contiguous():
+            raise ValueError("B must be a contiguous (ndof, nrhs, 2) float64 tensor")
+        X = torch.empty_like(B)
+        it = np.zeros(self.nrhs, dtype=np.int32)
+        rel = np.zeros(self.nrhs, dtype=np.float64)
+        st = np.zeros(self.nrhs, dtype=np.int32)
+        nat.check(nat.lib().pg_batch_solve(self.handle, nat.ptr(B), nat.ptr(X), nat.ptr(it),
+                                           nat.ptr(rel), nat.ptr(st), nat.stream_handle(stream)))
+        return X, dict(iterations=it, relres=rel, status=st)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                nat.lib().pg_batch_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 def apply_dirichlet(plan, values, dof_bc, b=None, stream=None):
     nat.check(nat.lib().pg_apply_dirichlet(plan.ndof, nat.ptr(plan.rowptr), nat.ptr(plan.cols),
                                            nat.ptr(dof_bc), nat.ptr(values), nat.ptr(b),
@@ -253,7 +299,7 @@ class KrylovSolver:
     """Device Krylov solver for a complex symmetric CSR matrix (values complex128)."""
 
     def __init__(self, plan, values, ksp="cocg", pc="jacobi", precision="fp64", rtol=1e-8,
-                 maxit=100000, check_every=25, inner_rtol=1e-3, restart=30, stream=None):
+                 maxit=100000, check_every=25, inner_rtol=1e-4, restart=30, stream=None):
         self.plan = plan
         self.values = values
         self.opts = nat.SolveOpts(_KSP[ksp], _PC[pc], _PREC[precision], int(restart), int(maxit),

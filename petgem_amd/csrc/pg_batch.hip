@@ -1,0 +1,606 @@
+// Batched multi-RHS solve for gfx950: many sources and frequencies on one mesh.
+//
+// The matrix is stored once as its stiffness and mass parts (K, M: one real
+// value each per non-zero), and every right-hand side r carries its own
+// frequency: A_r = K - i*omega_r*mu0*M is formed on the fly inside the SpMM,
+// so one pass over the matrix serves every column.  Vectors are laid out
+// row-major [N][R] (the R values of a row are contiguous), which makes the
+// gathers x[col][0..R) coalesced.  Each column runs its own COCG recurrence
+// (its own alpha, beta, rho, convergence flag) with Jacobi preconditioning on
+// its own diagonal K_ii - i*omega_r*mu0*M_ii; all columns share each launch.
+// Reductions are per-workgroup partials per column, finished in fixed order
+// by the last workgroup (bitwise reproducible).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pg_common.h"
+
+namespace pg {
+namespace batch {
+
+constexpr int kBS = 256;
+constexpr int kMaxGrid = 1024;
+constexpr int kMaxR = 1024;
+
+struct Col {  // per-column scalars
+  double2 rho, mu, alpha, beta;
+  double res2, bnorm2;
+  int32_t iter, done, status, pad;
+};
+
+struct Ctl {
+  double tol2;
+  int32_t maxit, ncols, all_done, pad;
+  uint32_t counter[4];
+};
+
+__device__ inline double2 cmul(double2 a, double2 b) { return zmul(a, b); }
+
+// chunk of rows for this workgroup (XCD-contiguous renumbering)
+__device__ inline void rows_of_block(int64_t n, int64_t* lo, int64_t* hi) {
+  const int64_t b = blockIdx.x, nb = gridDim.x;
+  const int64_t lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int64_t per = (n + nb - 1) / nb;
+  const int64_t a = lb * per, c = (lb + 1) * per;
+  *lo = a < n ? a : n;
+  *hi = c < n ? c : n;
+}
+
+// Threads are arranged (rsub, c): c = column lane (0..CT), rsub = row lane.
+// Column r handled by lane c is r = c + t*CT, t < R/CT.
+// Per-column partial sums (NV doubles each) are reduced over rsub in shared
+// memory and written to part[(k*R + r)*grid + block]; the last block sums
+// them in fixed order into tot (returned through `fin`).
+template <int CT, int NV, int TMAX>
+__device__ inline bool column_partials(int R, double (&v)[TMAX][NV], double* part,
+                                       uint32_t* counter) {
+  constexpr int RPB = kBS / CT;
+  __shared__ double sm[RPB][CT][NV];
+  __shared__ int am_last;
+  const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
+  const int T = R / CT;
+  for (int t = 0; t < T; ++t) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sm[rs][c][k] = v[t][k];
+    __syncthreads();
+    if (rs == 0) {
+      for (int k = 0; k < NV; ++k) {
+        double acc = 0.0;
+        for (int i = 0; i < RPB; ++i) acc += sm[i][c][k];
+        part[((int64_t)k * R + (c + t * CT)) * gridDim.x + blockIdx.x] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    am_last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!am_last) return false;
+  __threadfence();
+  return true;
+}
+
+// fixed-order total of one column's partials (called by one thread per (k, r))
+__device__ inline double column_total(const double* part, int k, int R, int r) {
+  double acc = 0.0;
+  const int64_t base = ((int64_t)k * R + r) * gridDim.x;
+  for (int64_t b = 0; b < gridDim.x; ++b)
+    acc += __hip_atomic_load(part + base + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return acc;
+}
+
+// q = A_r p for every column; mu_r = p_r^T q_r; alpha_r = rho_r / mu_r
+template <int CT, int TMAX>
+__global__ void __launch_bounds__(kBS) k_spmm(Ctl* ctl, Col* col, int64_t n, int R,
+                                              const int32_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ cols,
+                                              const double* __restrict__ K,
+                                              const double* __restrict__ M,
+                                              const double* __restrict__ omu,
+                                              const double2* __restrict__ P,
+                                              double2* __restrict__ Q, double* part) {
+  if (ctl->all_done) return;
+  constexpr int RPB = kBS / CT;
+  const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
+  const int T = R / CT;
+  int64_t lo, hi;
+  rows_of_block(n, &lo, &hi);
+  double v[TMAX][2];
+  double om[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    v[t][0] = v[t][1] = 0.0;
+    om[t] = t < T ? omu[c + t * CT] : 0.0;
+  }
+  for (int64_t row = lo + rs; row < hi; row += RPB) {
+    double2 acc[TMAX];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t) acc[t] = make_double2(0.0, 0.0);
+    const int s0 = rowptr[row], s1 = rowptr[row + 1];
+    for (int k = s0; k < s1; ++k) {
+      const double kv = K[k], mv = M[k];
+      const int64_t cj = (int64_t)cols[k] * R;
+#pragma unroll
+      for (int t = 0; t < TMAX; ++t)
+        if (t < T) {
+          const double2 x = P[cj + c + t * CT];
+          const double ai = -om[t] * mv;
+          acc[t].x += kv * x.x - ai * x.y;
+          acc[t].y += kv * x.y + ai * x.x;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int64_t idx = row * R + c + t * CT;
+        Q[idx] = acc[t];
+        const double2 pv = P[idx];
+        v[t][0] += pv.x * acc[t].x - pv.y * acc[t].y;
+        v[t][1] += pv.x * acc[t].y + pv.y * acc[t].x;
+      }
+  }
+  if (!column_partials<CT, 2, TMAX>(R, v, part, &ctl->counter[0])) return;
+  for (int r = threadIdx.x; r < R; r += kBS) {
+    Col& cc = col[r];
+    const double2 mu = make_double2(column_total(part, 0, R, r), column_total(part, 1, R, r));
+    cc.mu = mu;
+    if (cc.done) {
+      cc.alpha = make_double2(0.0, 0.0);
+      continue;
+    }
+    const double2 al = zdiv(cc.rho, mu);
+    if (!(mu.x != 0.0 || mu.y != 0.0) || !isfinite(al.x) || !isfinite(al.y)) {
+      cc.status = PG_ERR_BREAKDOWN;
+      cc.done = 1;
+      cc.alpha = make_double2(0.0, 0.0);
+    } else {
+      cc.alpha = al;
+    }
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(&ctl->counter[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// x += alpha p; r -= alpha q; z = r / diag_r; rho = r^T z; |r|^2 (per column)
+template <int CT, int TMAX>
+__global__ void __launch_bounds__(kBS) k_update(Ctl* ctl, Col* col, int64_t n, int R,
+                                                const double* __restrict__ dK,
+                                                const double* __restrict__ dM,
+                                                const double* __restrict__ omu,
+                                                double2* __restrict__ X, double2* __restrict__ Rr,
+                                                const double2* __restrict__ P,
+                                                const double2* __restrict__ Q,
+                                                double2* __restrict__ Z, double* part) {
+  if (ctl->all_done) return;
+  constexpr int RPB = kBS / CT;
+  const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
+  const int T = R / CT;
+  int64_t lo, hi;
+  rows_of_block(n, &lo, &hi);
+  double v[TMAX][3];
+  double2 al[TMAX];
+  double om[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    v[t][0] = v[t][1] = v[t][2] = 0.0;
+    al[t] = t < T ? col[c + t * CT].alpha : make_double2(0.0, 0.0);
+    om[t] = t < T ? omu[c + t * CT] : 0.0;
+  }
+  for (int64_t row = lo + rs; row < hi; row += RPB) {
+    const double kd = dK[row], md = dM[row];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int64_t idx = row * R + c + t * CT;
+        const double2 pv = P[idx], qv = Q[idx];
+        double2 xv = X[idx], rv = Rr[idx];
+        xv.x += al[t].x * pv.x - al[t].y * pv.y;
+        xv.y += al[t].x * pv.y + al[t].y * pv.x;
+        rv.x -= al[t].x * qv.x - al[t].y * qv.y;
+        rv.y -= al[t].x * qv.y + al[t].y * qv.x;
+        X[idx] = xv;
+        Rr[idx] = rv;
+        const double2 zv = zdiv(rv, make_double2(kd, -om[t] * md));
+        Z[idx] = zv;
+        v[t][0] += rv.x * zv.x - rv.y * zv.y;
+        v[t][1] += rv.x * zv.y + rv.y * zv.x;
+        v[t][2] += rv.x * rv.x + rv.y * rv.y;
+      }
+  }
+  if (!column_partials<CT, 3, TMAX>(R, v, part, &ctl->counter[1])) return;
+  __shared__ int ndone;
+  if (threadIdx.x == 0) ndone = 0;
+  __syncthreads();
+  for (int r = threadIdx.x; r < R; r += kBS) {
+    Col& cc = col[r];
+    const double2 rho_new = make_double2(column_total(part, 0, R, r), column_total(part, 1, R, r));
+    const double res2 = column_total(part, 2, R, r);
+    if (!cc.done) {
+      const double2 beta = zdiv(rho_new, cc.rho);
+      cc.beta = beta;
+      cc.rho = rho_new;
+      cc.res2 = res2;
+      cc.iter += 1;
+      if (!isfinite(res2) || !isfinite(beta.x) || !isfinite(beta.y)) {
+        cc.status = PG_ERR_BREAKDOWN;
+        cc.done = 1;
+      } else if (res2 <= ctl->tol2 * cc.bnorm2 || cc.iter >= ctl->maxit) {
+        cc.done = 1;
+      }
+    } else {
+      cc.beta = make_double2(0.0, 0.0);
+    }
+    if (cc.done) atomicAdd(&ndone, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ctl->all_done = (ndone == R) ? 1 : 0;
+    __hip_atomic_store(&ctl->counter[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// p = z + beta p (done columns keep p; their alpha is 0)
+template <int CT, int TMAX>
+__global__ void __launch_bounds__(kBS) k_pupdate(Ctl* ctl, const Col* col, int64_t n, int R,
+                                                 const double2* __restrict__ Z,
+                                                 double2* __restrict__ P) {
+  if (ctl->all_done) return;
+  constexpr int RPB = kBS / CT;
+  const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
+  const int T = R / CT;
+  int64_t lo, hi;
+  rows_of_block(n, &lo, &hi);
+  double2 be[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) be[t] = t < T ? col[c + t * CT].beta : make_double2(0.0, 0.0);
+  for (int64_t row = lo + rs; row < hi; row += RPB)
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int64_t idx = row * R + c + t * CT;
+        const double2 pv = P[idx], zv = Z[idx];
+        P[idx] = make_double2(zv.x + be[t].x * pv.x - be[t].y * pv.y,
+                              zv.y + be[t].x * pv.y + be[t].y * pv.x);
+      }
+}
+
+// init: r = b - A x (x = 0 -> r = b), z = r/diag, p = z; rho, |r|^2, |b|^2 per column
+template <int CT, int TMAX>
+__global__ void __launch_bounds__(kBS) k_init(Ctl* ctl, Col* col, int64_t n, int R,
+                                              const double* __restrict__ dK,
+                                              const double* __restrict__ dM,
+                                              const double* __restrict__ omu,
+                                              const double2* __restrict__ B,
+                                              double2* __restrict__ X, double2* __restrict__ Rr,
+                                              double2* __restrict__ Z, double2* __restrict__ P,
+                                              double* part) {
+  constexpr int RPB = kBS / CT;
+  const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
+  const int T = R / CT;
+  int64_t lo, hi;
+  rows_of_block(n, &lo, &hi);
+  double v[TMAX][3];
+  double om[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    v[t][0] = v[t][1] = v[t][2] = 0.0;
+    om[t] = t < T ? omu[c + t * CT] : 0.0;
+  }
+  for (int64_t row = lo + rs; row < hi; row += RPB) {
+    const double kd = dK[row], md = dM[row];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int64_t idx = row * R + c + t * CT;
+        const double2 bv = B[idx];
+        X[idx] = make_double2(0.0, 0.0);
+        Rr[idx] = bv;
+        const double2 zv = zdiv(bv, make_double2(kd, -om[t] * md));
+        Z[idx] = zv;
+        P[idx] = make_double2(0.0, 0.0);
+        v[t][0] += bv.x * zv.x - bv.y * zv.y;
+        v[t][1] += bv.x * zv.y + bv.y * zv.x;
+        v[t][2] += bv.x * bv.x + bv.y * bv.y;
+      }
+  }
+  if (!column_partials<CT, 3, TMAX>(R, v, part, &ctl->counter[2])) return;
+  for (int r = threadIdx.x; r < R; r += kBS) {
+    Col& cc = col[r];
+    cc.rho = make_double2(column_total(part, 0, R, r), column_total(part, 1, R, r));
+    cc.res2 = column_total(part, 2, R, r);
+    cc.bnorm2 = cc.res2;
+    cc.beta = make_double2(0.0, 0.0);
+    cc.alpha = make_double2(0.0, 0.0);
+    cc.iter = 0;
+    cc.status = 0;
+    cc.done = (cc.bnorm2 == 0.0) ? 1 : 0;
+  }
+  if (threadIdx.x == 0) {
+    ctl->all_done = 0;
+    __hip_atomic_store(&ctl->counter[2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// true residual per column: |b - A x|^2 into col[r].res2 (after the solve)
+template <int CT, int TMAX>
+__global__ void __launch_bounds__(kBS) k_true_res(Ctl* ctl, Col* col, int64_t n, int R,
+                                                  const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ cols,
+                                                  const double* __restrict__ K,
+                                                  const double* __restrict__ M,
+                                                  const double* __restrict__ omu,
+                                                  const double2* __restrict__ X,
+                                                  const double2* __restrict__ B, double* part) {
+  constexpr int RPB = kBS / CT;
+  const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
+  const int T = R / CT;
+  int64_t lo, hi;
+  rows_of_block(n, &lo, &hi);
+  double v[TMAX][1];
+  double om[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    v[t][0] = 0.0;
+    om[t] = t < T ? omu[c + t * CT] : 0.0;
+  }
+  for (int64_t row = lo + rs; row < hi; row += RPB) {
+    double2 acc[TMAX];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t) acc[t] = make_double2(0.0, 0.0);
+    for (int k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+      const double kv = K[k], mv = M[k];
+      const int64_t cj = (int64_t)cols[k] * R;
+#pragma unroll
+      for (int t = 0; t < TMAX; ++t)
+        if (t < T) {
+          const double2 x = X[cj + c + t * CT];
+          const double ai = -om[t] * mv;
+          acc[t].x += kv * x.x - ai * x.y;
+          acc[t].y += kv * x.y + ai * x.x;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const double2 bv = B[row * R + c + t * CT];
+        const double rx = bv.x - acc[t].x, ry = bv.y - acc[t].y;
+        v[t][0] += rx * rx + ry * ry;
+      }
+  }
+  if (!column_partials<CT, 1, TMAX>(R, v, part, &ctl->counter[3])) return;
+  for (int r = threadIdx.x; r < R; r += kBS) col[r].res2 = column_total(part, 0, R, r);
+  if (threadIdx.x == 0) __hip_atomic_store(&ctl->counter[3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_diag(int64_t n, const int32_t* __restrict__ rowptr,
+                       const int32_t* __restrict__ cols, const double* __restrict__ K,
+                       const double* __restrict__ M, double* __restrict__ dK,
+                       double* __restrict__ dM, int32_t* bad) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int lo = rowptr[r], hi = rowptr[r + 1] - 1;
+  double kd = 0.0, md = 0.0;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = cols[mid];
+    if (c == r) {
+      kd = K[mid];
+      md = M[mid];
+      break;
+    }
+    if (c < r) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  if (kd == 0.0 && md == 0.0) {
+    bad[0] = 1;
+    kd = 1.0;
+  }
+  dK[r] = kd;
+  dM[r] = md;
+}
+
+__global__ void k_ctl_init(Ctl* ctl, int maxit, double tol2, int R) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Ctl c{};
+  c.tol2 = tol2;
+  c.maxit = maxit;
+  c.ncols = R;
+  *ctl = c;
+}
+
+struct Batch {
+  int64_t n = 0, nnz = 0;
+  int R = 0, CT = 64, TM = 1;
+  int grid = 1;
+  const int32_t *rowptr = nullptr, *cols = nullptr;
+  const double *K = nullptr, *M = nullptr;
+  double *omu = nullptr, *dK = nullptr, *dM = nullptr, *part = nullptr;
+  double2 *Rr = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;
+  Ctl* ctl = nullptr;
+  Col* col = nullptr;
+  Ctl* hctl = nullptr;
+  Col* hcol = nullptr;
+  int32_t* bad = nullptr;
+  double rtol = 1e-8;
+  int maxit = 100000, check = 25;
+  std::vector<void*> owned;
+  ~Batch() {
+    for (void* v : owned) hipFree(v);
+    if (hctl) hipHostFree(hctl);
+    if (hcol) hipHostFree(hcol);
+  }
+  template <typename T>
+  int alloc(T** p, size_t count) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) PG_FAIL(PG_ERR_HIP, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+    owned.push_back(v);
+    *p = static_cast<T*>(v);
+    return PG_OK;
+  }
+};
+
+template <int CT, int TM>
+static void launch_iter(Batch* B, hipStream_t st, double2* X) {
+  const dim3 g(B->grid), b(kBS);
+  hipLaunchKernelGGL((k_pupdate<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->Z, B->P);
+  hipLaunchKernelGGL((k_spmm<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->rowptr,
+                     B->cols, B->K, B->M, B->omu, B->P, B->Q, B->part);
+  hipLaunchKernelGGL((k_update<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK, B->dM,
+                     B->omu, X, B->Rr, B->P, B->Q, B->Z, B->part);
+}
+
+template <int CT, int TM>
+static void launch_init(Batch* B, hipStream_t st, const double2* Bv, double2* X) {
+  hipLaunchKernelGGL((k_init<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->ctl, B->col, B->n,
+                     B->R, B->dK, B->dM, B->omu, Bv, X, B->Rr, B->Z, B->P, B->part);
+}
+
+template <int CT, int TM>
+static void launch_true(Batch* B, hipStream_t st, const double2* Bv, const double2* X) {
+  hipLaunchKernelGGL((k_true_res<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->ctl, B->col, B->n,
+                     B->R, B->rowptr, B->cols, B->K, B->M, B->omu, X, Bv, B->part);
+}
+
+// (CT, TM) configurations: R = CT * T with T <= TM
+#define PG_BATCH_DISPATCH(FN, ...)                                \
+  do {                                                            \
+    if (B->CT == 64 && B->TM == 4) FN<64, 4>(__VA_ARGS__);        \
+    else if (B->CT == 64 && B->TM == 1) FN<64, 1>(__VA_ARGS__);   \
+    else if (B->CT == 32) FN<32, 1>(__VA_ARGS__);                 \
+    else if (B->CT == 16) FN<16, 1>(__VA_ARGS__);                 \
+    else if (B->CT == 8) FN<8, 1>(__VA_ARGS__);                   \
+    else if (B->CT == 4) FN<4, 1>(__VA_ARGS__);                   \
+    else if (B->CT == 2) FN<2, 1>(__VA_ARGS__);                   \
+    else FN<1, 1>(__VA_ARGS__);                                   \
+  } while (0)
+
+}  // namespace batch
+}  // namespace pg
+
+using namespace pg;
+using namespace pg::batch;
+
+extern "C" {
+
+int pg_batch_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t* cols,
+                    const double* K, const double* M, int nrhs, const double* omega,
+                    double rtol, int maxit, void** out, void* stream) {
+  if (!out || n <= 0 || nnz <= 0 || !rowptr || !cols || !K || !M || !omega || nrhs <= 0)
+    PG_FAIL(PG_ERR_ARG, "bad arguments");
+  const int R = nrhs;
+  int CT, TM;
+  if (R >= 64) {
+    if (R % 64 != 0 || R > 256)
+      PG_FAIL(PG_ERR_ARG, "nrhs >= 64 must be a multiple of 64 and <= 256 (got %d)", R);
+    CT = 64;
+    TM = R == 64 ? 1 : 4;
+  } else {
+    if (R & (R - 1)) PG_FAIL(PG_ERR_ARG, "nrhs < 64 must be a power of two (got %d)", R);
+    CT = R;
+    TM = 1;
+  }
+  hipStream_t st = as_stream(stream);
+  Batch* B = new Batch();
+  B->n = n;
+  B->nnz = nnz;
+  B->R = R;
+  B->CT = CT;
+  B->TM = TM;
+  B->rowptr = rowptr;
+  B->cols = cols;
+  B->K = K;
+  B->M = M;
+  B->rtol = rtol > 0 ? rtol : 1e-8;
+  B->maxit = maxit > 0 ? maxit : 100000;
+  const int64_t rpb = kBS / CT;
+  B->grid = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, (n + rpb - 1) / rpb));
+  if (B->grid >= 64) B->grid = (B->grid / 8) * 8;
+  int rc;
+#define PG_BA(p, c)                                \
+  if ((rc = B->alloc(&(p), (c))) != PG_OK) {       \
+    delete B;                                      \
+    return rc;                                     \
+  }
+  PG_BA(B->omu, R);
+  PG_BA(B->dK, n);
+  PG_BA(B->dM, n);
+  PG_BA(B->part, (size_t)3 * R * B->grid);
+  PG_BA(B->Rr, (size_t)n * R);
+  PG_BA(B->P, (size_t)n * R);
+  PG_BA(B->Q, (size_t)n * R);
+  PG_BA(B->Z, (size_t)n * R);
+  PG_BA(B->ctl, 1);
+  PG_BA(B->col, R);
+  PG_BA(B->bad, 1);
+#undef PG_BA
+  if (hipHostMalloc(reinterpret_cast<void**>(&B->hctl), sizeof(Ctl), 0) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&B->hcol), R * sizeof(Col), 0) != hipSuccess) {
+    delete B;
+    PG_FAIL(PG_ERR_HIP, "hipHostMalloc failed");
+  }
+  std::vector<double> om(R);
+  for (int r = 0; r < R; ++r) om[r] = omega[r] * 4.0 * 3.14159265358979323846 * 1e-7;
+  hipMemcpy(B->omu, om.data(), R * sizeof(double), hipMemcpyHostToDevice);
+  hipMemsetAsync(B->bad, 0, sizeof(int32_t), st);
+  hipLaunchKernelGGL(k_diag, dim3((n + kBS - 1) / kBS), dim3(kBS), 0, st, n, rowptr, cols, K, M,
+                     B->dK, B->dM, B->bad);
+  int32_t bad = 0;
+  hipMemcpyAsync(&B->hctl->maxit, B->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    delete B;
+    PG_FAIL(PG_ERR_HIP, "batch setup failed");
+  }
+  bad = B->hctl->maxit;
+  if (bad) {
+    delete B;
+    PG_FAIL(PG_ERR_ARG, "zero diagonal entry: Jacobi preconditioner undefined");
+  }
+  *out = B;
+  return PG_OK;
+}
+
+int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations, double* relres,
+                   int32_t* status, void* stream) {
+  if (!batch || !b || !x) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  Batch* B = static_cast<Batch*>(batch);
+  hipStream_t st = as_stream(stream);
+  const double2* Bv = reinterpret_cast<const double2*>(b);
+  double2* X = reinterpret_cast<double2*>(x);
+  hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, B->ctl, B->maxit, B->rtol * B->rtol,
+                     B->R);
+  PG_BATCH_DISPATCH(launch_init, B, st, Bv, X);
+  PG_CHECK_LAUNCH();
+  while (true) {
+    for (int k = 0; k < B->check; ++k) PG_BATCH_DISPATCH(launch_iter, B, st, X);
+    PG_CHECK_LAUNCH();
+    PG_HIP(hipMemcpyAsync(B->hctl, B->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    if (B->hctl->all_done) break;
+  }
+  PG_BATCH_DISPATCH(launch_true, B, st, Bv, X);
+  PG_CHECK_LAUNCH();
+  PG_HIP(hipMemcpyAsync(B->hcol, B->col, B->R * sizeof(Col), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  for (int r = 0; r < B->R; ++r) {
+    const Col& c = B->hcol[r];
+    const double rel = c.bnorm2 > 0 ? std::sqrt(c.res2 / c.bnorm2) : 0.0;
+    if (iterations) iterations[r] = c.iter;
+    if (relres) relres[r] = rel;
+    if (status) status[r] = c.status != 0 ? c.status : (rel <= B->rtol * 1.0000001 ? 0 : -1);
+  }
+  return PG_OK;
+}
+
+int pg_batch_destroy(void* batch) {
+  delete static_cast<Batch*>(batch);
+  return PG_OK;
+}
+
+}  // extern "C"

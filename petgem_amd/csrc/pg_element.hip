@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(256) k_assemble(
     const int32_t* __restrict__ pair_cbeg, const int32_t* __restrict__ pair_ab,
     const uint32_t* __restrict__ contrib, const double* __restrict__ coef,
     const int32_t* __restrict__ orient, double omega_mu, const uint8_t* __restrict__ ent_bc,
-    double2* __restrict__ values) {
+    double2* __restrict__ values, double* __restrict__ Kout, double* __restrict__ Mout) {
   constexpr int MAXD = P == 1 ? 1 : P * (P - 1);
   constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -284,7 +284,8 @@ __global__ void __launch_bounds__(256) k_assemble(
         double m, kk;
         contract(trow + 12 * l, cf, &m, &kk);
         re[l] += kk;
-        im[l] += -(omega_mu * m);
+        // separately rounded product, as the reference forms i*w*mu0*M_e before adding
+        im[l] += Kout ? m : -__dmul_rn(omega_mu, m);
       }
     }
   }
@@ -296,9 +297,18 @@ __global__ void __launch_bounds__(256) k_assemble(
     }
   }
   const int64_t s0 = (int64_t)pair_slot0[pair] + (int64_t)k * pair_rowlen[pair];
+  if (Kout) {
 #pragma unroll
-  for (int l = 0; l < MAXD; ++l)
-    if (l < dB) values[s0 + l] = make_double2(re[l], im[l]);
+    for (int l = 0; l < MAXD; ++l)
+      if (l < dB) {
+        Kout[s0 + l] = re[l];
+        Mout[s0 + l] = im[l];
+      }
+  } else {
+#pragma unroll
+    for (int l = 0; l < MAXD; ++l)
+      if (l < dB) values[s0 + l] = make_double2(re[l], im[l]);
+  }
 }
 
 // -------------------------------------------------------------------------
@@ -310,6 +320,7 @@ __global__ void k_dirichlet(int64_t n, const int32_t* __restrict__ rowptr,
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const bool br = bc[r] != 0;
+  if (vals != nullptr)
   for (int s = rowptr[r]; s < rowptr[r + 1]; ++s) {
     const int c = cols[s];
     if (br) vals[s] = make_double2(c == r ? 1.0 : 0.0, 0.0);
@@ -472,6 +483,82 @@ __global__ void k_receivers(int64_t npts, const double* __restrict__ pts,
   }
 }
 
+
+// -------------------------------------------------------------------------
+// adjoint: receiver adjoint source and per-element gradient contraction
+// -------------------------------------------------------------------------
+// f[dofs_e(r)] += sum_c coef[r][c] * N_c(x_r) for every located receiver r,
+// accumulated by one thread in receiver order (deterministic).
+template <int P>
+__global__ void k_adjoint_rhs(int64_t npts, const double* __restrict__ pts,
+                              const int32_t* __restrict__ elem, const double* __restrict__ coords,
+                              const int32_t* __restrict__ en, const int32_t* __restrict__ ee,
+                              const int32_t* __restrict__ edn, const int32_t* __restrict__ fe,
+                              const int32_t* __restrict__ dofs, const double2* __restrict__ coef,
+                              double2* __restrict__ f) {
+  constexpr int n = P * (P + 2) * (P + 3) / 2;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int64_t q = 0; q < npts; ++q) {
+    const int64_t E = elem[q];
+    if (E < 0) continue;
+    V3 N[n], Cu[n];
+    physical_basis<P>(coords + 12 * E, en + 4 * E, ee + 6 * E, edn + 12 * E, fe + 12 * E,
+                      pts + 3 * q, N, Cu);
+    const double2 c0 = coef[3 * q], c1 = coef[3 * q + 1], c2 = coef[3 * q + 2];
+    for (int i = 0; i < n; ++i) {
+      const int g = dofs[E * n + i];
+      double2 v = f[g];
+      v.x += c0.x * N[i].x + c1.x * N[i].y + c2.x * N[i].z;
+      v.y += c0.y * N[i].x + c1.y * N[i].y + c2.y * N[i].z;
+      f[g] = v;
+    }
+  }
+}
+
+// g[e] += sum_r -omega_r*mu0 * Im(lam_e^T M_e x_e), M_e recomputed from the tables.
+// One thread per element; x and lam are (nrhs, N) complex, column r at r*N.
+template <int P>
+__global__ void __launch_bounds__(256) k_adjoint_grad(
+    int64_t E, const double* __restrict__ T, const double* __restrict__ coef,
+    const int32_t* __restrict__ orient, const int32_t* __restrict__ dofs, int nrhs, int64_t N,
+    const double2* __restrict__ x, const double2* __restrict__ lam,
+    const double* __restrict__ omega_mu, double* __restrict__ g) {
+  constexpr int n = P * (P + 2) * (P + 3) / 2;
+  constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int pk = orient[e];
+  const double* cf = coef + 12 * e;
+  const int32_t* de = dofs + e * n;
+  double acc_g = 0.0;
+  for (int r = 0; r < nrhs; ++r) {
+    const double2* xr = x + (int64_t)r * N;
+    const double2* lr = lam + (int64_t)r * N;
+    double sre = 0.0, sim = 0.0;
+    for (int i = 0; i < n; ++i) {
+      int la, ka;
+      dof_entity<P>(i, la, ka);
+      const int va = entity_variant_base(la, orient_of(pk, la), P) + ka;
+      double are = 0.0, aim = 0.0;
+      for (int j = 0; j < n; ++j) {
+        int lb, kb;
+        dof_entity<P>(j, lb, kb);
+        const int vb = entity_variant_base(lb, orient_of(pk, lb), P) + kb;
+        double m, kk;
+        contract(T + ((size_t)va * V + vb) * 12, cf, &m, &kk);
+        const double2 xj = xr[de[j]];
+        are += m * xj.x;
+        aim += m * xj.y;
+      }
+      const double2 li = lr[de[i]];
+      sre += li.x * are - li.y * aim;
+      sim += li.x * aim + li.y * are;
+    }
+    acc_g += -omega_mu[r] * sim;
+  }
+  g[e] += acc_g;
+}
+
 static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace pg
@@ -567,9 +654,9 @@ int pg_element_matrices(const void* tables, int64_t E, const void* ws, double* M
   return PG_OK;
 }
 
-int pg_assemble(const void* tables, const pg_plan_dev* pl, const void* ws, double omega,
-                const uint8_t* ent_bc, double* values, void* stream) {
-  if (!tables || !pl || !ws || !values) PG_FAIL(PG_ERR_ARG, "bad arguments");
+static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* ws, double omega,
+                         const uint8_t* ent_bc, double* values, double* Kout, double* Mout,
+                         void* stream) {
   const Tables* t = static_cast<const Tables*>(tables);
   if (pl->p != t->p) PG_FAIL(PG_ERR_ARG, "plan order %d != tables order %d", pl->p, t->p);
   if (pl->n_elems >= (1LL << 24)) PG_FAIL(PG_ERR_UNSUPPORTED, "more than 2^24 elements");
@@ -583,7 +670,7 @@ int pg_assemble(const void* tables, const pg_plan_dev* pl, const void* ws, doubl
 #define PG_ASM(P)                                                                          \
   hipLaunchKernelGGL(k_assemble<P>, g, b, 0, s, pl->npairs, pl->n_edges, pl->n_faces, t->d_T, \
                      pl->pair_slot0, pl->pair_rowlen, pl->pair_cbeg, pl->pair_ab, pl->contrib, \
-                     ws_coef(w), ws_orient(w, pl->n_elems), om, ent_bc, v)
+                     ws_coef(w), ws_orient(w, pl->n_elems), om, ent_bc, v, Kout, Mout)
   switch (t->p) {
     case 1: PG_ASM(1); break;
     case 2: PG_ASM(2); break;
@@ -594,9 +681,21 @@ int pg_assemble(const void* tables, const pg_plan_dev* pl, const void* ws, doubl
   return PG_OK;
 }
 
+int pg_assemble(const void* tables, const pg_plan_dev* pl, const void* ws, double omega,
+                const uint8_t* ent_bc, double* values, void* stream) {
+  if (!tables || !pl || !ws || !values) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  return assemble_impl(tables, pl, ws, omega, ent_bc, values, nullptr, nullptr, stream);
+}
+
+int pg_assemble_km(const void* tables, const pg_plan_dev* pl, const void* ws,
+                   const uint8_t* ent_bc, double* K, double* M, void* stream) {
+  if (!tables || !pl || !ws || !K || !M) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  return assemble_impl(tables, pl, ws, 0.0, ent_bc, nullptr, K, M, stream);
+}
+
 int pg_apply_dirichlet(int64_t n, const int32_t* rowptr, const int32_t* cols, const uint8_t* bc,
                        double* values, double* b, void* stream) {
-  if (n <= 0 || !rowptr || !cols || !bc || !values) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  if (n <= 0 || !rowptr || !cols || !bc || (!values && !b)) PG_FAIL(PG_ERR_ARG, "bad arguments");
   hipLaunchKernelGGL(k_dirichlet, dim3(nblocks(n, 256)), dim3(256), 0, as_stream(stream), n,
                      rowptr, cols, bc, reinterpret_cast<double2*>(values),
                      reinterpret_cast<double2*>(b));
@@ -656,6 +755,54 @@ int pg_receiver_fields(int p, int64_t npts, const double* pts, const int32_t* el
     case 3: hipLaunchKernelGGL(k_receivers<3>, g, b, 0, s, npts, pts, elem, coords, en, ee, edn, fe, dofs, xx, om, f); break;
   }
   PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_adjoint_rhs(int p, int64_t npts, const double* pts, const int32_t* elem,
+                   const double* coords, const int32_t* en, const int32_t* ee, const int32_t* edn,
+                   const int32_t* fe, const int32_t* dofs, const double* coef, double* f,
+                   void* stream) {
+  if (p < 1 || p > kMaxP) PG_FAIL(PG_ERR_UNSUPPORTED, "order p=%d not supported", p);
+  if (npts <= 0 || !pts || !elem || !coords || !en || !ee || !edn || !fe || !dofs || !coef || !f)
+    PG_FAIL(PG_ERR_ARG, "bad arguments");
+  hipStream_t s = as_stream(stream);
+  const double2* c = reinterpret_cast<const double2*>(coef);
+  double2* ff = reinterpret_cast<double2*>(f);
+  switch (p) {
+    case 1: hipLaunchKernelGGL(k_adjoint_rhs<1>, dim3(1), dim3(64), 0, s, npts, pts, elem, coords, en, ee, edn, fe, dofs, c, ff); break;
+    case 2: hipLaunchKernelGGL(k_adjoint_rhs<2>, dim3(1), dim3(64), 0, s, npts, pts, elem, coords, en, ee, edn, fe, dofs, c, ff); break;
+    case 3: hipLaunchKernelGGL(k_adjoint_rhs<3>, dim3(1), dim3(64), 0, s, npts, pts, elem, coords, en, ee, edn, fe, dofs, c, ff); break;
+  }
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_adjoint_gradient(const void* tables, int64_t E, const void* ws, const int32_t* dofs,
+                        int nrhs, int64_t ndof, const double* x, const double* lam,
+                        const double* omega, double* g, void* stream) {
+  if (!tables || !ws || !dofs || nrhs <= 0 || ndof <= 0 || !x || !lam || !omega || !g || E <= 0)
+    PG_FAIL(PG_ERR_ARG, "bad arguments");
+  if (nrhs > 4096) PG_FAIL(PG_ERR_ARG, "at most 4096 right-hand sides per call");
+  const Tables* t = static_cast<const Tables*>(tables);
+  std::vector<double> om(nrhs);
+  for (int r = 0; r < nrhs; ++r) om[r] = omega[r] * kMu0;
+  double* d_om = nullptr;
+  PG_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_om), nrhs * sizeof(double), as_stream(stream)));
+  hipStream_t s = as_stream(stream);
+  PG_HIP(hipMemcpyAsync(d_om, om.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s));
+  void* w = const_cast<void*>(ws);
+  const dim3 gr(nblocks(E, 256)), bl(256);
+  const double2* xx = reinterpret_cast<const double2*>(x);
+  const double2* ll = reinterpret_cast<const double2*>(lam);
+  switch (t->p) {
+    case 1: hipLaunchKernelGGL(k_adjoint_grad<1>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, ndof, xx, ll, d_om, g); break;
+    case 2: hipLaunchKernelGGL(k_adjoint_grad<2>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, ndof, xx, ll, d_om, g); break;
+    case 3: hipLaunchKernelGGL(k_adjoint_grad<3>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, ndof, xx, ll, d_om, g); break;
+  }
+  PG_CHECK_LAUNCH();
+  // the host copy above is synchronous with respect to `om` only once the stream drains
+  PG_HIP(hipStreamSynchronize(s));
+  PG_HIP(hipFreeAsync(d_om, s));
   return PG_OK;
 }
 

@@ -25,12 +25,12 @@
 //    row; lanes stride over the row's non-zeros, then combine with
 //    cross-lane shuffles.
 //
-// Mixed precision (PG_PREC_MIXED): the Krylov recursion, SpMV and Jacobi run
-// in complex64 on a complex64 copy of A; the solution is accumulated in FP64
-// and the FP64 residual b - A x is recomputed (and replaces the complex64
-// residual) whenever the complex64 residual has dropped by `inner_rtol`
-// since the last replacement ("reliable updates", van der Vorst & Ye 2000).
-// Dot products are always accumulated in FP64.
+// Mixed precision (PG_PREC_MIXED): iterative refinement.  The Krylov
+// recursion, SpMV and Jacobi run in complex64 on a complex64 copy of A; the
+// correction is accumulated into the FP64 solution and the FP64 residual
+// b - A x is recomputed whenever the complex64 residual has dropped by
+// `inner_rtol` since the last refinement, after which the complex64 COCG
+// restarts from that residual.  Dot products are always accumulated in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -421,6 +421,12 @@ struct Solver {
   float2 *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr, *x32 = nullptr,
          *dinv32 = nullptr;
   int32_t* bad = nullptr;
+  // GMRES(m): Krylov basis V[(m+1) x n], Hessenberg column buffer, partials
+  double2* V = nullptr;
+  double2* hdev = nullptr;
+  double2* hhost = nullptr;  // pinned
+  double* partg = nullptr;
+  int restart = 30;
   std::vector<void*> owned;
 
   template <typename T>
@@ -435,6 +441,7 @@ struct Solver {
   ~Solver() {
     for (void* v : owned) hipFree(v);
     if (h) hipHostFree(h);
+    if (hhost) hipHostFree(hhost);
   }
 };
 
@@ -576,8 +583,10 @@ static int solve(Solver* S, const double2* b, double2* x, pg_solve_info* info, h
       hipLaunchKernelGGL(k_cocg_reset<double2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->r64,
                          S->r, S->dinv, S->z, S->p, (const double2*)nullptr, S->part, 1);
     } else {
+      // restart the complex64 recursion from the FP64 residual (keeping the old
+      // search direction across a residual replacement stalls COCG)
       hipLaunchKernelGGL(k_cocg_reset<float2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->r64,
-                         S->r32, S->dinv32, S->z32, S->p32, (const double2*)nullptr, S->part, 0);
+                         S->r32, S->dinv32, S->z32, S->p32, (const double2*)nullptr, S->part, 1);
       hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
     }
     PG_CHECK_LAUNCH();
@@ -589,6 +598,220 @@ static int solve(Solver* S, const double2* b, double2* x, pg_solve_info* info, h
   return PG_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// GMRES(m) kernels (complex128, Hermitian inner products)
+// ---------------------------------------------------------------------------
+constexpr int kMaxRestart = 128;
+
+// h[i] = V_i^H w, i < k; fixed-order last-block reduction into h_out
+__global__ void __launch_bounds__(kBS) k_mdot(int64_t n, const double2* __restrict__ V,
+                                              int64_t ldv, int k, const double2* __restrict__ w,
+                                              double* part, uint32_t* counter,
+                                              double2* __restrict__ h_out) {
+  int64_t lo, hi;
+  chunk(n, kBS, &lo, &hi);
+  const int64_t G = gridDim.x;
+  for (int i = 0; i < k; ++i) {
+    double red[2] = {0.0, 0.0};
+    const double2* Vi = V + (int64_t)i * ldv;
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kBS) {
+      const double2 v = Vi[j], ww = w[j];
+      red[0] += v.x * ww.x + v.y * ww.y;
+      red[1] += v.x * ww.y - v.y * ww.x;
+    }
+    block_reduce<2>(red);
+    if (threadIdx.x == 0) {
+      part[(2 * i) * G + blockIdx.x] = red[0];
+      part[(2 * i + 1) * G + blockIdx.x] = red[1];
+    }
+    __syncthreads();
+  }
+  __shared__ int am_last;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t prev =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    am_last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!am_last) return;
+  __threadfence();
+  for (int i = 0; i < k; ++i) {
+    double red[2] = {0.0, 0.0};
+    for (int64_t b = threadIdx.x; b < G; b += kBS) {
+      red[0] += __hip_atomic_load(part + (2 * i) * G + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      red[1] += __hip_atomic_load(part + (2 * i + 1) * G + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_reduce<2>(red);
+    if (threadIdx.x == 0) h_out[i] = make_double2(red[0], red[1]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// w -= sum_i coef[i] V_i ; optional |w|^2 into coef_out (last block)
+__global__ void __launch_bounds__(kBS) k_maxpy_norm(int64_t n, const double2* __restrict__ V,
+                                                    int64_t ldv, int k,
+                                                    const double2* __restrict__ coef,
+                                                    double2* __restrict__ w, double* part,
+                                                    uint32_t* counter, double* norm_out) {
+  int64_t lo, hi;
+  chunk(n, kBS, &lo, &hi);
+  double red[1] = {0.0};
+  for (int64_t j = lo + threadIdx.x; j < hi; j += kBS) {
+    double2 acc = w[j];
+    for (int i = 0; i < k; ++i) {
+      const double2 c = coef[i], v = V[(int64_t)i * ldv + j];
+      acc.x -= c.x * v.x - c.y * v.y;
+      acc.y -= c.x * v.y + c.y * v.x;
+    }
+    w[j] = acc;
+    red[0] += acc.x * acc.x + acc.y * acc.y;
+  }
+  double tot[1];
+  if (!last_block_total<1>(red, part, counter, tot)) return;
+  if (threadIdx.x == 0) *norm_out = tot[0];
+}
+
+// out = dinv * v
+__global__ void k_dmul(int64_t n, const double2* __restrict__ d, const double2* __restrict__ v,
+                       double2* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = zmul(d[i], v[i]);
+}
+
+// v *= s (real)
+__global__ void k_rscale(int64_t n, double2* __restrict__ v, double s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = make_double2(v[i].x * s, v[i].y * s);
+}
+
+// x += dinv * sum_i y_i V_i
+__global__ void k_gmres_update(int64_t n, double2* __restrict__ x, const double2* __restrict__ d,
+                               const double2* __restrict__ V, int64_t ldv, int k,
+                               const double2* __restrict__ y) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int i = 0; i < k; ++i) acc = zadd(acc, zmul(y[i], V[(int64_t)i * ldv + j]));
+  x[j] = zadd(x[j], zmul(d[j], acc));
+}
+
+static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* info,
+                       hipStream_t st) {
+  const int64_t n = S->n;
+  const int m = S->restart;
+  const double tol = S->o.rtol;
+  const unsigned nb = nblk(n, kBS);
+  hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, S->o.maxit, tol * tol);
+  // |b|^2 through the residual kernel with x = 0 would need a zero vector: use w = b copy
+  double2* w0 = S->V;  // scratch: V_0
+  PG_HIP(hipMemcpyAsync(w0, b, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_maxpy_norm, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, 0, S->hdev, w0,
+                     S->part, &S->s->counter[3], &S->s->bnorm2);
+  PG_TRY(true_residual(S, st, b, x));
+  PG_TRY(read_scalars(S, st));
+  const double bnorm = std::sqrt(S->h->bnorm2);
+  double beta = std::sqrt(S->h->res64);
+  info->status = -1;
+  info->refinements = 0;
+  int iters = 0;
+  if (bnorm == 0.0) {
+    PG_HIP(hipMemsetAsync(x, 0, n * sizeof(double2), st));
+    info->iterations = 0;
+    info->relres = 0.0;
+    info->status = PG_OK;
+    return PG_OK;
+  }
+  std::vector<double2> H((size_t)(m + 1) * m), g(m + 1), sn(m);
+  std::vector<double> cs(m);
+  auto Hij = [&](int i, int j) -> double2& { return H[(size_t)i * m + j]; };
+  while (beta > tol * bnorm && iters < S->o.maxit) {
+    // V_0 = r / beta
+    PG_HIP(hipMemcpyAsync(S->V, S->r64, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_rscale, dim3(nb), dim3(kBS), 0, st, n, S->V, 1.0 / beta);
+    std::fill(H.begin(), H.end(), make_double2(0.0, 0.0));
+    std::fill(g.begin(), g.end(), make_double2(0.0, 0.0));
+    g[0] = make_double2(beta, 0.0);
+    int k = 0;
+    for (int j = 0; j < m && iters < S->o.maxit; ++j) {
+      double2* Vj = S->V + (int64_t)j * n;
+      double2* w = S->V + (int64_t)(j + 1) * n;
+      hipLaunchKernelGGL(k_dmul, dim3(nb), dim3(kBS), 0, st, n, S->dinv, Vj, S->z);
+      launch_spmv<0, double2, double2>(S->G64, S->gs64, st, S->s, n, S->rowptr, S->cols, S->vals,
+                                       S->z, w, nullptr, S->part);
+      // classical Gram-Schmidt, twice
+      for (int pass = 0; pass < 2; ++pass) {
+        double2* hcol = S->hdev + pass * (m + 1);
+        hipLaunchKernelGGL(k_mdot, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, j + 1, w, S->partg,
+                           &S->s->counter[2], hcol);
+        hipLaunchKernelGGL(k_maxpy_norm, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, j + 1, hcol,
+                           w, S->part, &S->s->counter[3], &S->s->res2);
+      }
+      PG_CHECK_LAUNCH();
+      PG_HIP(hipMemcpyAsync(S->hhost, S->hdev, 2 * (m + 1) * sizeof(double2),
+                            hipMemcpyDeviceToHost, st));
+      PG_TRY(read_scalars(S, st));
+      const double hn = std::sqrt(S->h->res2);
+      for (int i = 0; i <= j; ++i)
+        Hij(i, j) = zadd(S->hhost[i], S->hhost[(m + 1) + i]);
+      Hij(j + 1, j) = make_double2(hn, 0.0);
+      for (int i = 0; i < j; ++i) {  // apply previous rotations
+        const double2 a = Hij(i, j), bb = Hij(i + 1, j);
+        const double2 sc = make_double2(sn[i].x, -sn[i].y);
+        const double2 t = zadd(make_double2(cs[i] * a.x, cs[i] * a.y), zmul(sn[i], bb));
+        Hij(i + 1, j) = zsub(make_double2(cs[i] * bb.x, cs[i] * bb.y), zmul(sc, a));
+        Hij(i, j) = t;
+      }
+      {
+        const double2 a = Hij(j, j), bb = Hij(j + 1, j);
+        const double na = std::hypot(a.x, a.y), nbb = std::hypot(bb.x, bb.y);
+        const double rho = std::hypot(na, nbb);
+        if (na == 0.0) {
+          cs[j] = 0.0;
+          sn[j] = make_double2(1.0, 0.0);
+        } else {
+          cs[j] = na / rho;
+          const double2 ph = make_double2(a.x / na, a.y / na);
+          sn[j] = zmul(ph, make_double2(bb.x / rho, -bb.y / rho));
+        }
+        Hij(j, j) = zadd(make_double2(cs[j] * a.x, cs[j] * a.y), zmul(sn[j], bb));
+        Hij(j + 1, j) = make_double2(0.0, 0.0);
+        const double2 sc = make_double2(sn[j].x, -sn[j].y);
+        g[j + 1] = zmul(make_double2(-sc.x, -sc.y), g[j]);
+        g[j] = make_double2(cs[j] * g[j].x, cs[j] * g[j].y);
+      }
+      ++iters;
+      k = j + 1;
+      if (hn > 0.0) hipLaunchKernelGGL(k_rscale, dim3(nb), dim3(kBS), 0, st, n, w, 1.0 / hn);
+      const double est = std::hypot(g[j + 1].x, g[j + 1].y);
+      if (est <= tol * bnorm || hn == 0.0) break;
+    }
+    // y = H^-1 g (upper triangular), x += Dinv V y
+    std::vector<double2> y(k);
+    for (int i = k - 1; i >= 0; --i) {
+      double2 acc = g[i];
+      for (int l = i + 1; l < k; ++l) acc = zsub(acc, zmul(Hij(i, l), y[l]));
+      y[i] = zdiv(acc, Hij(i, i));
+    }
+    for (int i = 0; i < k; ++i) S->hhost[i] = y[i];
+    PG_HIP(hipMemcpyAsync(S->hdev, S->hhost, k * sizeof(double2), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_gmres_update, dim3(nb), dim3(kBS), 0, st, n, x, S->dinv, S->V, n, k,
+                       S->hdev);
+    PG_TRY(true_residual(S, st, b, x));
+    PG_TRY(read_scalars(S, st));
+    beta = std::sqrt(S->h->res64);
+    if (!std::isfinite(beta)) {
+      info->status = PG_ERR_BREAKDOWN;
+      break;
+    }
+  }
+  if (beta <= tol * bnorm) info->status = PG_OK;
+  info->iterations = iters;
+  info->relres = beta / bnorm;
+  return PG_OK;
+}
 
 // (Re)build everything derived from the caller's matrix values: the Jacobi
 // inverse diagonal and, for mixed precision, the complex64 copy of A.
@@ -658,8 +881,12 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
                      const double* values, const pg_solve_opts* opts, void** out, void* stream) {
   if (!out || !opts || n <= 0 || nnz <= 0 || !rowptr || !cols || !values)
     PG_FAIL(PG_ERR_ARG, "bad arguments");
-  if (opts->ksp != PG_KSP_COCG)
-    PG_FAIL(PG_ERR_UNSUPPORTED, "ksp type %d not supported by this solver", opts->ksp);
+  if (opts->ksp != PG_KSP_COCG && opts->ksp != PG_KSP_GMRES)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "ksp type %d not supported", opts->ksp);
+  if (opts->ksp == PG_KSP_GMRES && opts->precision != PG_PREC_FP64)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "GMRES runs in FP64 only (use COCG for mixed precision)");
+  if (opts->ksp == PG_KSP_GMRES && (opts->restart < 1 || opts->restart > kMaxRestart))
+    PG_FAIL(PG_ERR_ARG, "GMRES restart must be in 1..%d", kMaxRestart);
   if (opts->pc != PG_PC_NONE && opts->pc != PG_PC_JACOBI)
     PG_FAIL(PG_ERR_UNSUPPORTED, "pc type %d not supported", opts->pc);
   if (!(opts->rtol > 0.0)) PG_FAIL(PG_ERR_ARG, "rtol must be positive");
@@ -670,7 +897,7 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   S->o = *opts;
   if (S->o.maxit <= 0) S->o.maxit = 100000;
   if (S->o.check_every <= 0) S->o.check_every = 25;
-  if (!(S->o.inner_rtol > 0.0) || S->o.inner_rtol >= 1.0) S->o.inner_rtol = 1e-3;
+  if (!(S->o.inner_rtol > 0.0) || S->o.inner_rtol >= 1.0) S->o.inner_rtol = 1e-4;
   S->rowptr = rowptr;
   S->cols = cols;
   S->vals = reinterpret_cast<const double2*>(values);
@@ -699,6 +926,16 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
     PG_A(S->q, n);
     PG_A(S->z, n);
     PG_A(S->dinv, n);
+  }
+  if (opts->ksp == PG_KSP_GMRES) {
+    S->restart = opts->restart;
+    PG_A(S->V, (size_t)(S->restart + 1) * n);
+    PG_A(S->hdev, 2 * (S->restart + 1));
+    PG_A(S->partg, (size_t)2 * (S->restart + 1) * kMaxGrid);
+    if (hipHostMalloc(reinterpret_cast<void**>(&S->hhost), 2 * (S->restart + 1) * sizeof(double2), 0) != hipSuccess) {
+      delete S;
+      PG_FAIL(PG_ERR_HIP, "hipHostMalloc failed");
+    }
   }
 #undef PG_A
   if (hipHostMalloc(reinterpret_cast<void**>(&S->h), sizeof(Scal), 0) != hipSuccess) {
@@ -749,6 +986,9 @@ int pg_solver_refresh(void* solver, void* stream) {
 int pg_solver_solve(void* solver, const double* b, double* x, pg_solve_info* info, void* stream) {
   if (!solver || !b || !x || !info) PG_FAIL(PG_ERR_ARG, "bad arguments");
   Solver* S = static_cast<Solver*>(solver);
+  if (S->o.ksp == PG_KSP_GMRES)
+    return gmres_solve(S, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x),
+                       info, as_stream(stream));
   return solve(S, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x), info,
                as_stream(stream));
 }

@@ -1,0 +1,45 @@
+"""GPU adjoint gradient: against the FP64 CPU oracle gradient and against
+central finite differences of the device misfit, through the autograd Function."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from petgem_amd.adjoint import InversionProblem, misfit  # noqa: E402
+from petgem_amd.synthetic import Workload, receiver_line  # noqa: E402
+from oracle.adjoint import OracleInversion  # noqa: E402
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_device_gradient_vs_oracle_and_fd(p):
+    w = Workload(5, p, block=False)
+    T = w.topo
+    src = np.array([[0.0, 0.0, -950.0], [700.0, 300.0, -1200.0]])
+    freqs = [0.5, 1.0]
+    rx = receiver_line(9, z=-2000.0, x0=-3000.0, x1=3000.0, y=500.0)
+    true = np.log(w.sigma[:, 0]) + 0.3 * np.random.default_rng(1).standard_normal(T.n_elems)
+    m0 = np.log(w.sigma[:, 0])
+    ora = OracleInversion(T, src, freqs, rx)
+    ora.observed = ora.forward(true)
+    phi_ref, g_ref = ora.misfit_and_gradient(m0)
+
+    prob = InversionProblem(T, src, freqs, rx, rtol=1e-12)
+    obs = prob.forward(true)
+    assert np.abs(obs - ora.observed).max() <= 1e-8 * np.abs(ora.observed).max()
+    prob.observed = obs
+    ls = torch.tensor(m0, dtype=torch.float64, device="cuda", requires_grad=True)
+    phi = misfit(ls, prob)
+    phi.backward()
+    g = ls.grad.cpu().numpy()
+    err = np.abs(g - g_ref).max() / np.abs(g_ref).max()
+    print(f"p={p} phi={phi.item():.6e} (oracle {phi_ref:.6e}) grad rel err {err:.2e}", flush=True)
+    assert abs(phi.item() - phi_ref) <= 1e-8 * phi_ref
+    assert err <= 1e-7
+    h = 1e-4
+    for e in np.random.default_rng(3).choice(T.n_elems, 4, replace=False):
+        mp, mm = m0.copy(), m0.copy()
+        mp[e] += h
+        mm[e] -= h
+        fd = (prob.misfit(prob.forward(mp)) - prob.misfit(prob.forward(mm))) / (2 * h)
+        assert abs(fd - g[e]) <= 1e-5 * np.abs(g).max() + 1e-4 * abs(fd), (e, fd, g[e])
