@@ -193,8 +193,15 @@ int pg_demote_z2c(int64_t n, const double *src, float *dst, void *stream);
 #define PG_KSP_GMRES 1
 #define PG_PC_NONE 0
 #define PG_PC_JACOBI 1
+#define PG_PC_BJACOBI 2 /* entity-block Jacobi: blocks = DOFs of one edge/face/volume */
 #define PG_PREC_FP64 0
 #define PG_PREC_MIXED 1
+
+/* SpMV kernel families: one group of lanes per CSR row, or per mesh entity
+ * (the rows of an entity's DOFs share one column list; needs block_ptr). */
+#define PG_SPMV_AUTO 0
+#define PG_SPMV_ROW 1
+#define PG_SPMV_ENTITY 2
 
 typedef struct pg_solve_opts {
   int32_t ksp;        /* PG_KSP_* */
@@ -204,7 +211,11 @@ typedef struct pg_solve_opts {
   int32_t maxit;      /* max total Krylov iterations */
   int32_t check_every;/* iterations between host convergence checks */
   double rtol;        /* on the true residual ||b - A x|| / ||b|| */
-  double inner_rtol;  /* mixed precision: reliable-update threshold */
+  double inner_rtol;  /* mixed precision: residual drop between FP64 refinements */
+  const int32_t *block_ptr; /* PG_PC_BJACOBI: host array (nblocks+1), DOF ranges, sizes 1..8 */
+  int64_t nblocks;
+  int32_t spmv;       /* PG_SPMV_*: SpMV kernel family (AUTO = fastest by timing) */
+  int32_t pad;
 } pg_solve_opts;
 
 typedef struct pg_solve_info {

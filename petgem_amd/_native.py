@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libpetgem_amd.so")
 PG_OK = 0
 PG_ERR_ARG, PG_ERR_HIP, PG_ERR_UNSUPPORTED, PG_ERR_NOT_FOUND, PG_ERR_BREAKDOWN = 1, 2, 3, 4, 5
 PG_KSP_COCG, PG_KSP_GMRES = 0, 1
-PG_PC_NONE, PG_PC_JACOBI = 0, 1
+PG_PC_NONE, PG_PC_JACOBI, PG_PC_BJACOBI = 0, 1, 2
 PG_PREC_FP64, PG_PREC_MIXED = 0, 1
 
 # (name, restype, argtypes) for every exported entry point
@@ -30,7 +30,8 @@ class PlanDev(ctypes.Structure):
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("ksp", _I32), ("pc", _I32), ("precision", _I32), ("restart", _I32),
-                ("maxit", _I32), ("check_every", _I32), ("rtol", _D), ("inner_rtol", _D)]
+                ("maxit", _I32), ("check_every", _I32), ("rtol", _D), ("inner_rtol", _D),
+                ("block_ptr", _P), ("nblocks", _I64), ("spmv", _I32), ("pad", _I32)]
 
 
 class SolveInfo(ctypes.Structure):

@@ -113,8 +113,8 @@ class SolverOptions:
 
     KSP = {"cocg": "cocg", "gmres": "gmres", "fgmres": "gmres", "lgmres": "gmres",
            "cg": "cocg", "bcgs": "cocg", "tfqmr": "cocg", "preonly": "cocg"}
-    PC = {"jacobi": "jacobi", "none": "none", "sor": "jacobi", "bjacobi": "jacobi",
-          "pbjacobi": "jacobi", "asm": "jacobi", "ilu": "jacobi", "gamg": "jacobi", "lu": "jacobi"}
+    PC = {"jacobi": "jacobi", "none": "none", "bjacobi": "bjacobi", "pbjacobi": "bjacobi",
+          "sor": "bjacobi", "asm": "bjacobi", "ilu": "bjacobi", "gamg": "bjacobi", "lu": "bjacobi"}
 
     def __init__(self, argv=None, options_file=None):
         self.ksp_type = "gmres"            # PETSc's default KSP
@@ -164,7 +164,7 @@ class SolverOptions:
                 self.notes.append(f"-ksp_type {v} -> {self.ksp_type}")
         elif k == "-pc_type" and v:
             self.pc_type = self.PC.get(v, "jacobi")
-            if v not in ("jacobi", "none"):
+            if v not in ("jacobi", "none", "bjacobi"):
                 self.notes.append(f"-pc_type {v} -> {self.pc_type}")
         elif k == "-ksp_rtol" and v:
             self.rtol = float(v)

@@ -74,6 +74,16 @@ class DeviceMesh:
                    topo.elem_edges_nodes, topo.elem_faces_edges, topo.dofs, topo.n_edges,
                    topo.n_faces, topo.elems_faces, device)
 
+    def block_ptr(self):
+        """DOF ranges of the mesh entities (edges, faces, volumes): entity-block Jacobi blocks."""
+        de, df, dv, _ = dof_counts(self.p)
+        sizes = [np.full(self.n_edges, de)]
+        if df:
+            sizes.append(np.full(self.n_faces, df))
+        if dv:
+            sizes.append(np.full(self.n_elems, dv))
+        return np.concatenate([[0], np.cumsum(np.concatenate(sizes))]).astype(np.int32)
+
     def entity_mask(self, boundary_dofs):
         """(per-entity, per-DOF) uint8 boundary masks from a list of boundary DOFs."""
         de, df, dv, _ = dof_counts(self.p)
@@ -291,19 +301,30 @@ def receiver_fields(mesh, points, elements, x, omega, stream=None):
 
 
 _KSP = {"cocg": nat.PG_KSP_COCG, "gmres": nat.PG_KSP_GMRES}
-_PC = {"none": nat.PG_PC_NONE, "jacobi": nat.PG_PC_JACOBI}
+_PC = {"none": nat.PG_PC_NONE, "jacobi": nat.PG_PC_JACOBI, "bjacobi": nat.PG_PC_BJACOBI}
 _PREC = {"fp64": nat.PG_PREC_FP64, "mixed": nat.PG_PREC_MIXED}
+
+
+_SPMV = {"auto": 0, "row": 1, "entity": 2}
 
 
 class KrylovSolver:
     """Device Krylov solver for a complex symmetric CSR matrix (values complex128)."""
 
     def __init__(self, plan, values, ksp="cocg", pc="jacobi", precision="fp64", rtol=1e-8,
-                 maxit=100000, check_every=25, inner_rtol=1e-4, restart=30, stream=None):
+                 maxit=100000, check_every=25, inner_rtol=1e-4, restart=30, block_ptr=None,
+                 spmv="auto", stream=None):
         self.plan = plan
         self.values = values
+        if pc == "bjacobi" and block_ptr is None:
+            raise ValueError("pc='bjacobi' needs block_ptr (DeviceMesh.block_ptr())")
+        # block_ptr (the mesh entities' DOF ranges) also enables the entity-row SpMV
+        self._bptr = None if block_ptr is None else np.ascontiguousarray(block_ptr, np.int32)
+        nb = 0 if self._bptr is None else self._bptr.size - 1
         self.opts = nat.SolveOpts(_KSP[ksp], _PC[pc], _PREC[precision], int(restart), int(maxit),
-                                  int(check_every), float(rtol), float(inner_rtol))
+                                  int(check_every), float(rtol), float(inner_rtol),
+                                  None if self._bptr is None else self._bptr.ctypes.data, nb,
+                                  _SPMV[spmv], 0)
         h = ctypes.c_void_p()
         nat.check(nat.lib().pg_solver_create(plan.ndof, plan.nnz, nat.ptr(plan.rowptr),
                                              nat.ptr(plan.cols), nat.ptr(values),

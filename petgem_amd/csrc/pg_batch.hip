@@ -8,8 +8,9 @@
 // gathers x[col][0..R) coalesced.  Each column runs its own COCG recurrence
 // (its own alpha, beta, rho, convergence flag) with Jacobi preconditioning on
 // its own diagonal K_ii - i*omega_r*mu0*M_ii; all columns share each launch.
-// Reductions are per-workgroup partials per column, finished in fixed order
-// by the last workgroup (bitwise reproducible).
+// Reductions are per-workgroup partials per column written with plain
+// stores, finished in a fixed order by a one-workgroup-per-column kernel
+// (bitwise reproducible, no device-scope fences).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,10 +35,7 @@ struct Col {  // per-column scalars
 struct Ctl {
   double tol2;
   int32_t maxit, ncols, all_done, pad;
-  uint32_t counter[4];
 };
-
-__device__ inline double2 cmul(double2 a, double2 b) { return zmul(a, b); }
 
 // chunk of rows for this workgroup (XCD-contiguous renumbering)
 __device__ inline void rows_of_block(int64_t n, int64_t* lo, int64_t* hi) {
@@ -52,14 +50,13 @@ __device__ inline void rows_of_block(int64_t n, int64_t* lo, int64_t* hi) {
 // Threads are arranged (rsub, c): c = column lane (0..CT), rsub = row lane.
 // Column r handled by lane c is r = c + t*CT, t < R/CT.
 // Per-column partial sums (NV doubles each) are reduced over rsub in shared
-// memory and written to part[(k*R + r)*grid + block]; the last block sums
-// them in fixed order into tot (returned through `fin`).
+// memory and written with plain stores to part[(k*R + r)*grid + block]; a
+// finish kernel with one workgroup per column adds them in a fixed order
+// after the kernel boundary has made them visible.
 template <int CT, int NV, int TMAX>
-__device__ inline bool column_partials(int R, double (&v)[TMAX][NV], double* part,
-                                       uint32_t* counter) {
+__device__ inline void column_partials(int R, double (&v)[TMAX][NV], double* part) {
   constexpr int RPB = kBS / CT;
   __shared__ double sm[RPB][CT][NV];
-  __shared__ int am_last;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
   const int T = R / CT;
   for (int t = 0; t < T; ++t) {
@@ -75,30 +72,38 @@ __device__ inline bool column_partials(int R, double (&v)[TMAX][NV], double* par
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t prev =
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    am_last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
+}
+
+// in a finish kernel (one workgroup per column r = blockIdx.x): fixed-order
+// totals of that column's np partials; valid in every thread
+template <int NV>
+__device__ inline void column_totals(const double* part, int R, int np, double (&tot)[NV]) {
+  const int r = blockIdx.x;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int i = threadIdx.x; i < np; i += kBS)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += part[((int64_t)k * R + r) * np + i];
+  __shared__ double sm[NV][kBS / kWave];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sm[k][w] = acc[k];
   __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
-  return true;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < kBS / kWave; ++i) s += sm[k][i];
+    tot[k] = s;
+  }
 }
 
-// fixed-order total of one column's partials (called by one thread per (k, r))
-__device__ inline double column_total(const double* part, int k, int R, int r) {
-  double acc = 0.0;
-  const int64_t base = ((int64_t)k * R + r) * gridDim.x;
-  for (int64_t b = 0; b < gridDim.x; ++b)
-    acc += __hip_atomic_load(part + base + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return acc;
-}
-
-// q = A_r p for every column; mu_r = p_r^T q_r; alpha_r = rho_r / mu_r
+// q = A_r p for every column, partials of mu_r = p_r^T q_r
 template <int CT, int TMAX>
-__global__ void __launch_bounds__(kBS) k_spmm(Ctl* ctl, Col* col, int64_t n, int R,
+__global__ void __launch_bounds__(kBS) k_spmm(const Ctl* ctl, int64_t n, int R,
                                               const int32_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ cols,
                                               const double* __restrict__ K,
@@ -146,30 +151,36 @@ __global__ void __launch_bounds__(kBS) k_spmm(Ctl* ctl, Col* col, int64_t n, int
         v[t][1] += pv.x * acc[t].y + pv.y * acc[t].x;
       }
   }
-  if (!column_partials<CT, 2, TMAX>(R, v, part, &ctl->counter[0])) return;
-  for (int r = threadIdx.x; r < R; r += kBS) {
-    Col& cc = col[r];
-    const double2 mu = make_double2(column_total(part, 0, R, r), column_total(part, 1, R, r));
-    cc.mu = mu;
-    if (cc.done) {
-      cc.alpha = make_double2(0.0, 0.0);
-      continue;
-    }
-    const double2 al = zdiv(cc.rho, mu);
-    if (!(mu.x != 0.0 || mu.y != 0.0) || !isfinite(al.x) || !isfinite(al.y)) {
-      cc.status = PG_ERR_BREAKDOWN;
-      cc.done = 1;
-      cc.alpha = make_double2(0.0, 0.0);
-    } else {
-      cc.alpha = al;
-    }
-  }
-  if (threadIdx.x == 0) __hip_atomic_store(&ctl->counter[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  column_partials<CT, 2, TMAX>(R, v, part);
 }
 
-// x += alpha p; r -= alpha q; z = r / diag_r; rho = r^T z; |r|^2 (per column)
+// finish mu_r and alpha_r = rho_r / mu_r (one workgroup per column)
+__global__ void __launch_bounds__(kBS) k_fin_mu(const Ctl* ctl, Col* col, int R,
+                                                const double* part, int np) {
+  if (ctl->all_done) return;
+  double tot[2];
+  column_totals<2>(part, R, np, tot);
+  if (threadIdx.x != 0) return;
+  Col& cc = col[blockIdx.x];
+  const double2 mu = make_double2(tot[0], tot[1]);
+  cc.mu = mu;
+  if (cc.done) {
+    cc.alpha = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2 al = zdiv(cc.rho, mu);
+  if (!(mu.x != 0.0 || mu.y != 0.0) || !isfinite(al.x) || !isfinite(al.y)) {
+    cc.status = PG_ERR_BREAKDOWN;
+    cc.done = 1;
+    cc.alpha = make_double2(0.0, 0.0);
+  } else {
+    cc.alpha = al;
+  }
+}
+
+// x += alpha p; r -= alpha q; z = r / diag_r; partials of rho = r^T z and |r|^2
 template <int CT, int TMAX>
-__global__ void __launch_bounds__(kBS) k_update(Ctl* ctl, Col* col, int64_t n, int R,
+__global__ void __launch_bounds__(kBS) k_update(const Ctl* ctl, const Col* col, int64_t n, int R,
                                                 const double* __restrict__ dK,
                                                 const double* __restrict__ dM,
                                                 const double* __restrict__ omu,
@@ -213,41 +224,51 @@ __global__ void __launch_bounds__(kBS) k_update(Ctl* ctl, Col* col, int64_t n, i
         v[t][2] += rv.x * rv.x + rv.y * rv.y;
       }
   }
-  if (!column_partials<CT, 3, TMAX>(R, v, part, &ctl->counter[1])) return;
-  __shared__ int ndone;
-  if (threadIdx.x == 0) ndone = 0;
-  __syncthreads();
-  for (int r = threadIdx.x; r < R; r += kBS) {
-    Col& cc = col[r];
-    const double2 rho_new = make_double2(column_total(part, 0, R, r), column_total(part, 1, R, r));
-    const double res2 = column_total(part, 2, R, r);
-    if (!cc.done) {
-      const double2 beta = zdiv(rho_new, cc.rho);
-      cc.beta = beta;
-      cc.rho = rho_new;
-      cc.res2 = res2;
-      cc.iter += 1;
-      if (!isfinite(res2) || !isfinite(beta.x) || !isfinite(beta.y)) {
-        cc.status = PG_ERR_BREAKDOWN;
-        cc.done = 1;
-      } else if (res2 <= ctl->tol2 * cc.bnorm2 || cc.iter >= ctl->maxit) {
-        cc.done = 1;
-      }
-    } else {
-      cc.beta = make_double2(0.0, 0.0);
-    }
-    if (cc.done) atomicAdd(&ndone, 1);
+  column_partials<CT, 3, TMAX>(R, v, part);
+}
+
+// finish rho_r, |r_r|^2, beta_r and column r's convergence (one workgroup per column)
+__global__ void __launch_bounds__(kBS) k_fin_update(const Ctl* ctl, Col* col, int R,
+                                                    const double* part, int np) {
+  if (ctl->all_done) return;
+  double tot[3];
+  column_totals<3>(part, R, np, tot);
+  if (threadIdx.x != 0) return;
+  Col& cc = col[blockIdx.x];
+  if (cc.done) {
+    cc.beta = make_double2(0.0, 0.0);
+    return;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ctl->all_done = (ndone == R) ? 1 : 0;
-    __hip_atomic_store(&ctl->counter[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const double2 rho_new = make_double2(tot[0], tot[1]);
+  const double2 beta = zdiv(rho_new, cc.rho);
+  cc.beta = beta;
+  cc.rho = rho_new;
+  cc.res2 = tot[2];
+  cc.iter += 1;
+  if (!isfinite(tot[2]) || !isfinite(beta.x) || !isfinite(beta.y)) {
+    cc.status = PG_ERR_BREAKDOWN;
+    cc.done = 1;
+  } else if (tot[2] <= ctl->tol2 * cc.bnorm2 || cc.iter >= ctl->maxit) {
+    cc.done = 1;
   }
+}
+
+// all_done = every column done (one workgroup)
+__global__ void __launch_bounds__(kBS) k_fin_all(Ctl* ctl, const Col* col, int R) {
+  if (ctl->all_done) return;
+  __shared__ int notdone;
+  if (threadIdx.x == 0) notdone = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int r = threadIdx.x; r < R; r += kBS) mine |= col[r].done ? 0 : 1;
+  if (mine) notdone = 1;  // benign race: every writer stores 1
+  __syncthreads();
+  if (threadIdx.x == 0) ctl->all_done = notdone ? 0 : 1;
 }
 
 // p = z + beta p (done columns keep p; their alpha is 0)
 template <int CT, int TMAX>
-__global__ void __launch_bounds__(kBS) k_pupdate(Ctl* ctl, const Col* col, int64_t n, int R,
+__global__ void __launch_bounds__(kBS) k_pupdate(const Ctl* ctl, const Col* col, int64_t n, int R,
                                                  const double2* __restrict__ Z,
                                                  double2* __restrict__ P) {
   if (ctl->all_done) return;
@@ -270,10 +291,9 @@ __global__ void __launch_bounds__(kBS) k_pupdate(Ctl* ctl, const Col* col, int64
       }
 }
 
-// init: r = b - A x (x = 0 -> r = b), z = r/diag, p = z; rho, |r|^2, |b|^2 per column
+// init: x = 0, r = b, z = r/diag, p = 0; partials of rho, |r|^2 (= |b|^2)
 template <int CT, int TMAX>
-__global__ void __launch_bounds__(kBS) k_init(Ctl* ctl, Col* col, int64_t n, int R,
-                                              const double* __restrict__ dK,
+__global__ void __launch_bounds__(kBS) k_init(int64_t n, int R, const double* __restrict__ dK,
                                               const double* __restrict__ dM,
                                               const double* __restrict__ omu,
                                               const double2* __restrict__ B,
@@ -309,27 +329,27 @@ __global__ void __launch_bounds__(kBS) k_init(Ctl* ctl, Col* col, int64_t n, int
         v[t][2] += bv.x * bv.x + bv.y * bv.y;
       }
   }
-  if (!column_partials<CT, 3, TMAX>(R, v, part, &ctl->counter[2])) return;
-  for (int r = threadIdx.x; r < R; r += kBS) {
-    Col& cc = col[r];
-    cc.rho = make_double2(column_total(part, 0, R, r), column_total(part, 1, R, r));
-    cc.res2 = column_total(part, 2, R, r);
-    cc.bnorm2 = cc.res2;
-    cc.beta = make_double2(0.0, 0.0);
-    cc.alpha = make_double2(0.0, 0.0);
-    cc.iter = 0;
-    cc.status = 0;
-    cc.done = (cc.bnorm2 == 0.0) ? 1 : 0;
-  }
-  if (threadIdx.x == 0) {
-    ctl->all_done = 0;
-    __hip_atomic_store(&ctl->counter[2], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  column_partials<CT, 3, TMAX>(R, v, part);
 }
 
-// true residual per column: |b - A x|^2 into col[r].res2 (after the solve)
+__global__ void __launch_bounds__(kBS) k_fin_init(Col* col, int R, const double* part, int np) {
+  double tot[3];
+  column_totals<3>(part, R, np, tot);
+  if (threadIdx.x != 0) return;
+  Col& cc = col[blockIdx.x];
+  cc.rho = make_double2(tot[0], tot[1]);
+  cc.res2 = tot[2];
+  cc.bnorm2 = tot[2];
+  cc.beta = make_double2(0.0, 0.0);
+  cc.alpha = make_double2(0.0, 0.0);
+  cc.iter = 0;
+  cc.status = 0;
+  cc.done = (cc.bnorm2 == 0.0) ? 1 : 0;
+}
+
+// true residual per column: partials of |b - A x|^2
 template <int CT, int TMAX>
-__global__ void __launch_bounds__(kBS) k_true_res(Ctl* ctl, Col* col, int64_t n, int R,
+__global__ void __launch_bounds__(kBS) k_true_res(int64_t n, int R,
                                                   const int32_t* __restrict__ rowptr,
                                                   const int32_t* __restrict__ cols,
                                                   const double* __restrict__ K,
@@ -373,9 +393,13 @@ __global__ void __launch_bounds__(kBS) k_true_res(Ctl* ctl, Col* col, int64_t n,
         v[t][0] += rx * rx + ry * ry;
       }
   }
-  if (!column_partials<CT, 1, TMAX>(R, v, part, &ctl->counter[3])) return;
-  for (int r = threadIdx.x; r < R; r += kBS) col[r].res2 = column_total(part, 0, R, r);
-  if (threadIdx.x == 0) __hip_atomic_store(&ctl->counter[3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  column_partials<CT, 1, TMAX>(R, v, part);
+}
+
+__global__ void __launch_bounds__(kBS) k_fin_true(Col* col, int R, const double* part, int np) {
+  double tot[1];
+  column_totals<1>(part, R, np, tot);
+  if (threadIdx.x == 0) col[blockIdx.x].res2 = tot[0];
 }
 
 __global__ void k_diag(int64_t n, const int32_t* __restrict__ rowptr,
@@ -448,24 +472,29 @@ struct Batch {
 
 template <int CT, int TM>
 static void launch_iter(Batch* B, hipStream_t st, double2* X) {
-  const dim3 g(B->grid), b(kBS);
+  const dim3 g(B->grid), b(kBS), gr(B->R);
   hipLaunchKernelGGL((k_pupdate<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->Z, B->P);
-  hipLaunchKernelGGL((k_spmm<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->rowptr,
-                     B->cols, B->K, B->M, B->omu, B->P, B->Q, B->part);
+  hipLaunchKernelGGL((k_spmm<CT, TM>), g, b, 0, st, B->ctl, B->n, B->R, B->rowptr, B->cols, B->K,
+                     B->M, B->omu, B->P, B->Q, B->part);
+  hipLaunchKernelGGL(k_fin_mu, gr, b, 0, st, B->ctl, B->col, B->R, B->part, B->grid);
   hipLaunchKernelGGL((k_update<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK, B->dM,
                      B->omu, X, B->Rr, B->P, B->Q, B->Z, B->part);
+  hipLaunchKernelGGL(k_fin_update, gr, b, 0, st, B->ctl, B->col, B->R, B->part, B->grid);
+  hipLaunchKernelGGL(k_fin_all, dim3(1), b, 0, st, B->ctl, B->col, B->R);
 }
 
 template <int CT, int TM>
 static void launch_init(Batch* B, hipStream_t st, const double2* Bv, double2* X) {
-  hipLaunchKernelGGL((k_init<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->ctl, B->col, B->n,
-                     B->R, B->dK, B->dM, B->omu, Bv, X, B->Rr, B->Z, B->P, B->part);
+  hipLaunchKernelGGL((k_init<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->dK, B->dM,
+                     B->omu, Bv, X, B->Rr, B->Z, B->P, B->part);
+  hipLaunchKernelGGL(k_fin_init, dim3(B->R), dim3(kBS), 0, st, B->col, B->R, B->part, B->grid);
 }
 
 template <int CT, int TM>
 static void launch_true(Batch* B, hipStream_t st, const double2* Bv, const double2* X) {
-  hipLaunchKernelGGL((k_true_res<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->ctl, B->col, B->n,
-                     B->R, B->rowptr, B->cols, B->K, B->M, B->omu, X, Bv, B->part);
+  hipLaunchKernelGGL((k_true_res<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->rowptr,
+                     B->cols, B->K, B->M, B->omu, X, Bv, B->part);
+  hipLaunchKernelGGL(k_fin_true, dim3(B->R), dim3(kBS), 0, st, B->col, B->R, B->part, B->grid);
 }
 
 // (CT, TM) configurations: R = CT * T with T <= TM

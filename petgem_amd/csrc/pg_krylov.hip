@@ -3,38 +3,41 @@
 // Replaces PETSc's MatMult and KSPSolve as called from Solver.run
 // (petgem/solver.py:584-590).  The system matrix is complex symmetric
 // (A = A^T: K and M are symmetric), so the default method is COCG
-// (conjugate orthogonal CG, van der Vorst & Melissen 1990) with a Jacobi
-// preconditioner; the stopping test is the true-norm residual
-// ||b - A x||_2 / ||b||_2 (PETSc's GMRES default is the preconditioned
-// residual, which is weaker).
+// (conjugate orthogonal CG, van der Vorst & Melissen 1990) with point or
+// entity-block Jacobi; GMRES(m) is provided as well.  The stopping test is the
+// true-norm residual ||b - A x||_2 / ||b||_2 (PETSc's GMRES default is the
+// preconditioned residual, which is weaker).
 //
 // Device organisation:
-//  * one HIP stream; every iteration is three kernels
-//      K3 p = z + beta p,   K1 q = A p with mu = p^T q,   K2 x,r,z updates with
-//      rho = r^T z and |r|^2;
-//    the scalars (alpha, beta, rho, ...) live in device memory and are
-//    finished by the last workgroup of each kernel (fixed-order reduction of
-//    per-workgroup partials, so results are bitwise reproducible);
+//  * one HIP stream; a COCG iteration is
+//      p = z + beta p | q = A p (+ partials of p^T q) | finish mu, alpha |
+//      x += alpha p, r -= alpha q, z = P r (+ partials of r^T z, |r|^2) |
+//      finish rho, beta, convergence
+//    with every scalar in device memory;
+//  * reductions: each workgroup writes its partial sums with plain stores and
+//    a one-workgroup "finish" kernel adds them in a fixed order (bitwise
+//    reproducible).  The kernel boundary makes the partials visible, so no
+//    workgroup issues a device-scope fence (those force L2 write-backs
+//    across the XCDs and cost ~0.2 ms per kernel at this size);
 //  * every kernel returns immediately once the device-side `done` flag is
-//    set, so the host enqueues batches of iterations and looks at the flags
-//    only between batches;
-//  * rows are split into one contiguous chunk per workgroup and workgroups
-//    are renumbered so that each XCD owns a contiguous eighth of the rows
-//    (its L2 then sees a compact window of the gathered vector);
-//  * the SpMV assigns G lanes (G in 8..64, chosen per matrix by timing) to a
-//    row; lanes stride over the row's non-zeros, then combine with
-//    cross-lane shuffles.
+//    set, so the host enqueues batches of iterations and reads the flags only
+//    between batches;
+//  * rows are split into one contiguous chunk per workgroup, workgroups
+//    renumbered so that each XCD owns a contiguous eighth of the rows;
+//  * SpMV kernels (chosen per matrix by timing): "row" = G lanes per CSR row,
+//    "entity-row" = G lanes per mesh entity whose d rows share one column
+//    list (each index and gathered x value read once for d rows); each lane
+//    issues U independent value/index/gather loads before any FMA.
 //
 // Mixed precision (PG_PREC_MIXED): iterative refinement.  The Krylov
-// recursion, SpMV and Jacobi run in complex64 on a complex64 copy of A; the
-// correction is accumulated into the FP64 solution and the FP64 residual
-// b - A x is recomputed whenever the complex64 residual has dropped by
-// `inner_rtol` since the last refinement, after which the complex64 COCG
+// recursion, SpMV and preconditioner run in complex64 on a complex64 copy of
+// A; the correction is accumulated into the FP64 solution and the FP64
+// residual b - A x is recomputed whenever the complex64 residual has dropped
+// by `inner_rtol` since the last refinement, after which the complex64 COCG
 // restarts from that residual.  Dot products are always accumulated in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -44,15 +47,19 @@ namespace pg {
 
 constexpr int kBS = 256;
 constexpr int kMaxGrid = 2048;
+constexpr int kMaxBlock = 8;
+constexpr int kMaxRestart = 128;
 
 struct Scal {
   double2 rho, mu, alpha, beta;
   double res2, bnorm2, tol2, ref2;
   double res64;
   int32_t iter, maxit, done, status;
-  uint32_t counter[4];
 };
 
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
 __device__ inline double2 to_d(double2 v) { return v; }
 __device__ inline double2 to_d(float2 v) { return make_double2(v.x, v.y); }
 template <typename T>
@@ -64,8 +71,6 @@ __device__ inline float2 from_d<float2>(double2 v) { return make_float2((float)v
 
 __device__ inline double2 cm(double2 a, double2 b) { return zmul(a, b); }
 __device__ inline float2 cm(float2 a, float2 b) { return cmul(a, b); }
-__device__ inline double2 ca(double2 a, double2 b) { return zadd(a, b); }
-__device__ inline float2 ca(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 cfma(double2 a, double2 b, double2 c) {
   return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
 }
@@ -79,13 +84,14 @@ __device__ inline double2 czero<double2>() { return make_double2(0.0, 0.0); }
 template <>
 __device__ inline float2 czero<float2>() { return make_float2(0.f, 0.f); }
 
-// logical workgroup id: each XCD (blockIdx % 8) gets a contiguous range
+// ---------------------------------------------------------------------------
+// work distribution and reductions
+// ---------------------------------------------------------------------------
 __device__ inline int64_t logical_block() {
   const int64_t b = blockIdx.x, nb = gridDim.x;
   if (nb % 8 != 0) return b;
   return (b % 8) * (nb / 8) + b / 8;
 }
-// contiguous chunk [lo, hi) of n items for this workgroup, in units of `unit`
 __device__ inline void chunk(int64_t n, int64_t unit, int64_t* lo, int64_t* hi) {
   const int64_t nb = gridDim.x;
   const int64_t units = (n + unit - 1) / unit;
@@ -114,53 +120,43 @@ __device__ inline void block_reduce(double (&v)[NV]) {
       for (int i = 0; i < kBS / kWave; ++i) s += sm[k][i];
       v[k] = s;
     }
+  __syncthreads();
 }
 
-// Write this workgroup's NV partials; returns true in the last workgroup to
-// arrive, after which `tot` (all threads) holds the fixed-order grand totals.
+// this workgroup's partial sums -> part[k * gridDim.x + blockIdx.x]
 template <int NV>
-__device__ inline bool last_block_total(double (&v)[NV], double* part, uint32_t* counter,
-                                        double (&tot)[NV]) {
-  __shared__ int am_last;
+__device__ inline void write_partials(double (&v)[NV], double* part) {
   block_reduce<NV>(v);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < NV; ++k) part[(int64_t)k * gridDim.x + blockIdx.x] = v[k];
-    __threadfence();
-    const uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-    am_last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
+}
+
+// in a one-workgroup finish kernel: fixed-order totals of np partials; all threads
+template <int NV>
+__device__ inline void sum_partials(const double* part, int np, double (&tot)[NV]) {
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (int64_t i = threadIdx.x; i < gridDim.x; i += kBS)
+  for (int i = threadIdx.x; i < np; i += kBS)
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      acc[k] += __hip_atomic_load(part + (int64_t)k * gridDim.x + i, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < NV; ++k) acc[k] += part[(int64_t)k * np + i];
   block_reduce<NV>(acc);
   __shared__ double res[NV];
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < NV; ++k) res[k] = acc[k];
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; ++k) tot[k] = res[k];
-  return true;
 }
 
 // ---------------------------------------------------------------------------
-// SpMV.  MODE 0: y = A x.  MODE 1: y = A x and mu = x^T y (COCG).
-// MODE 2: y = b - A x and |y|^2 (true residual).
+// SpMV.  MODE 0: y = A x.  MODE 1: y = A x and partials of x^T y (COCG mu).
+// MODE 2: y = b - A x and partials of |y|^2 (true residual).
 // ---------------------------------------------------------------------------
-template <int G, int MODE, typename VT, typename XT>
-__global__ void __launch_bounds__(kBS) k_spmv(Scal* __restrict__ s, int64_t n,
+template <int G, int U, int MODE, typename VT, typename XT>
+__global__ void __launch_bounds__(kBS) k_spmv(const Scal* __restrict__ s, int64_t n,
                                               const int32_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ cols,
                                               const VT* __restrict__ vals,
@@ -175,10 +171,20 @@ __global__ void __launch_bounds__(kBS) k_spmv(Scal* __restrict__ s, int64_t n,
   for (int64_t row = lo + threadIdx.x / G; row < hi; row += RPB) {
     XT acc = czero<XT>();
     const int s0 = rowptr[row], s1 = rowptr[row + 1];
-    for (int k = s0 + lane; k < s1; k += G) {
-      const VT a = vals[k];
-      const XT xv = x[cols[k]];
-      acc = cfma(XT{(decltype(acc.x))a.x, (decltype(acc.x))a.y}, xv, acc);
+    for (int kb = s0 + lane; kb < s1; kb += G * U) {
+      int c[U];
+      VT a[U];
+      XT xv[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int k = kb + j * G;
+        c[j] = k < s1 ? cols[k] : -1;
+        a[j] = k < s1 ? vals[k] : VT{0, 0};
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : czero<XT>();
+#pragma unroll
+      for (int j = 0; j < U; ++j) acc = cfma(XT{a[j].x, a[j].y}, xv[j], acc);
     }
     acc.x = group_sum<G>(acc.x);
     acc.y = group_sum<G>(acc.y);
@@ -191,43 +197,124 @@ __global__ void __launch_bounds__(kBS) k_spmv(Scal* __restrict__ s, int64_t n,
       } else {
         y[row] = acc;
         if (MODE == 1) {
-          const double2 pv = to_d(x[row]);
-          const double2 qv = to_d(acc);
+          const double2 pv = to_d(x[row]), qv = to_d(acc);
           red[0] += pv.x * qv.x - pv.y * qv.y;
           red[1] += pv.x * qv.y + pv.y * qv.x;
         }
       }
     }
   }
-  if (MODE == 0) return;
-  double tot[2];
-  if (!last_block_total<2>(red, part, &s->counter[0], tot)) return;
-  if (threadIdx.x != 0) return;
-  if (MODE == 1) {
-    const double2 mu = make_double2(tot[0], tot[1]);
-    s->mu = mu;
-    const double2 al = zdiv(s->rho, mu);
-    if (!(mu.x != 0.0 || mu.y != 0.0) || !isfinite(al.x) || !isfinite(al.y)) {
-      s->status = PG_ERR_BREAKDOWN;
-      s->done = 1;
-      s->alpha = make_double2(0.0, 0.0);
-    } else {
-      s->alpha = al;
+  if (MODE != 0) write_partials<2>(red, part);
+}
+
+// entity-row SpMV: the d rows of entity e (DOFs bptr[e]..bptr[e+1]) are
+// consecutive CSR rows of equal length L with the same column list
+template <int G, int U, int D, int MODE, typename VT, typename XT>
+__global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int64_t nent,
+                                                 const int32_t* __restrict__ bptr,
+                                                 const int32_t* __restrict__ rowptr,
+                                                 const int32_t* __restrict__ cols,
+                                                 const VT* __restrict__ vals,
+                                                 const XT* __restrict__ x, XT* __restrict__ y,
+                                                 const double2* __restrict__ b, double* part) {
+  if (MODE != 0 && s->done) return;
+  constexpr int EPB = kBS / G;
+  int64_t lo, hi;
+  chunk(nent, EPB, &lo, &hi);
+  const int lane = threadIdx.x % G;
+  double red[2] = {0.0, 0.0};
+  for (int64_t ent = lo + threadIdx.x / G; ent < hi; ent += EPB) {
+    const int r0 = bptr[ent], d = bptr[ent + 1] - r0;
+    const int base = rowptr[r0];
+    const int L = rowptr[r0 + 1] - base;
+    XT acc[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc[k] = czero<XT>();
+    for (int tb = lane; tb < L; tb += G * U) {
+      XT xv[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int t = tb + j * G;
+        xv[j] = t < L ? x[cols[base + t]] : czero<XT>();
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        if (k < d)
+#pragma unroll
+          for (int j = 0; j < U; ++j) {
+            const int t = tb + j * G;
+            const VT a = t < L ? vals[base + k * L + t] : VT{0, 0};
+            acc[k] = cfma(XT{a.x, a.y}, xv[j], acc[k]);
+          }
     }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      acc[k].x = group_sum<G>(acc[k].x);
+      acc[k].y = group_sum<G>(acc[k].y);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        if (k < d) {
+          const int64_t row = r0 + k;
+          if (MODE == 2) {
+            const double2 bb = b[row];
+            const double2 rr = make_double2(bb.x - (double)acc[k].x, bb.y - (double)acc[k].y);
+            y[row] = from_d<XT>(rr);
+            red[0] += rr.x * rr.x + rr.y * rr.y;
+          } else {
+            y[row] = acc[k];
+            if (MODE == 1) {
+              const double2 pv = to_d(x[row]), qv = to_d(acc[k]);
+              red[0] += pv.x * qv.x - pv.y * qv.y;
+              red[1] += pv.x * qv.y + pv.y * qv.x;
+            }
+          }
+        }
+    }
+  }
+  if (MODE != 0) write_partials<2>(red, part);
+}
+
+// finish mu = p^T q, alpha = rho / mu
+__global__ void __launch_bounds__(kBS) k_fin_mu(Scal* __restrict__ s, const double* part,
+                                                int np) {
+  if (s->done) return;
+  double tot[2];
+  sum_partials<2>(part, np, tot);
+  if (threadIdx.x != 0) return;
+  const double2 mu = make_double2(tot[0], tot[1]);
+  s->mu = mu;
+  const double2 al = zdiv(s->rho, mu);
+  if (!(mu.x != 0.0 || mu.y != 0.0) || !isfinite(al.x) || !isfinite(al.y)) {
+    s->status = PG_ERR_BREAKDOWN;
+    s->done = 1;
+    s->alpha = make_double2(0.0, 0.0);
   } else {
-    s->res64 = tot[0];
+    s->alpha = al;
   }
 }
 
-// K2: x += alpha p; r -= alpha q; z = Dinv r; rho = r^T z; |r|^2.
+// finish |b - A x|^2
+__global__ void __launch_bounds__(kBS) k_fin_res64(Scal* __restrict__ s, const double* part,
+                                                   int np) {
+  if (s->done) return;
+  double tot[1];
+  sum_partials<1>(part, np, tot);
+  if (threadIdx.x == 0) s->res64 = tot[0];
+}
+
+// ---------------------------------------------------------------------------
+// COCG vector kernels
+// ---------------------------------------------------------------------------
+// x += alpha p; r -= alpha q; z = Dinv r; partials of r^T z and |r|^2
 template <typename XT>
-__global__ void __launch_bounds__(kBS) k_cocg_update(Scal* __restrict__ s, int64_t n,
-                                                     XT* __restrict__ x, XT* __restrict__ r,
-                                                     const XT* __restrict__ p,
-                                                     const XT* __restrict__ q,
-                                                     const XT* __restrict__ dinv,
-                                                     XT* __restrict__ z, double* part,
-                                                     int stop_on_converged) {
+__global__ void __launch_bounds__(kBS) k_update(const Scal* __restrict__ s, int64_t n,
+                                                XT* __restrict__ x, XT* __restrict__ r,
+                                                const XT* __restrict__ p,
+                                                const XT* __restrict__ q,
+                                                const XT* __restrict__ dinv,
+                                                XT* __restrict__ z, double* part) {
   if (s->done) return;
   const double2 al = s->alpha;
   const XT a = from_d<XT>(al);
@@ -236,8 +323,7 @@ __global__ void __launch_bounds__(kBS) k_cocg_update(Scal* __restrict__ s, int64
   chunk(n, kBS, &lo, &hi);
   double red[3] = {0.0, 0.0, 0.0};
   for (int64_t i = lo + threadIdx.x; i < hi; i += kBS) {
-    const XT pi = p[i];
-    x[i] = cfma(a, pi, x[i]);
+    x[i] = cfma(a, p[i], x[i]);
     const XT ri = cfma(na, q[i], r[i]);
     r[i] = ri;
     const XT zi = cm(dinv[i], ri);
@@ -247,8 +333,58 @@ __global__ void __launch_bounds__(kBS) k_cocg_update(Scal* __restrict__ s, int64
     red[1] += rd.x * zd.y + rd.y * zd.x;
     red[2] += rd.x * rd.x + rd.y * rd.y;
   }
+  write_partials<3>(red, part);
+}
+
+// same with the entity-block preconditioner: one thread per block
+template <typename XT>
+__global__ void __launch_bounds__(kBS) k_update_blk(
+    const Scal* __restrict__ s, int64_t nb, const int32_t* __restrict__ bptr,
+    const int32_t* __restrict__ boff, XT* __restrict__ x, XT* __restrict__ r,
+    const XT* __restrict__ p, const XT* __restrict__ q, const XT* __restrict__ binv,
+    XT* __restrict__ z, double* part) {
+  if (s->done) return;
+  const double2 al = s->alpha;
+  const XT a = from_d<XT>(al);
+  const XT na = from_d<XT>(make_double2(-al.x, -al.y));
+  int64_t lo, hi;
+  chunk(nb, kBS, &lo, &hi);
+  double red[3] = {0.0, 0.0, 0.0};
+  for (int64_t k = lo + threadIdx.x; k < hi; k += kBS) {
+    const int i0 = bptr[k], d = bptr[k + 1] - i0;
+    const XT* B = binv + boff[k];
+    XT rl[kMaxBlock];
+#pragma unroll
+    for (int t = 0; t < kMaxBlock; ++t)
+      if (t < d) {
+        const int i = i0 + t;
+        x[i] = cfma(a, p[i], x[i]);
+        rl[t] = cfma(na, q[i], r[i]);
+        r[i] = rl[t];
+      }
+#pragma unroll
+    for (int t = 0; t < kMaxBlock; ++t)
+      if (t < d) {
+        XT zt = czero<XT>();
+#pragma unroll
+        for (int u = 0; u < kMaxBlock; ++u)
+          if (u < d) zt = cfma(B[t * d + u], rl[u], zt);
+        z[i0 + t] = zt;
+        const double2 rd = to_d(rl[t]), zd = to_d(zt);
+        red[0] += rd.x * zd.x - rd.y * zd.y;
+        red[1] += rd.x * zd.y + rd.y * zd.x;
+        red[2] += rd.x * rd.x + rd.y * rd.y;
+      }
+  }
+  write_partials<3>(red, part);
+}
+
+// finish rho, beta, |r|^2 and the convergence / pause decision
+__global__ void __launch_bounds__(kBS) k_fin_update(Scal* __restrict__ s, const double* part,
+                                                    int np) {
+  if (s->done) return;
   double tot[3];
-  if (!last_block_total<3>(red, part, &s->counter[1], tot)) return;
+  sum_partials<3>(part, np, tot);
   if (threadIdx.x != 0) return;
   const double2 rho_new = make_double2(tot[0], tot[1]);
   const double2 beta = zdiv(rho_new, s->rho);
@@ -259,19 +395,15 @@ __global__ void __launch_bounds__(kBS) k_cocg_update(Scal* __restrict__ s, int64
   if (!isfinite(tot[2]) || !isfinite(beta.x) || !isfinite(beta.y)) {
     s->status = PG_ERR_BREAKDOWN;
     s->done = 1;
-  } else if (tot[2] <= s->tol2 * s->bnorm2) {
-    if (stop_on_converged) s->done = 1;
-  } else if (s->iter >= s->maxit) {
-    s->done = 1;
+  } else if (tot[2] <= s->tol2 * s->bnorm2 || tot[2] <= s->ref2 || s->iter >= s->maxit) {
+    s->done = 1;  // converged recursion, refinement point, or iteration limit
   }
-  if (stop_on_converged && tot[2] <= s->ref2) s->done = 1;
 }
 
-// K3: p = z + beta p
+// p = z + beta p
 template <typename XT>
-__global__ void __launch_bounds__(kBS) k_cocg_pupdate(Scal* __restrict__ s, int64_t n,
-                                                      const XT* __restrict__ z,
-                                                      XT* __restrict__ p) {
+__global__ void __launch_bounds__(kBS) k_pupdate(const Scal* __restrict__ s, int64_t n,
+                                                 const XT* __restrict__ z, XT* __restrict__ p) {
   if (s->done) return;
   const XT be = from_d<XT>(s->beta);
   int64_t lo, hi;
@@ -279,17 +411,12 @@ __global__ void __launch_bounds__(kBS) k_cocg_pupdate(Scal* __restrict__ s, int6
   for (int64_t i = lo + threadIdx.x; i < hi; i += kBS) p[i] = cfma(be, p[i], z[i]);
 }
 
-// init / residual replacement: r = demote(r64) (or r64 itself), z = Dinv r,
-// rho_new = r^T z, |r|^2; optionally |b|^2.  With `restart` the recursion
-// restarts (beta = 0, p = 0); otherwise beta = rho_new / rho_old.
+// (re)start: r = r64 (demoted), z = P r, p = 0; partials of r^T z, |r|^2, |b|^2
 template <typename XT>
-__global__ void __launch_bounds__(kBS) k_cocg_reset(Scal* __restrict__ s, int64_t n,
-                                                    const double2* __restrict__ r64,
-                                                    XT* __restrict__ r,
-                                                    const XT* __restrict__ dinv,
-                                                    XT* __restrict__ z, XT* __restrict__ p,
-                                                    const double2* __restrict__ b,
-                                                    double* part, int restart) {
+__global__ void __launch_bounds__(kBS) k_reset(int64_t n, const double2* __restrict__ r64,
+                                               XT* __restrict__ r, const XT* __restrict__ dinv,
+                                               XT* __restrict__ z, XT* __restrict__ p,
+                                               const double2* __restrict__ b, double* part) {
   int64_t lo, hi;
   chunk(n, kBS, &lo, &hi);
   double red[4] = {0.0, 0.0, 0.0, 0.0};
@@ -298,7 +425,7 @@ __global__ void __launch_bounds__(kBS) k_cocg_reset(Scal* __restrict__ s, int64_
     r[i] = ri;
     const XT zi = cm(dinv[i], ri);
     z[i] = zi;
-    if (restart) p[i] = czero<XT>();
+    p[i] = czero<XT>();
     const double2 rd = to_d(ri), zd = to_d(zi);
     red[0] += rd.x * zd.x - rd.y * zd.y;
     red[1] += rd.x * zd.y + rd.y * zd.x;
@@ -308,18 +435,63 @@ __global__ void __launch_bounds__(kBS) k_cocg_reset(Scal* __restrict__ s, int64_
       red[3] += bb.x * bb.x + bb.y * bb.y;
     }
   }
-  double tot[4];
-  if (!last_block_total<4>(red, part, &s->counter[2], tot)) return;
-  if (threadIdx.x != 0) return;
-  const double2 rho_new = make_double2(tot[0], tot[1]);
-  if (restart) {
-    s->beta = make_double2(0.0, 0.0);
-  } else {
-    s->beta = zdiv(rho_new, s->rho);
+  write_partials<4>(red, part);
+}
+
+template <typename XT>
+__global__ void __launch_bounds__(kBS) k_reset_blk(int64_t nb, const int32_t* __restrict__ bptr,
+                                                   const int32_t* __restrict__ boff,
+                                                   const double2* __restrict__ r64,
+                                                   XT* __restrict__ r,
+                                                   const XT* __restrict__ binv,
+                                                   XT* __restrict__ z, XT* __restrict__ p,
+                                                   const double2* __restrict__ b, double* part) {
+  int64_t lo, hi;
+  chunk(nb, kBS, &lo, &hi);
+  double red[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t k = lo + threadIdx.x; k < hi; k += kBS) {
+    const int i0 = bptr[k], d = bptr[k + 1] - i0;
+    const XT* B = binv + boff[k];
+    XT rl[kMaxBlock];
+#pragma unroll
+    for (int t = 0; t < kMaxBlock; ++t)
+      if (t < d) {
+        const int i = i0 + t;
+        rl[t] = from_d<XT>(r64[i]);
+        r[i] = rl[t];
+        p[i] = czero<XT>();
+        if (b) {
+          const double2 bb = b[i];
+          red[3] += bb.x * bb.x + bb.y * bb.y;
+        }
+      }
+#pragma unroll
+    for (int t = 0; t < kMaxBlock; ++t)
+      if (t < d) {
+        XT zt = czero<XT>();
+#pragma unroll
+        for (int u = 0; u < kMaxBlock; ++u)
+          if (u < d) zt = cfma(B[t * d + u], rl[u], zt);
+        z[i0 + t] = zt;
+        const double2 rd = to_d(rl[t]), zd = to_d(zt);
+        red[0] += rd.x * zd.x - rd.y * zd.y;
+        red[1] += rd.x * zd.y + rd.y * zd.x;
+        red[2] += rd.x * rd.x + rd.y * rd.y;
+      }
   }
-  s->rho = rho_new;
+  write_partials<4>(red, part);
+}
+
+// finish a (re)start: rho, |r|^2 (and |b|^2); beta = 0, resume iterating
+__global__ void __launch_bounds__(kBS) k_fin_reset(Scal* __restrict__ s, const double* part,
+                                                   int np, int has_b) {
+  double tot[4];
+  sum_partials<4>(part, np, tot);
+  if (threadIdx.x != 0) return;
+  s->rho = make_double2(tot[0], tot[1]);
+  s->beta = make_double2(0.0, 0.0);
   s->res2 = tot[2];
-  if (b) s->bnorm2 = tot[3];
+  if (has_b) s->bnorm2 = tot[3];
   s->done = 0;
 }
 
@@ -332,6 +504,9 @@ __global__ void k_accumulate(int64_t n, double2* __restrict__ x, float2* __restr
   c[i] = make_float2(0.f, 0.f);
 }
 
+// ---------------------------------------------------------------------------
+// preconditioners
+// ---------------------------------------------------------------------------
 template <typename VT>
 __global__ void k_jacobi_inv(int64_t n, const int32_t* __restrict__ rowptr,
                              const int32_t* __restrict__ cols, const double2* __restrict__ vals,
@@ -358,11 +533,6 @@ __global__ void k_jacobi_inv(int64_t n, const int32_t* __restrict__ rowptr,
   dinv[r] = from_d<VT>(zdiv(make_double2(1.0, 0.0), d));
 }
 
-__global__ void k_demote(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (float)a[i];
-}
-
 __global__ void k_ones(int64_t n, double2* __restrict__ d) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = make_double2(1.0, 0.0);
@@ -371,130 +541,126 @@ __global__ void k_ones32(int64_t n, float2* __restrict__ d) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) d[i] = make_float2(1.f, 0.f);
 }
+__global__ void k_demote(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
 
-static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// entity-block Jacobi set-up: one thread per block gathers A_BB from the CSR and
+// inverts it (Gauss-Jordan with partial pivoting); row-major at boff[k]
+__global__ void k_bjacobi_setup(int64_t nb, const int32_t* __restrict__ bptr,
+                                const int32_t* __restrict__ boff,
+                                const int32_t* __restrict__ rowptr,
+                                const int32_t* __restrict__ cols,
+                                const double2* __restrict__ vals, double2* __restrict__ binv,
+                                int32_t* __restrict__ bad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const int i0 = bptr[k], d = bptr[k + 1] - i0;
+  if (d <= 0 || d > kMaxBlock) {
+    bad[0] = 1;
+    return;
+  }
+  double2 a[kMaxBlock][kMaxBlock], inv[kMaxBlock][kMaxBlock];
+  for (int t = 0; t < d; ++t) {
+    const int row = i0 + t;
+    int lo = rowptr[row], hi = rowptr[row + 1] - 1, pos = -1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int c = cols[mid];
+      if (c == i0) {
+        pos = mid;
+        break;
+      }
+      if (c < i0) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    if (pos < 0 || pos + d > rowptr[row + 1] || cols[pos + d - 1] != i0 + d - 1) {
+      bad[0] = 1;
+      return;
+    }
+    for (int u = 0; u < d; ++u) {
+      a[t][u] = vals[pos + u];
+      inv[t][u] = make_double2(t == u ? 1.0 : 0.0, 0.0);
+    }
+  }
+  for (int c = 0; c < d; ++c) {
+    int piv = c;
+    double best = hypot(a[c][c].x, a[c][c].y);
+    for (int t = c + 1; t < d; ++t) {
+      const double v = hypot(a[t][c].x, a[t][c].y);
+      if (v > best) {
+        best = v;
+        piv = t;
+      }
+    }
+    if (!(best > 0.0)) {
+      bad[0] = 1;
+      return;
+    }
+    if (piv != c)
+      for (int u = 0; u < d; ++u) {
+        const double2 t1 = a[c][u], t2 = inv[c][u];
+        a[c][u] = a[piv][u];
+        inv[c][u] = inv[piv][u];
+        a[piv][u] = t1;
+        inv[piv][u] = t2;
+      }
+    const double2 ip = zdiv(make_double2(1.0, 0.0), a[c][c]);
+    for (int u = 0; u < d; ++u) {
+      a[c][u] = zmul(a[c][u], ip);
+      inv[c][u] = zmul(inv[c][u], ip);
+    }
+    for (int t = 0; t < d; ++t) {
+      if (t == c) continue;
+      const double2 f = a[t][c];
+      for (int u = 0; u < d; ++u) {
+        a[t][u] = zsub(a[t][u], zmul(f, a[c][u]));
+        inv[t][u] = zsub(inv[t][u], zmul(f, inv[c][u]));
+      }
+    }
+  }
+  double2* out = binv + boff[k];
+  for (int t = 0; t < d; ++t)
+    for (int u = 0; u < d; ++u) out[t * d + u] = inv[t][u];
+}
 
-// ---------------------------------------------------------------------------
-// host-side dispatch
-// ---------------------------------------------------------------------------
-template <int MODE, typename VT, typename XT>
-static void launch_spmv(int G, int grid, hipStream_t st, Scal* s, int64_t n, const int32_t* rp,
-                        const int32_t* ci, const VT* v, const XT* x, XT* y, const double2* b,
-                        double* part) {
-  const dim3 g(grid), bl(kBS);
-  switch (G) {
-    case 8: hipLaunchKernelGGL((k_spmv<8, MODE, VT, XT>), g, bl, 0, st, s, n, rp, ci, v, x, y, b, part); break;
-    case 16: hipLaunchKernelGGL((k_spmv<16, MODE, VT, XT>), g, bl, 0, st, s, n, rp, ci, v, x, y, b, part); break;
-    case 32: hipLaunchKernelGGL((k_spmv<32, MODE, VT, XT>), g, bl, 0, st, s, n, rp, ci, v, x, y, b, part); break;
-    default: hipLaunchKernelGGL((k_spmv<64, MODE, VT, XT>), g, bl, 0, st, s, n, rp, ci, v, x, y, b, part); break;
+// z = P v (block), one thread per block
+__global__ void k_bmul(int64_t nb, const int32_t* __restrict__ bptr,
+                       const int32_t* __restrict__ boff, const double2* __restrict__ binv,
+                       const double2* __restrict__ v, double2* __restrict__ z) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const int i0 = bptr[k], d = bptr[k + 1] - i0;
+  const double2* B = binv + boff[k];
+  for (int t = 0; t < d; ++t) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int u = 0; u < d; ++u) acc = zadd(acc, zmul(B[t * d + u], v[i0 + u]));
+    z[i0 + t] = acc;
   }
 }
 
-static int grid_for(int64_t n, int G) {
-  const int64_t rpb = kBS / G;
-  int64_t g = (n + rpb - 1) / rpb;
-  g = std::min<int64_t>(g, kMaxGrid);
-  if (g >= 64) g = (g / 8) * 8;
-  return (int)std::max<int64_t>(g, 1);
-}
-static int vgrid_for(int64_t n) {
-  int64_t g = (n + kBS - 1) / kBS;
-  g = std::min<int64_t>(g, kMaxGrid);
-  if (g >= 64) g = (g / 8) * 8;
-  return (int)std::max<int64_t>(g, 1);
-}
-
-struct Solver {
-  int64_t n = 0, nnz = 0;
-  pg_solve_opts o{};
-  const int32_t* rowptr = nullptr;
-  const int32_t* cols = nullptr;
-  const double2* vals = nullptr;  // caller's complex128 values
-  float2* vals32 = nullptr;       // owned complex64 copy (mixed)
-  int G64 = 16, G32 = 16;
-  int gs64 = 1, gs32 = 1, gv = 1;
-  Scal* s = nullptr;
-  Scal* h = nullptr;  // pinned host mirror
-  double* part = nullptr;
-  // vectors
-  double2 *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *dinv = nullptr, *r64 = nullptr;
-  float2 *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr, *x32 = nullptr,
-         *dinv32 = nullptr;
-  int32_t* bad = nullptr;
-  // GMRES(m): Krylov basis V[(m+1) x n], Hessenberg column buffer, partials
-  double2* V = nullptr;
-  double2* hdev = nullptr;
-  double2* hhost = nullptr;  // pinned
-  double* partg = nullptr;
-  int restart = 30;
-  std::vector<void*> owned;
-
-  template <typename T>
-  int alloc(T** ptr, size_t count) {
-    void* v = nullptr;
-    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
-    if (e != hipSuccess) PG_FAIL(PG_ERR_HIP, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
-    owned.push_back(v);
-    *ptr = static_cast<T*>(v);
-    return PG_OK;
-  }
-  ~Solver() {
-    for (void* v : owned) hipFree(v);
-    if (h) hipHostFree(h);
-    if (hhost) hipHostFree(hhost);
-  }
-};
-
-#define PG_TRY(x)                \
-  do {                           \
-    int _rc = (x);               \
-    if (_rc != PG_OK) return _rc; \
-  } while (0)
-
-// time one SpMV configuration (milliseconds, averaged)
-template <typename VT, typename XT>
-static float time_spmv(Solver* S, int G, hipStream_t st, const VT* v, const XT* x, XT* y) {
-  hipEvent_t a, b;
-  hipEventCreate(&a);
-  hipEventCreate(&b);
-  const int grid = grid_for(S->n, G);
-  launch_spmv<0, VT, XT>(G, grid, st, S->s, S->n, S->rowptr, S->cols, v, x, y, nullptr, S->part);
-  hipEventRecord(a, st);
-  const int reps = 5;
-  for (int i = 0; i < reps; ++i)
-    launch_spmv<0, VT, XT>(G, grid, st, S->s, S->n, S->rowptr, S->cols, v, x, y, nullptr,
-                           S->part);
-  hipEventRecord(b, st);
-  hipEventSynchronize(b);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, a, b);
-  hipEventDestroy(a);
-  hipEventDestroy(b);
-  return ms / reps;
+// entity rows must be consecutive rows of equal length sharing one column list
+__global__ void k_check_entity_rows(int64_t nent, const int32_t* __restrict__ bptr,
+                                    const int32_t* __restrict__ rowptr,
+                                    const int32_t* __restrict__ cols, int32_t* bad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nent) return;
+  const int r0 = bptr[e], d = bptr[e + 1] - r0;
+  const int base = rowptr[r0], L = rowptr[r0 + 1] - base;
+  for (int k = 1; k <= d; ++k)
+    if (rowptr[r0 + k] != base + k * L) {
+      bad[0] = 1;
+      return;
+    }
+  for (int k = 1; k < d; ++k)
+    for (int t = 0; t < L; ++t)
+      if (cols[base + k * L + t] != cols[base + t]) {
+        bad[0] = 1;
+        return;
+      }
 }
 
-static int read_scalars(Solver* S, hipStream_t st) {
-  PG_HIP(hipMemcpyAsync(S->h, S->s, sizeof(Scal), hipMemcpyDeviceToHost, st));
-  PG_HIP(hipStreamSynchronize(st));
-  return PG_OK;
-}
-
-// true FP64 residual r64 = b - A x, returns |r|^2 in S->h->res64
-static int true_residual(Solver* S, hipStream_t st, const double2* b, const double2* x) {
-  // MODE 2 checks s->done; clear it for this launch through a tiny kernel-free path:
-  // the residual kernel is only launched when done == 0 on the device.
-  launch_spmv<2, double2, double2>(S->G64, S->gs64, st, S->s, S->n, S->rowptr, S->cols, S->vals,
-                                   x, S->r64, b, S->part);
-  PG_CHECK_LAUNCH();
-  return PG_OK;
-}
-
-__global__ void k_set_done(Scal* s, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) s->done = v;
-}
-__global__ void k_set_ref2(Scal* s, double v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) s->ref2 = v;
-}
 __global__ void k_init_scalars(Scal* s, int maxit, double tol2) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Scal z{};
@@ -502,113 +668,20 @@ __global__ void k_init_scalars(Scal* s, int maxit, double tol2) {
   z.tol2 = tol2;
   *s = z;
 }
-
-static int solve(Solver* S, const double2* b, double2* x, pg_solve_info* info, hipStream_t st) {
-  const bool mixed = S->o.precision == PG_PREC_MIXED;
-  const int64_t n = S->n;
-  hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, S->o.maxit,
-                     S->o.rtol * S->o.rtol);
-  if (mixed) PG_HIP(hipMemsetAsync(S->x32, 0, n * sizeof(float2), st));
-  // r64 = b - A x; bnorm
-  PG_TRY(true_residual(S, st, b, x));
-  if (mixed)
-    hipLaunchKernelGGL(k_cocg_reset<float2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->r64,
-                       S->r32, S->dinv32, S->z32, S->p32, b, S->part, 1);
-  else
-    hipLaunchKernelGGL(k_cocg_reset<double2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->r64,
-                       S->r, S->dinv, S->z, S->p, b, S->part, 1);
-  PG_CHECK_LAUNCH();
-  PG_TRY(read_scalars(S, st));
-  const double bnorm2 = S->h->bnorm2;
-  const double tol2 = S->o.rtol * S->o.rtol;
-  int refinements = 0, restarts = 0;
-  info->status = -1;
-  if (bnorm2 == 0.0) {
-    PG_HIP(hipMemsetAsync(x, 0, n * sizeof(double2), st));
-    info->iterations = 0;
-    info->refinements = 0;
-    info->relres = 0.0;
-    info->status = PG_OK;
-    return PG_OK;
-  }
-  double res64 = S->h->res2;
-  if (res64 <= tol2 * bnorm2) {
-    info->iterations = 0;
-    info->refinements = 0;
-    info->relres = std::sqrt(res64 / bnorm2);
-    info->status = PG_OK;
-    return PG_OK;
-  }
-  // residual-replacement threshold for mixed precision (squared, relative to last ref)
-  const double delta2 = S->o.inner_rtol * S->o.inner_rtol;
-  if (mixed) hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
-  const int check = std::max(1, S->o.check_every);
-  while (true) {
-    for (int k = 0; k < check; ++k) {
-      if (mixed) {
-        hipLaunchKernelGGL(k_cocg_pupdate<float2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->z32, S->p32);
-        launch_spmv<1, float2, float2>(S->G32, S->gs32, st, S->s, n, S->rowptr, S->cols, S->vals32, S->p32, S->q32, nullptr, S->part);
-        hipLaunchKernelGGL(k_cocg_update<float2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->x32, S->r32, S->p32, S->q32, S->dinv32, S->z32, S->part, 1);
-      } else {
-        hipLaunchKernelGGL(k_cocg_pupdate<double2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->z, S->p);
-        launch_spmv<1, double2, double2>(S->G64, S->gs64, st, S->s, n, S->rowptr, S->cols, S->vals, S->p, S->q, nullptr, S->part);
-        hipLaunchKernelGGL(k_cocg_update<double2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, x, S->r, S->p, S->q, S->dinv, S->z, S->part, 1);
-      }
-    }
-    PG_CHECK_LAUNCH();
-    PG_TRY(read_scalars(S, st));
-    Scal hs = *S->h;
-    if (hs.status == PG_ERR_BREAKDOWN) {
-      info->status = PG_ERR_BREAKDOWN;
-      break;
-    }
-    if (!hs.done) continue;
-    // paused: converged recursion, replacement threshold, or maxit
-    if (mixed) {
-      hipLaunchKernelGGL(k_accumulate, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, x, S->x32);
-      refinements++;
-    }
-    hipLaunchKernelGGL(k_set_done, dim3(1), dim3(64), 0, st, S->s, 0);
-    PG_TRY(true_residual(S, st, b, x));
-    PG_TRY(read_scalars(S, st));
-    res64 = S->h->res64;
-    if (res64 <= tol2 * bnorm2) {
-      info->status = PG_OK;
-      break;
-    }
-    if (hs.iter >= S->o.maxit) break;
-    if (!mixed) {
-      // recursive residual converged but the true one did not: restart from x
-      if (++restarts > 20) break;
-      hipLaunchKernelGGL(k_cocg_reset<double2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->r64,
-                         S->r, S->dinv, S->z, S->p, (const double2*)nullptr, S->part, 1);
-    } else {
-      // restart the complex64 recursion from the FP64 residual (keeping the old
-      // search direction across a residual replacement stalls COCG)
-      hipLaunchKernelGGL(k_cocg_reset<float2>, dim3(S->gv), dim3(kBS), 0, st, S->s, n, S->r64,
-                         S->r32, S->dinv32, S->z32, S->p32, (const double2*)nullptr, S->part, 1);
-      hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
-    }
-    PG_CHECK_LAUNCH();
-  }
-  PG_TRY(read_scalars(S, st));
-  info->iterations = S->h->iter;
-  info->refinements = refinements;
-  info->relres = std::sqrt(res64 / bnorm2);
-  return PG_OK;
+__global__ void k_set_done(Scal* s, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->done = v;
 }
-
+__global__ void k_set_ref2(Scal* s, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->ref2 = v;
+}
 
 // ---------------------------------------------------------------------------
 // GMRES(m) kernels (complex128, Hermitian inner products)
 // ---------------------------------------------------------------------------
-constexpr int kMaxRestart = 128;
-
-// h[i] = V_i^H w, i < k; fixed-order last-block reduction into h_out
+// partials of V_i^H w for i < k: part[(2i+c) * grid + block]
 __global__ void __launch_bounds__(kBS) k_mdot(int64_t n, const double2* __restrict__ V,
                                               int64_t ldv, int k, const double2* __restrict__ w,
-                                              double* part, uint32_t* counter,
-                                              double2* __restrict__ h_out) {
+                                              double* part) {
   int64_t lo, hi;
   chunk(n, kBS, &lo, &hi);
   const int64_t G = gridDim.x;
@@ -625,37 +698,23 @@ __global__ void __launch_bounds__(kBS) k_mdot(int64_t n, const double2* __restri
       part[(2 * i) * G + blockIdx.x] = red[0];
       part[(2 * i + 1) * G + blockIdx.x] = red[1];
     }
-    __syncthreads();
   }
-  __shared__ int am_last;
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t prev =
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    am_last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!am_last) return;
-  __threadfence();
-  for (int i = 0; i < k; ++i) {
-    double red[2] = {0.0, 0.0};
-    for (int64_t b = threadIdx.x; b < G; b += kBS) {
-      red[0] += __hip_atomic_load(part + (2 * i) * G + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      red[1] += __hip_atomic_load(part + (2 * i + 1) * G + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_reduce<2>(red);
-    if (threadIdx.x == 0) h_out[i] = make_double2(red[0], red[1]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// w -= sum_i coef[i] V_i ; optional |w|^2 into coef_out (last block)
+// one workgroup per vector i: h[i] = sum of its partials (fixed order)
+__global__ void __launch_bounds__(kBS) k_fin_mdot(const double* part, int np,
+                                                  double2* __restrict__ h) {
+  const int i = blockIdx.x;
+  double tot[2];
+  sum_partials<2>(part + (int64_t)(2 * i) * np, np, tot);
+  if (threadIdx.x == 0) h[i] = make_double2(tot[0], tot[1]);
+}
+
+// w -= sum_i coef[i] V_i ; partials of |w|^2
 __global__ void __launch_bounds__(kBS) k_maxpy_norm(int64_t n, const double2* __restrict__ V,
                                                     int64_t ldv, int k,
                                                     const double2* __restrict__ coef,
-                                                    double2* __restrict__ w, double* part,
-                                                    uint32_t* counter, double* norm_out) {
+                                                    double2* __restrict__ w, double* part) {
   int64_t lo, hi;
   chunk(n, kBS, &lo, &hi);
   double red[1] = {0.0};
@@ -669,33 +728,336 @@ __global__ void __launch_bounds__(kBS) k_maxpy_norm(int64_t n, const double2* __
     w[j] = acc;
     red[0] += acc.x * acc.x + acc.y * acc.y;
   }
-  double tot[1];
-  if (!last_block_total<1>(red, part, counter, tot)) return;
-  if (threadIdx.x == 0) *norm_out = tot[0];
+  write_partials<1>(red, part);
 }
 
-// out = dinv * v
+__global__ void __launch_bounds__(kBS) k_fin_sum1(const double* part, int np, double* out) {
+  double tot[1];
+  sum_partials<1>(part, np, tot);
+  if (threadIdx.x == 0) *out = tot[0];
+}
+
 __global__ void k_dmul(int64_t n, const double2* __restrict__ d, const double2* __restrict__ v,
                        double2* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = zmul(d[i], v[i]);
 }
-
-// v *= s (real)
 __global__ void k_rscale(int64_t n, double2* __restrict__ v, double s) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = make_double2(v[i].x * s, v[i].y * s);
 }
-
-// x += dinv * sum_i y_i V_i
-__global__ void k_gmres_update(int64_t n, double2* __restrict__ x, const double2* __restrict__ d,
-                               const double2* __restrict__ V, int64_t ldv, int k,
-                               const double2* __restrict__ y) {
+__global__ void k_vcomb(int64_t n, const double2* __restrict__ V, int64_t ldv, int k,
+                        const double2* __restrict__ y, double2* __restrict__ z) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   double2 acc = make_double2(0.0, 0.0);
   for (int i = 0; i < k; ++i) acc = zadd(acc, zmul(y[i], V[(int64_t)i * ldv + j]));
-  x[j] = zadd(x[j], zmul(d[j], acc));
+  z[j] = acc;
+}
+__global__ void k_axpy1(int64_t n, const double2* __restrict__ v, double2* __restrict__ x) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) x[j] = zadd(x[j], v[j]);
+}
+
+static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// SpMV configurations and dispatch
+// ---------------------------------------------------------------------------
+struct SpmvCfg {
+  int kind = 0;  // 0 = row, 1 = entity-row
+  int G = 16, U = 2, D = 1;
+  int grid = 1;
+};
+
+static int grid_for(int64_t n, int G) {
+  const int64_t per = kBS / G;
+  int64_t g = (n + per - 1) / per;
+  g = std::min<int64_t>(g, kMaxGrid);
+  if (g >= 64) g = (g / 8) * 8;
+  return (int)std::max<int64_t>(g, 1);
+}
+static int vgrid_for(int64_t n) { return grid_for(n, 1); }
+
+// the (G, U) pairs instantiated for both kernels
+#define PG_GU_LIST(X) X(8, 4) X(16, 2) X(16, 4) X(32, 1) X(32, 2) X(64, 1)
+
+template <int MODE, typename VT, typename XT>
+static void spmv_row(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t n,
+                     const int32_t* rp, const int32_t* ci, const VT* v, const XT* x, XT* y,
+                     const double2* b, double* part) {
+  const dim3 g(c.grid), bl(kBS);
+#define PG_ROW(GG, UU)                                                                         \
+  if (c.G == GG && c.U == UU) {                                                                \
+    hipLaunchKernelGGL((k_spmv<GG, UU, MODE, VT, XT>), g, bl, 0, st, s, n, rp, ci, v, x, y, b, \
+                       part);                                                                  \
+    return;                                                                                    \
+  }
+  PG_GU_LIST(PG_ROW)
+#undef PG_ROW
+  hipLaunchKernelGGL((k_spmv<16, 2, MODE, VT, XT>), g, bl, 0, st, s, n, rp, ci, v, x, y, b, part);
+}
+
+template <int MODE, typename VT, typename XT, int D>
+static void spmv_er_d(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t nent,
+                      const int32_t* bp, const int32_t* rp, const int32_t* ci, const VT* v,
+                      const XT* x, XT* y, const double2* b, double* part) {
+  const dim3 g(c.grid), bl(kBS);
+#define PG_ER(GG, UU)                                                                       \
+  if (c.G == GG && c.U == UU) {                                                             \
+    hipLaunchKernelGGL((k_spmv_er<GG, UU, D, MODE, VT, XT>), g, bl, 0, st, s, nent, bp, rp, \
+                       ci, v, x, y, b, part);                                               \
+    return;                                                                                 \
+  }
+  PG_GU_LIST(PG_ER)
+#undef PG_ER
+  hipLaunchKernelGGL((k_spmv_er<16, 2, D, MODE, VT, XT>), g, bl, 0, st, s, nent, bp, rp, ci, v, x,
+                     y, b, part);
+}
+
+template <int MODE, typename VT, typename XT>
+static void spmv_launch(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t n, int64_t nent,
+                        const int32_t* bp, const int32_t* rp, const int32_t* ci, const VT* v,
+                        const XT* x, XT* y, const double2* b, double* part) {
+  if (c.kind == 0) {
+    spmv_row<MODE, VT, XT>(c, st, s, n, rp, ci, v, x, y, b, part);
+  } else if (c.D == 1) {
+    spmv_er_d<MODE, VT, XT, 1>(c, st, s, nent, bp, rp, ci, v, x, y, b, part);
+  } else if (c.D == 2) {
+    spmv_er_d<MODE, VT, XT, 2>(c, st, s, nent, bp, rp, ci, v, x, y, b, part);
+  } else {
+    spmv_er_d<MODE, VT, XT, 6>(c, st, s, nent, bp, rp, ci, v, x, y, b, part);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// solver object
+// ---------------------------------------------------------------------------
+struct Solver {
+  int64_t n = 0, nnz = 0;
+  pg_solve_opts o{};
+  const int32_t* rowptr = nullptr;
+  const int32_t* cols = nullptr;
+  const double2* vals = nullptr;  // caller's complex128 values
+  float2* vals32 = nullptr;       // owned complex64 copy (mixed)
+  SpmvCfg c64, c32;               // chosen SpMV kernels
+  int gv = 1;
+  Scal* s = nullptr;
+  Scal* h = nullptr;  // pinned host mirror
+  double* part = nullptr;
+  double2 *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *dinv = nullptr, *r64 = nullptr;
+  float2 *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr, *x32 = nullptr,
+         *dinv32 = nullptr;
+  int32_t* bad = nullptr;
+  // mesh entities (DOF ranges): entity-row SpMV and entity-block Jacobi
+  int64_t nblocks = 0;
+  int gb = 1, maxblk = 0;
+  bool er_ok = false;
+  int32_t *bptr = nullptr, *boff = nullptr;
+  double2* binv = nullptr;
+  float2* binv32 = nullptr;
+  // GMRES(m)
+  double2* V = nullptr;
+  double2* hdev = nullptr;
+  double2* hhost = nullptr;  // pinned
+  double* partg = nullptr;
+  double* dnorm = nullptr;
+  int restart = 30;
+  std::vector<void*> owned;
+
+  template <typename T>
+  int alloc(T** ptr, size_t count) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess)
+      PG_FAIL(PG_ERR_HIP, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+    owned.push_back(v);
+    *ptr = static_cast<T*>(v);
+    return PG_OK;
+  }
+  ~Solver() {
+    for (void* v : owned) hipFree(v);
+    if (h) hipHostFree(h);
+    if (hhost) hipHostFree(hhost);
+  }
+};
+
+#define PG_TRY(x)                 \
+  do {                            \
+    int _rc = (x);                \
+    if (_rc != PG_OK) return _rc; \
+  } while (0)
+
+static int read_scalars(Solver* S, hipStream_t st) {
+  PG_HIP(hipMemcpyAsync(S->h, S->s, sizeof(Scal), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  return PG_OK;
+}
+
+template <int MODE, typename VT, typename XT>
+static void spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x, XT* y,
+                 const double2* b) {
+  spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->bptr, S->rowptr, S->cols, v, x, y,
+                            b, S->part);
+}
+
+// r64 = b - A x (FP64) and |r64|^2 into s->res64
+static int true_residual(Solver* S, hipStream_t st, const double2* b, const double2* x) {
+  spmv<2, double2, double2>(S, S->c64, st, S->vals, x, S->r64, b);
+  hipLaunchKernelGGL(k_fin_res64, dim3(1), dim3(kBS), 0, st, S->s, S->part, S->c64.grid);
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+template <typename VT, typename XT>
+static float time_spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x,
+                       XT* y) {
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  spmv<0, VT, XT>(S, c, st, v, x, y, nullptr);
+  hipEventRecord(a, st);
+  const int reps = 5;
+  for (int i = 0; i < reps; ++i) spmv<0, VT, XT>(S, c, st, v, x, y, nullptr);
+  hipEventRecord(b, st);
+  hipEventSynchronize(b);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return ms / reps;
+}
+
+// preconditioned (re)start and iteration, point or entity-block Jacobi
+template <typename XT>
+static void launch_reset(Solver* S, hipStream_t st, XT* r, const XT* dinv, const XT* binv, XT* z,
+                         XT* p, const double2* b) {
+  int np;
+  if (S->o.pc == PG_PC_BJACOBI) {
+    hipLaunchKernelGGL(k_reset_blk<XT>, dim3(S->gb), dim3(kBS), 0, st, S->nblocks, S->bptr,
+                       S->boff, S->r64, r, binv, z, p, b, S->part);
+    np = S->gb;
+  } else {
+    hipLaunchKernelGGL(k_reset<XT>, dim3(S->gv), dim3(kBS), 0, st, S->n, S->r64, r, dinv, z, p, b,
+                       S->part);
+    np = S->gv;
+  }
+  hipLaunchKernelGGL(k_fin_reset, dim3(1), dim3(kBS), 0, st, S->s, S->part, np, b ? 1 : 0);
+}
+
+template <typename VT, typename XT>
+static void launch_iteration(Solver* S, hipStream_t st, const SpmvCfg& c, const VT* vals, XT* x,
+                             XT* r, XT* p, XT* q, const XT* dinv, const XT* binv, XT* z) {
+  hipLaunchKernelGGL(k_pupdate<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, z, p);
+  spmv<1, VT, XT>(S, c, st, vals, p, q, nullptr);
+  hipLaunchKernelGGL(k_fin_mu, dim3(1), dim3(kBS), 0, st, S->s, S->part, c.grid);
+  int np;
+  if (S->o.pc == PG_PC_BJACOBI) {
+    hipLaunchKernelGGL(k_update_blk<XT>, dim3(S->gb), dim3(kBS), 0, st, S->s, S->nblocks, S->bptr,
+                       S->boff, x, r, p, q, binv, z, S->part);
+    np = S->gb;
+  } else {
+    hipLaunchKernelGGL(k_update<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, x, r, p, q, dinv,
+                       z, S->part);
+    np = S->gv;
+  }
+  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBS), 0, st, S->s, S->part, np);
+}
+
+static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* info,
+                      hipStream_t st) {
+  const bool mixed = S->o.precision == PG_PREC_MIXED;
+  const int64_t n = S->n;
+  const double tol2 = S->o.rtol * S->o.rtol;
+  hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, S->o.maxit, tol2);
+  if (mixed) PG_HIP(hipMemsetAsync(S->x32, 0, n * sizeof(float2), st));
+  PG_TRY(true_residual(S, st, b, x));
+  if (mixed)
+    launch_reset<float2>(S, st, S->r32, S->dinv32, S->binv32, S->z32, S->p32, b);
+  else
+    launch_reset<double2>(S, st, S->r, S->dinv, S->binv, S->z, S->p, b);
+  PG_CHECK_LAUNCH();
+  PG_TRY(read_scalars(S, st));
+  const double bnorm2 = S->h->bnorm2;
+  int refinements = 0, restarts = 0;
+  info->status = -1;
+  info->refinements = 0;
+  if (bnorm2 == 0.0) {
+    PG_HIP(hipMemsetAsync(x, 0, n * sizeof(double2), st));
+    info->iterations = 0;
+    info->relres = 0.0;
+    info->status = PG_OK;
+    return PG_OK;
+  }
+  double res64 = S->h->res64;
+  if (res64 <= tol2 * bnorm2) {
+    info->iterations = 0;
+    info->relres = std::sqrt(res64 / bnorm2);
+    info->status = PG_OK;
+    return PG_OK;
+  }
+  const double delta2 = S->o.inner_rtol * S->o.inner_rtol;
+  if (mixed) hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
+  const int check = std::max(1, S->o.check_every);
+  while (true) {
+    for (int k = 0; k < check; ++k) {
+      if (mixed)
+        launch_iteration<float2, float2>(S, st, S->c32, S->vals32, S->x32, S->r32, S->p32, S->q32,
+                                         S->dinv32, S->binv32, S->z32);
+      else
+        launch_iteration<double2, double2>(S, st, S->c64, S->vals, x, S->r, S->p, S->q, S->dinv,
+                                           S->binv, S->z);
+    }
+    PG_CHECK_LAUNCH();
+    PG_TRY(read_scalars(S, st));
+    const Scal hs = *S->h;
+    if (hs.status == PG_ERR_BREAKDOWN) {
+      info->status = PG_ERR_BREAKDOWN;
+      break;
+    }
+    if (!hs.done) continue;
+    // paused: converged recursion, refinement point, or iteration limit
+    if (mixed) {
+      hipLaunchKernelGGL(k_accumulate, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, x, S->x32);
+      refinements++;
+    }
+    hipLaunchKernelGGL(k_set_done, dim3(1), dim3(64), 0, st, S->s, 0);
+    PG_TRY(true_residual(S, st, b, x));
+    PG_TRY(read_scalars(S, st));
+    res64 = S->h->res64;
+    if (res64 <= tol2 * bnorm2) {
+      info->status = PG_OK;
+      break;
+    }
+    if (hs.iter >= S->o.maxit) break;
+    if (!mixed) {
+      // recursive residual converged but the true one did not: restart from x
+      if (++restarts > 20) break;
+      launch_reset<double2>(S, st, S->r, S->dinv, S->binv, S->z, S->p, nullptr);
+    } else {
+      // restart the complex64 recursion from the FP64 residual (keeping the old
+      // search direction across a residual replacement stalls COCG)
+      launch_reset<float2>(S, st, S->r32, S->dinv32, S->binv32, S->z32, S->p32, nullptr);
+      hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
+    }
+    PG_CHECK_LAUNCH();
+  }
+  PG_TRY(read_scalars(S, st));
+  info->iterations = S->h->iter;
+  info->refinements = refinements;
+  info->relres = std::sqrt(res64 / bnorm2);
+  return PG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES(m), right-preconditioned, CGS with one re-orthogonalisation pass
+// ---------------------------------------------------------------------------
+static void apply_pc(Solver* S, hipStream_t st, const double2* v, double2* out) {
+  if (S->o.pc == PG_PC_BJACOBI)
+    hipLaunchKernelGGL(k_bmul, dim3(nblk(S->nblocks, kBS)), dim3(kBS), 0, st, S->nblocks,
+                       S->bptr, S->boff, S->binv, v, out);
+  else
+    hipLaunchKernelGGL(k_dmul, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->dinv, v, out);
 }
 
 static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* info,
@@ -705,11 +1067,11 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
   const double tol = S->o.rtol;
   const unsigned nb = nblk(n, kBS);
   hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, S->o.maxit, tol * tol);
-  // |b|^2 through the residual kernel with x = 0 would need a zero vector: use w = b copy
-  double2* w0 = S->V;  // scratch: V_0
-  PG_HIP(hipMemcpyAsync(w0, b, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
-  hipLaunchKernelGGL(k_maxpy_norm, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, 0, S->hdev, w0,
-                     S->part, &S->s->counter[3], &S->s->bnorm2);
+  // |b|^2: w0 = b, no projection
+  PG_HIP(hipMemcpyAsync(S->V, b, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_maxpy_norm, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, 0, S->hdev, S->V,
+                     S->part);
+  hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(kBS), 0, st, S->part, S->gv, &S->s->bnorm2);
   PG_TRY(true_residual(S, st, b, x));
   PG_TRY(read_scalars(S, st));
   const double bnorm = std::sqrt(S->h->bnorm2);
@@ -728,7 +1090,6 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
   std::vector<double> cs(m);
   auto Hij = [&](int i, int j) -> double2& { return H[(size_t)i * m + j]; };
   while (beta > tol * bnorm && iters < S->o.maxit) {
-    // V_0 = r / beta
     PG_HIP(hipMemcpyAsync(S->V, S->r64, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_rscale, dim3(nb), dim3(kBS), 0, st, n, S->V, 1.0 / beta);
     std::fill(H.begin(), H.end(), make_double2(0.0, 0.0));
@@ -738,33 +1099,32 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
     for (int j = 0; j < m && iters < S->o.maxit; ++j) {
       double2* Vj = S->V + (int64_t)j * n;
       double2* w = S->V + (int64_t)(j + 1) * n;
-      hipLaunchKernelGGL(k_dmul, dim3(nb), dim3(kBS), 0, st, n, S->dinv, Vj, S->z);
-      launch_spmv<0, double2, double2>(S->G64, S->gs64, st, S->s, n, S->rowptr, S->cols, S->vals,
-                                       S->z, w, nullptr, S->part);
-      // classical Gram-Schmidt, twice
+      apply_pc(S, st, Vj, S->z);
+      spmv<0, double2, double2>(S, S->c64, st, S->vals, S->z, w, nullptr);
       for (int pass = 0; pass < 2; ++pass) {
         double2* hcol = S->hdev + pass * (m + 1);
-        hipLaunchKernelGGL(k_mdot, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, j + 1, w, S->partg,
-                           &S->s->counter[2], hcol);
+        hipLaunchKernelGGL(k_mdot, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, j + 1, w, S->partg);
+        hipLaunchKernelGGL(k_fin_mdot, dim3(j + 1), dim3(kBS), 0, st, S->partg, S->gv, hcol);
         hipLaunchKernelGGL(k_maxpy_norm, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, j + 1, hcol,
-                           w, S->part, &S->s->counter[3], &S->s->res2);
+                           w, S->part);
       }
+      hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(kBS), 0, st, S->part, S->gv, S->dnorm);
       PG_CHECK_LAUNCH();
       PG_HIP(hipMemcpyAsync(S->hhost, S->hdev, 2 * (m + 1) * sizeof(double2),
                             hipMemcpyDeviceToHost, st));
-      PG_TRY(read_scalars(S, st));
+      PG_HIP(hipMemcpyAsync(&S->h->res2, S->dnorm, sizeof(double), hipMemcpyDeviceToHost, st));
+      PG_HIP(hipStreamSynchronize(st));
       const double hn = std::sqrt(S->h->res2);
-      for (int i = 0; i <= j; ++i)
-        Hij(i, j) = zadd(S->hhost[i], S->hhost[(m + 1) + i]);
+      for (int i = 0; i <= j; ++i) Hij(i, j) = zadd(S->hhost[i], S->hhost[(m + 1) + i]);
       Hij(j + 1, j) = make_double2(hn, 0.0);
-      for (int i = 0; i < j; ++i) {  // apply previous rotations
+      for (int i = 0; i < j; ++i) {  // previous rotations
         const double2 a = Hij(i, j), bb = Hij(i + 1, j);
         const double2 sc = make_double2(sn[i].x, -sn[i].y);
         const double2 t = zadd(make_double2(cs[i] * a.x, cs[i] * a.y), zmul(sn[i], bb));
         Hij(i + 1, j) = zsub(make_double2(cs[i] * bb.x, cs[i] * bb.y), zmul(sc, a));
         Hij(i, j) = t;
       }
-      {
+      {  // new rotation [c s; -conj(s) c] zeroing H(j+1,j)
         const double2 a = Hij(j, j), bb = Hij(j + 1, j);
         const double na = std::hypot(a.x, a.y), nbb = std::hypot(bb.x, bb.y);
         const double rho = std::hypot(na, nbb);
@@ -785,10 +1145,9 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
       ++iters;
       k = j + 1;
       if (hn > 0.0) hipLaunchKernelGGL(k_rscale, dim3(nb), dim3(kBS), 0, st, n, w, 1.0 / hn);
-      const double est = std::hypot(g[j + 1].x, g[j + 1].y);
-      if (est <= tol * bnorm || hn == 0.0) break;
+      if (std::hypot(g[j + 1].x, g[j + 1].y) <= tol * bnorm || hn == 0.0) break;
     }
-    // y = H^-1 g (upper triangular), x += Dinv V y
+    // y = H^-1 g (upper triangular); x += P (V y)
     std::vector<double2> y(k);
     for (int i = k - 1; i >= 0; --i) {
       double2 acc = g[i];
@@ -797,8 +1156,9 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
     }
     for (int i = 0; i < k; ++i) S->hhost[i] = y[i];
     PG_HIP(hipMemcpyAsync(S->hdev, S->hhost, k * sizeof(double2), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_gmres_update, dim3(nb), dim3(kBS), 0, st, n, x, S->dinv, S->V, n, k,
-                       S->hdev);
+    hipLaunchKernelGGL(k_vcomb, dim3(nb), dim3(kBS), 0, st, n, S->V, n, k, S->hdev, S->z);
+    apply_pc(S, st, S->z, S->q);
+    hipLaunchKernelGGL(k_axpy1, dim3(nb), dim3(kBS), 0, st, n, S->q, x);
     PG_TRY(true_residual(S, st, b, x));
     PG_TRY(read_scalars(S, st));
     beta = std::sqrt(S->h->res64);
@@ -813,34 +1173,105 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
   return PG_OK;
 }
 
-// (Re)build everything derived from the caller's matrix values: the Jacobi
-// inverse diagonal and, for mixed precision, the complex64 copy of A.
+// (Re)build what depends on the caller's matrix values: Jacobi or block
+// inverses and, for mixed precision, the complex64 copy of A.
 static int refresh_values(Solver* S, hipStream_t st) {
   const int64_t n = S->n, nnz = S->nnz;
   const bool mixed = S->o.precision == PG_PREC_MIXED;
   PG_HIP(hipMemsetAsync(S->bad, 0, sizeof(int32_t), st));
-  if (mixed) {
+  if (mixed)
     hipLaunchKernelGGL(k_demote, dim3(nblk(2 * nnz, kBS)), dim3(kBS), 0, st, 2 * nnz,
                        reinterpret_cast<const double*>(S->vals),
                        reinterpret_cast<float*>(S->vals32));
-    if (S->o.pc == PG_PC_JACOBI)
-      hipLaunchKernelGGL(k_jacobi_inv<float2>, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n,
-                         S->rowptr, S->cols, S->vals, S->dinv32, S->bad);
+  if (S->o.pc == PG_PC_BJACOBI) {
+    hipLaunchKernelGGL(k_bjacobi_setup, dim3(nblk(S->nblocks, kBS)), dim3(kBS), 0, st,
+                       S->nblocks, S->bptr, S->boff, S->rowptr, S->cols, S->vals, S->binv,
+                       S->bad);
+    if (mixed) {
+      int32_t tot = 0;
+      PG_HIP(hipMemcpyAsync(&tot, S->boff + S->nblocks, sizeof(int32_t), hipMemcpyDeviceToHost,
+                            st));
+      PG_HIP(hipStreamSynchronize(st));
+      hipLaunchKernelGGL(k_demote, dim3(nblk(2 * (int64_t)tot, kBS)), dim3(kBS), 0, st,
+                         2 * (int64_t)tot, reinterpret_cast<const double*>(S->binv),
+                         reinterpret_cast<float*>(S->binv32));
+    }
+  } else if (S->o.pc == PG_PC_JACOBI) {
+    if (mixed)
+      hipLaunchKernelGGL(k_jacobi_inv<float2>, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->rowptr,
+                         S->cols, S->vals, S->dinv32, S->bad);
     else
-      hipLaunchKernelGGL(k_ones32, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->dinv32);
-  } else {
-    if (S->o.pc == PG_PC_JACOBI)
       hipLaunchKernelGGL(k_jacobi_inv<double2>, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n,
                          S->rowptr, S->cols, S->vals, S->dinv, S->bad);
+  } else {
+    if (mixed)
+      hipLaunchKernelGGL(k_ones32, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->dinv32);
     else
       hipLaunchKernelGGL(k_ones, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->dinv);
   }
   PG_CHECK_LAUNCH();
   PG_HIP(hipMemcpyAsync(&S->h->iter, S->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   PG_HIP(hipStreamSynchronize(st));
-  if (S->h->iter != 0) PG_FAIL(PG_ERR_ARG, "zero diagonal entry: Jacobi preconditioner undefined");
+  if (S->h->iter != 0)
+    PG_FAIL(PG_ERR_ARG, "singular diagonal entry or block: preconditioner undefined");
   return PG_OK;
 }
+
+// pick the fastest SpMV configuration for this matrix (row / entity-row, G, U)
+static int autotune(Solver* S, hipStream_t st) {
+  const bool mixed = S->o.precision == PG_PREC_MIXED;
+  if (S->bptr) {
+    PG_HIP(hipMemsetAsync(S->bad, 0, sizeof(int32_t), st));
+    hipLaunchKernelGGL(k_check_entity_rows, dim3(nblk(S->nblocks, kBS)), dim3(kBS), 0, st,
+                       S->nblocks, S->bptr, S->rowptr, S->cols, S->bad);
+    PG_HIP(hipMemcpyAsync(&S->h->iter, S->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    S->er_ok = S->h->iter == 0;
+  }
+  if (S->o.spmv == PG_SPMV_ENTITY && !S->er_ok)
+    PG_FAIL(PG_ERR_ARG, "entity-row SpMV needs block_ptr ranges whose rows share one column list");
+  std::vector<SpmvCfg> cands;
+  const int gu[6][2] = {{8, 4}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {64, 1}};
+  for (const auto& g : gu) {
+    SpmvCfg c;
+    c.kind = 0;
+    c.G = g[0];
+    c.U = g[1];
+    c.grid = grid_for(S->n, c.G);
+    if (S->o.spmv != PG_SPMV_ENTITY) cands.push_back(c);
+    if (S->er_ok && S->o.spmv != PG_SPMV_ROW) {
+      SpmvCfg e = c;
+      e.kind = 1;
+      e.D = S->maxblk <= 1 ? 1 : (S->maxblk <= 2 ? 2 : 6);
+      e.grid = grid_for(S->nblocks, e.G);
+      cands.push_back(e);
+    }
+  }
+  // timing inputs: x = 1 (finite), y = FP64 scratch q (allocated in every mode)
+  hipLaunchKernelGGL(k_ones, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->r64);
+  float best = 1e30f;
+  for (const SpmvCfg& c : cands) {
+    const float t = time_spmv<double2, double2>(S, c, st, S->vals, S->r64, S->q);
+    if (t < best) {
+      best = t;
+      S->c64 = c;
+    }
+  }
+  if (mixed) {
+    hipLaunchKernelGGL(k_ones32, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->p32);
+    best = 1e30f;
+    for (const SpmvCfg& c : cands) {
+      const float t = time_spmv<float2, float2>(S, c, st, S->vals32, S->p32, S->q32);
+      if (t < best) {
+        best = t;
+        S->c32 = c;
+      }
+    }
+  }
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
 }  // namespace pg
 
 using namespace pg;
@@ -850,7 +1281,11 @@ extern "C" {
 int pg_spmv_z(int64_t n, const int32_t* rp, const int32_t* ci, const double* v, const double* x,
               double* y, void* stream) {
   if (n <= 0 || !rp || !ci || !v || !x || !y) PG_FAIL(PG_ERR_ARG, "bad arguments");
-  launch_spmv<0, double2, double2>(16, grid_for(n, 16), as_stream(stream), nullptr, n, rp, ci,
+  SpmvCfg c;
+  c.G = 16;
+  c.U = 2;
+  c.grid = grid_for(n, 16);
+  spmv_launch<0, double2, double2>(c, as_stream(stream), nullptr, n, 0, nullptr, rp, ci,
                                    reinterpret_cast<const double2*>(v),
                                    reinterpret_cast<const double2*>(x),
                                    reinterpret_cast<double2*>(y), nullptr, nullptr);
@@ -861,10 +1296,14 @@ int pg_spmv_z(int64_t n, const int32_t* rp, const int32_t* ci, const double* v, 
 int pg_spmv_c(int64_t n, const int32_t* rp, const int32_t* ci, const float* v, const float* x,
               float* y, void* stream) {
   if (n <= 0 || !rp || !ci || !v || !x || !y) PG_FAIL(PG_ERR_ARG, "bad arguments");
-  launch_spmv<0, float2, float2>(16, grid_for(n, 16), as_stream(stream), nullptr, n, rp, ci,
+  SpmvCfg c;
+  c.G = 16;
+  c.U = 2;
+  c.grid = grid_for(n, 16);
+  spmv_launch<0, float2, float2>(c, as_stream(stream), nullptr, n, 0, nullptr, rp, ci,
                                  reinterpret_cast<const float2*>(v),
-                                 reinterpret_cast<const float2*>(x),
-                                 reinterpret_cast<float2*>(y), nullptr, nullptr);
+                                 reinterpret_cast<const float2*>(x), reinterpret_cast<float2*>(y),
+                                 nullptr, nullptr);
   PG_CHECK_LAUNCH();
   return PG_OK;
 }
@@ -887,20 +1326,42 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
     PG_FAIL(PG_ERR_UNSUPPORTED, "GMRES runs in FP64 only (use COCG for mixed precision)");
   if (opts->ksp == PG_KSP_GMRES && (opts->restart < 1 || opts->restart > kMaxRestart))
     PG_FAIL(PG_ERR_ARG, "GMRES restart must be in 1..%d", kMaxRestart);
-  if (opts->pc != PG_PC_NONE && opts->pc != PG_PC_JACOBI)
+  if (opts->pc != PG_PC_NONE && opts->pc != PG_PC_JACOBI && opts->pc != PG_PC_BJACOBI)
     PG_FAIL(PG_ERR_UNSUPPORTED, "pc type %d not supported", opts->pc);
   if (!(opts->rtol > 0.0)) PG_FAIL(PG_ERR_ARG, "rtol must be positive");
+  if (opts->spmv < PG_SPMV_AUTO || opts->spmv > PG_SPMV_ENTITY)
+    PG_FAIL(PG_ERR_ARG, "spmv kind %d not supported", opts->spmv);
+  std::vector<int32_t> hboff;
+  int maxblk = 0;
+  if (opts->pc == PG_PC_BJACOBI && (!opts->block_ptr || opts->nblocks <= 0))
+    PG_FAIL(PG_ERR_ARG, "block Jacobi needs block_ptr and nblocks");
+  if (opts->block_ptr && opts->nblocks > 0) {
+    const int32_t* bp = opts->block_ptr;
+    if (bp[0] != 0 || bp[opts->nblocks] != n)
+      PG_FAIL(PG_ERR_ARG, "block_ptr must start at 0 and end at n");
+    hboff.resize(opts->nblocks + 1);
+    hboff[0] = 0;
+    for (int64_t k = 0; k < opts->nblocks; ++k) {
+      const int d = bp[k + 1] - bp[k];
+      if (d < 1 || d > kMaxBlock) PG_FAIL(PG_ERR_ARG, "block sizes must be in 1..%d", kMaxBlock);
+      maxblk = std::max(maxblk, d);
+      if ((int64_t)hboff[k] + d * d >= (1LL << 31)) PG_FAIL(PG_ERR_ARG, "block storage too large");
+      hboff[k + 1] = hboff[k] + d * d;
+    }
+  }
   hipStream_t st = as_stream(stream);
   Solver* S = new Solver();
   S->n = n;
   S->nnz = nnz;
   S->o = *opts;
+  S->o.block_ptr = nullptr;  // the host pointer is not kept
   if (S->o.maxit <= 0) S->o.maxit = 100000;
   if (S->o.check_every <= 0) S->o.check_every = 25;
   if (!(S->o.inner_rtol > 0.0) || S->o.inner_rtol >= 1.0) S->o.inner_rtol = 1e-4;
   S->rowptr = rowptr;
   S->cols = cols;
   S->vals = reinterpret_cast<const double2*>(values);
+  S->gv = vgrid_for(n);
   const bool mixed = opts->precision == PG_PREC_MIXED;
   int rc = PG_OK;
 #define PG_A(ptr, cnt)                           \
@@ -911,7 +1372,9 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   PG_A(S->s, 1);
   PG_A(S->part, 4 * kMaxGrid);
   PG_A(S->r64, n);
+  PG_A(S->q, n);
   PG_A(S->bad, 1);
+  PG_A(S->dnorm, 1);
   if (mixed) {
     PG_A(S->vals32, nnz);
     PG_A(S->r32, n);
@@ -923,16 +1386,34 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   } else {
     PG_A(S->r, n);
     PG_A(S->p, n);
-    PG_A(S->q, n);
     PG_A(S->z, n);
     PG_A(S->dinv, n);
+  }
+  if (opts->block_ptr && opts->nblocks > 0) {
+    S->nblocks = opts->nblocks;
+    S->maxblk = maxblk;
+    S->gb = vgrid_for(S->nblocks);
+    PG_A(S->bptr, S->nblocks + 1);
+    PG_A(S->boff, S->nblocks + 1);
+    if (opts->pc == PG_PC_BJACOBI) {
+      PG_A(S->binv, hboff[S->nblocks]);
+      if (mixed) PG_A(S->binv32, hboff[S->nblocks]);
+    }
+    if (hipMemcpy(S->bptr, opts->block_ptr, (S->nblocks + 1) * sizeof(int32_t),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(S->boff, hboff.data(), (S->nblocks + 1) * sizeof(int32_t),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      delete S;
+      PG_FAIL(PG_ERR_HIP, "block upload failed");
+    }
   }
   if (opts->ksp == PG_KSP_GMRES) {
     S->restart = opts->restart;
     PG_A(S->V, (size_t)(S->restart + 1) * n);
     PG_A(S->hdev, 2 * (S->restart + 1));
     PG_A(S->partg, (size_t)2 * (S->restart + 1) * kMaxGrid);
-    if (hipHostMalloc(reinterpret_cast<void**>(&S->hhost), 2 * (S->restart + 1) * sizeof(double2), 0) != hipSuccess) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&S->hhost),
+                      2 * (S->restart + 1) * sizeof(double2), 0) != hipSuccess) {
       delete S;
       PG_FAIL(PG_ERR_HIP, "hipHostMalloc failed");
     }
@@ -943,33 +1424,10 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
     PG_FAIL(PG_ERR_HIP, "hipHostMalloc failed");
   }
   hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, 0, 0.0);
-  if ((rc = refresh_values(S, st)) != PG_OK) {
+  if ((rc = refresh_values(S, st)) != PG_OK || (rc = autotune(S, st)) != PG_OK) {
     delete S;
     return rc;
   }
-  // autotune the SpMV lane-group size on this matrix (vectors hold scratch data)
-  const int cands[4] = {8, 16, 32, 64};
-  float best = 1e30f;
-  for (int G : cands) {
-    const float t = time_spmv<double2, double2>(S, G, st, S->vals, S->r64, mixed ? S->r64 : S->q);
-    if (t < best) {
-      best = t;
-      S->G64 = G;
-    }
-  }
-  if (mixed) {
-    best = 1e30f;
-    for (int G : cands) {
-      const float t = time_spmv<float2, float2>(S, G, st, S->vals32, S->r32, S->q32);
-      if (t < best) {
-        best = t;
-        S->G32 = G;
-      }
-    }
-  }
-  S->gs64 = grid_for(n, S->G64);
-  S->gs32 = grid_for(n, S->G32);
-  S->gv = vgrid_for(n);
   if (hipStreamSynchronize(st) != hipSuccess) {
     delete S;
     PG_FAIL(PG_ERR_HIP, "solver setup failed");
@@ -989,8 +1447,8 @@ int pg_solver_solve(void* solver, const double* b, double* x, pg_solve_info* inf
   if (S->o.ksp == PG_KSP_GMRES)
     return gmres_solve(S, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x),
                        info, as_stream(stream));
-  return solve(S, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x), info,
-               as_stream(stream));
+  return cocg_solve(S, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x), info,
+                    as_stream(stream));
 }
 
 int pg_solver_profile(void* solver, int reps, double* ms64, double* ms32, int32_t* g64,
@@ -1002,41 +1460,32 @@ int pg_solver_profile(void* solver, int reps, double* ms64, double* ms32, int32_
   hipEvent_t a, b;
   PG_HIP(hipEventCreate(&a));
   PG_HIP(hipEventCreate(&b));
-  // the solver's own COCG SpMV kernel (q = A p, mu = p^T q), on its own vectors
+  // the solve's own SpMV kernel (MODE 1: q = A p and the mu partials) on a finite
+  // p, with the scalars reset so that the early-exit flag is clear
   hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, 1, 0.0);
   float t = 0.f;
-  if (!mixed) {
-    hipMemsetAsync(S->p, 0, S->n * sizeof(double2), st);
+  *ms32 = 0.0;
+  if (mixed) {
+    hipLaunchKernelGGL(k_ones32, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->p32);
     hipEventRecord(a, st);
     for (int i = 0; i < reps; ++i)
-      launch_spmv<1, double2, double2>(S->G64, S->gs64, st, S->s, S->n, S->rowptr, S->cols,
-                                       S->vals, S->p, S->q, nullptr, S->part);
-    hipEventRecord(b, st);
-    hipEventSynchronize(b);
-    hipEventElapsedTime(&t, a, b);
-    *ms64 = t / reps;
-    *ms32 = 0.0;
-  } else {
-    hipMemsetAsync(S->p32, 0, S->n * sizeof(float2), st);
-    hipEventRecord(a, st);
-    for (int i = 0; i < reps; ++i)
-      launch_spmv<1, float2, float2>(S->G32, S->gs32, st, S->s, S->n, S->rowptr, S->cols,
-                                     S->vals32, S->p32, S->q32, nullptr, S->part);
+      spmv<1, float2, float2>(S, S->c32, st, S->vals32, S->p32, S->q32, nullptr);
     hipEventRecord(b, st);
     hipEventSynchronize(b);
     hipEventElapsedTime(&t, a, b);
     *ms32 = t / reps;
-    hipEventRecord(a, st);
-    for (int i = 0; i < reps; ++i)
-      launch_spmv<2, double2, double2>(S->G64, S->gs64, st, S->s, S->n, S->rowptr, S->cols,
-                                       S->vals, S->r64, S->r64 /*unused y*/, reinterpret_cast<const double2*>(S->r64), S->part);
-    hipEventRecord(b, st);
-    hipEventSynchronize(b);
-    hipEventElapsedTime(&t, a, b);
-    *ms64 = t / reps;
   }
-  *g64 = S->G64;
-  *g32 = S->G32;
+  hipLaunchKernelGGL(k_ones, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->r64);
+  hipEventRecord(a, st);
+  for (int i = 0; i < reps; ++i)
+    spmv<1, double2, double2>(S, S->c64, st, S->vals, S->r64, S->q, nullptr);
+  hipEventRecord(b, st);
+  hipEventSynchronize(b);
+  hipEventElapsedTime(&t, a, b);
+  *ms64 = t / reps;
+  // configuration code: kind*1000 + G*10 + U
+  *g64 = S->c64.kind * 1000 + S->c64.G * 10 + S->c64.U;
+  *g32 = S->c32.kind * 1000 + S->c32.G * 10 + S->c32.U;
   hipEventDestroy(a);
   hipEventDestroy(b);
   PG_CHECK_LAUNCH();

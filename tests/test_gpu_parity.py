@@ -234,7 +234,7 @@ def test_spmv_vs_scipy():
     assert np.abs(y - yr).max() <= 1e-13 * np.abs(yr).max()
 
 
-def _forward_gpu(w, precision="fp64", rtol=1e-8, ksp="cocg"):
+def _forward_gpu(w, precision="fp64", rtol=1e-8, ksp="cocg", pc="jacobi", spmv="auto"):
     T = w.topo
     mesh = core.DeviceMesh.from_topology(T, DEV)
     asm = core.Assembler(mesh)
@@ -245,7 +245,8 @@ def _forward_gpu(w, precision="fp64", rtol=1e-8, ksp="cocg"):
     el = int(core.locate_points(mesh, np.array(w.source["position"])[None]).cpu()[0])
     b = core.source_vector(mesh, el, w.source["position"], core.dipole_moment(w.source), omega)
     core.apply_dirichlet(asm.plan, vals, dof_bc, b)
-    solver = core.KrylovSolver(asm.plan, vals, ksp=ksp, precision=precision, rtol=rtol)
+    solver = core.KrylovSolver(asm.plan, vals, ksp=ksp, precision=precision, rtol=rtol, pc=pc,
+                               spmv=spmv, block_ptr=mesh.block_ptr())
     x, info = solver.solve(b)
     rx_el = core.locate_points(mesh, w.receivers)
     f = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, x, omega))
@@ -263,17 +264,20 @@ def _oracle_forward(w):
 
 
 @pytest.mark.parametrize("p,n", [(1, 10), (2, 6), (3, 4)])
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_forward_solve_vs_superlu(p, n, precision):
+@pytest.mark.parametrize("precision,pc,spmv", [("fp64", "jacobi", "row"),
+                                               ("fp64", "bjacobi", "entity"),
+                                               ("mixed", "jacobi", "entity"),
+                                               ("mixed", "bjacobi", "row")])
+def test_forward_solve_vs_superlu(p, n, precision, pc, spmv):
     w = Workload(n, p, noise=0.1, seed=7, n_rx=21)
-    g = _forward_gpu(w, precision=precision)
+    g = _forward_gpu(w, precision=precision, pc=pc, spmv=spmv)
     r = _oracle_forward(w)
     info = g["info"]
     true_rel = np.linalg.norm(g["b"] - g["A"] @ g["x"]) / np.linalg.norm(g["b"])
     assert np.abs(g["b"] - r["bbc"]).max() <= 1e-12 * np.abs(r["bbc"]).max()
     assert np.abs(g["A"].data - r["Abc"].data).max() <= 1e-12 * np.abs(r["Abc"].data).max()
     ferr = np.abs(g["fields"] - r["fields"]).max() / np.abs(r["fields"]).max()
-    print(f"p={p} n={n} {precision}: iters={info['iterations']} refinements={info['refinements']} "
+    print(f"p={p} n={n} {precision} {pc} {spmv}: iters={info['iterations']} refinements={info['refinements']} "
           f"relres={info['relres']:.2e} true={true_rel:.2e} field err={ferr:.2e}", flush=True)
     assert info["status"] == 0
     assert info["relres"] <= 1e-8 and true_rel <= 1e-8

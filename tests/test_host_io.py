@@ -90,7 +90,7 @@ def test_solver_options_from_reference_opts_file(tmp_path):
     f = tmp_path / "petsc.opts"
     f.write_text("# Solver options for PETSc\n-ksp_type gmres\n-pc_type sor\n-ksp_rtol 1e-8\n")
     o = SolverOptions(["kernel.py", "-options_file", str(f), "params.yaml"])
-    assert (o.ksp_type, o.pc_type, o.rtol) == ("gmres", "jacobi", 1e-8)
+    assert (o.ksp_type, o.pc_type, o.rtol) == ("gmres", "bjacobi", 1e-8)
     assert any("sor" in n for n in o.notes)
     o = SolverOptions(["-ksp_type", "cocg", "-pg_precision", "mixed", "-ksp_max_it", "50"])
     assert (o.ksp_type, o.precision, o.max_it) == ("cocg", "mixed", 50)
