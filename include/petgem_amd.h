@@ -175,6 +175,10 @@ int pg_adjoint_rhs(int p, int64_t n_points, const double *points, const int32_t 
 int pg_adjoint_gradient(const void *tables, int64_t n_elems, const void *workspace,
                         const int32_t *dofs, int nrhs, int64_t ndof, const double *x,
                         const double *lam, const double *omega, double *g, void *stream);
+/* The same with x, lam in the batched solver's (ndof, nrhs) row-major layout. */
+int pg_adjoint_gradient_rows(const void *tables, int64_t n_elems, const void *workspace,
+                             const int32_t *dofs, int nrhs, int64_t ndof, const double *x,
+                             const double *lam, const double *omega, double *g, void *stream);
 
 /* ------------------------------------------------------------------------
  * Sparse kernels and Krylov solve.  Replace MatMult and KSPSolve

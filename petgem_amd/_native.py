@@ -65,6 +65,8 @@ SIGNATURES = {
                                       _P]),
     "pg_adjoint_gradient": (ctypes.c_int, [_P, _I64, _P, _P, ctypes.c_int, _I64, _P, _P, _P, _P,
                                            _P]),
+    "pg_adjoint_gradient_rows": (ctypes.c_int, [_P, _I64, _P, _P, ctypes.c_int, _I64, _P, _P, _P,
+                                                _P, _P]),
     "pg_spmv_z": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     "pg_spmv_c": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     "pg_demote_z2c": (ctypes.c_int, [_I64, _P, _P, _P]),

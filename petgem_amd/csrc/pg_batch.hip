@@ -47,6 +47,19 @@ __device__ inline void rows_of_block(int64_t n, int64_t* lo, int64_t* hi) {
   *hi = c < n ? c : n;
 }
 
+// rows of the SpMM for this workgroup: XCD x (= blockIdx.x % 8, the dispatch
+// order) owns the x-th eighth of the rows and its workgroups sweep it
+// together, interleaved by `per` rows, so the gathers of P stay in its L2
+struct Sweep {
+  int64_t first, end, stride;
+};
+__device__ inline Sweep xcd_sweep(int64_t n, int per) {
+  const int64_t b = blockIdx.x, nb = gridDim.x;
+  if (nb % 8 != 0) return {b * per, n, nb * per};
+  const int64_t x = b % 8, nbx = nb / 8;
+  return {n * x / 8 + (b / 8) * per, n * (x + 1) / 8, nbx * per};
+}
+
 // Threads are arranged (rsub, c): c = column lane (0..CT), rsub = row lane.
 // Column r handled by lane c is r = c + t*CT, t < R/CT.
 // Per-column partial sums (NV doubles each) are reduced over rsub in shared
@@ -115,8 +128,7 @@ __global__ void __launch_bounds__(kBS) k_spmm(const Ctl* ctl, int64_t n, int R,
   constexpr int RPB = kBS / CT;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
   const int T = R / CT;
-  int64_t lo, hi;
-  rows_of_block(n, &lo, &hi);
+  const Sweep sw = xcd_sweep(n, RPB);
   double v[TMAX][2];
   double om[TMAX];
 #pragma unroll
@@ -124,7 +136,7 @@ __global__ void __launch_bounds__(kBS) k_spmm(const Ctl* ctl, int64_t n, int R,
     v[t][0] = v[t][1] = 0.0;
     om[t] = t < T ? omu[c + t * CT] : 0.0;
   }
-  for (int64_t row = lo + rs; row < hi; row += RPB) {
+  for (int64_t row = sw.first + rs; row < sw.end; row += sw.stride) {
     double2 acc[TMAX];
 #pragma unroll
     for (int t = 0; t < TMAX; ++t) acc[t] = make_double2(0.0, 0.0);
@@ -360,8 +372,7 @@ __global__ void __launch_bounds__(kBS) k_true_res(int64_t n, int R,
   constexpr int RPB = kBS / CT;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
   const int T = R / CT;
-  int64_t lo, hi;
-  rows_of_block(n, &lo, &hi);
+  const Sweep sw = xcd_sweep(n, RPB);
   double v[TMAX][1];
   double om[TMAX];
 #pragma unroll
@@ -369,7 +380,7 @@ __global__ void __launch_bounds__(kBS) k_true_res(int64_t n, int R,
     v[t][0] = 0.0;
     om[t] = t < T ? omu[c + t * CT] : 0.0;
   }
-  for (int64_t row = lo + rs; row < hi; row += RPB) {
+  for (int64_t row = sw.first + rs; row < sw.end; row += sw.stride) {
     double2 acc[TMAX];
 #pragma unroll
     for (int t = 0; t < TMAX; ++t) acc[t] = make_double2(0.0, 0.0);

@@ -312,21 +312,26 @@ __global__ void __launch_bounds__(256) k_assemble(
 }
 
 // -------------------------------------------------------------------------
-// PEC boundary condition on an assembled CSR (one thread per row)
+// PEC boundary condition on an assembled CSR (16 lanes per row, so that the
+// column and value reads of a row are coalesced)
 // -------------------------------------------------------------------------
-__global__ void k_dirichlet(int64_t n, const int32_t* __restrict__ rowptr,
-                            const int32_t* __restrict__ cols, const uint8_t* __restrict__ bc,
-                            double2* __restrict__ vals, double2* __restrict__ b) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_dirichlet(int64_t n, const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ cols,
+                                                   const uint8_t* __restrict__ bc,
+                                                   double2* __restrict__ vals,
+                                                   double2* __restrict__ b) {
+  constexpr int G = 16;
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int lane = threadIdx.x % G;
   if (r >= n) return;
   const bool br = bc[r] != 0;
   if (vals != nullptr)
-  for (int s = rowptr[r]; s < rowptr[r + 1]; ++s) {
-    const int c = cols[s];
-    if (br) vals[s] = make_double2(c == r ? 1.0 : 0.0, 0.0);
-    else if (bc[c]) vals[s] = make_double2(0.0, 0.0);
-  }
-  if (br && b != nullptr) b[r] = make_double2(0.0, 0.0);
+    for (int s = rowptr[r] + lane; s < rowptr[r + 1]; s += G) {
+      const int c = cols[s];
+      if (br) vals[s] = make_double2(c == r ? 1.0 : 0.0, 0.0);
+      else if (bc[c]) vals[s] = make_double2(0.0, 0.0);
+    }
+  if (br && b != nullptr && lane == 0) b[r] = make_double2(0.0, 0.0);
 }
 
 // -------------------------------------------------------------------------
@@ -516,45 +521,62 @@ __global__ void k_adjoint_rhs(int64_t npts, const double* __restrict__ pts,
 }
 
 // g[e] += sum_r -omega_r*mu0 * Im(lam_e^T M_e x_e), M_e recomputed from the tables.
-// One thread per element; x and lam are (nrhs, N) complex, column r at r*N.
+// One thread per element.  Column r of x and lam holds DOF d at d*sd + r*sr
+// ((nrhs, N) layout: sd = 1, sr = N; (N, nrhs) layout: sd = nrhs, sr = 1).  Each
+// M_ij is contracted once per chunk of RC columns.
 template <int P>
 __global__ void __launch_bounds__(256) k_adjoint_grad(
     int64_t E, const double* __restrict__ T, const double* __restrict__ coef,
-    const int32_t* __restrict__ orient, const int32_t* __restrict__ dofs, int nrhs, int64_t N,
-    const double2* __restrict__ x, const double2* __restrict__ lam,
+    const int32_t* __restrict__ orient, const int32_t* __restrict__ dofs, int nrhs, int64_t sd,
+    int64_t sr, const double2* __restrict__ x, const double2* __restrict__ lam,
     const double* __restrict__ omega_mu, double* __restrict__ g) {
   constexpr int n = P * (P + 2) * (P + 3) / 2;
   constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
+  constexpr int RC = 8;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   const int pk = orient[e];
   const double* cf = coef + 12 * e;
   const int32_t* de = dofs + e * n;
   double acc_g = 0.0;
-  for (int r = 0; r < nrhs; ++r) {
-    const double2* xr = x + (int64_t)r * N;
-    const double2* lr = lam + (int64_t)r * N;
-    double sre = 0.0, sim = 0.0;
+  for (int r0 = 0; r0 < nrhs; r0 += RC) {
+    const int nr = nrhs - r0 < RC ? nrhs - r0 : RC;
+    double sim[RC];
+#pragma unroll
+    for (int t = 0; t < RC; ++t) sim[t] = 0.0;
     for (int i = 0; i < n; ++i) {
       int la, ka;
       dof_entity<P>(i, la, ka);
       const int va = entity_variant_base(la, orient_of(pk, la), P) + ka;
-      double are = 0.0, aim = 0.0;
+      double are[RC], aim[RC];
+#pragma unroll
+      for (int t = 0; t < RC; ++t) are[t] = aim[t] = 0.0;
       for (int j = 0; j < n; ++j) {
         int lb, kb;
         dof_entity<P>(j, lb, kb);
         const int vb = entity_variant_base(lb, orient_of(pk, lb), P) + kb;
         double m, kk;
         contract(T + ((size_t)va * V + vb) * 12, cf, &m, &kk);
-        const double2 xj = xr[de[j]];
-        are += m * xj.x;
-        aim += m * xj.y;
+        const double2* xj = x + (int64_t)de[j] * sd + (int64_t)r0 * sr;
+#pragma unroll
+        for (int t = 0; t < RC; ++t)
+          if (t < nr) {
+            const double2 v = xj[t * sr];
+            are[t] += m * v.x;
+            aim[t] += m * v.y;
+          }
       }
-      const double2 li = lr[de[i]];
-      sre += li.x * are - li.y * aim;
-      sim += li.x * aim + li.y * are;
+      const double2* li = lam + (int64_t)de[i] * sd + (int64_t)r0 * sr;
+#pragma unroll
+      for (int t = 0; t < RC; ++t)
+        if (t < nr) {
+          const double2 l = li[t * sr];
+          sim[t] += l.x * aim[t] + l.y * are[t];
+        }
     }
-    acc_g += -omega_mu[r] * sim;
+#pragma unroll
+    for (int t = 0; t < RC; ++t)
+      if (t < nr) acc_g += -omega_mu[r0 + t] * sim[t];
   }
   g[e] += acc_g;
 }
@@ -696,7 +718,7 @@ int pg_assemble_km(const void* tables, const pg_plan_dev* pl, const void* ws,
 int pg_apply_dirichlet(int64_t n, const int32_t* rowptr, const int32_t* cols, const uint8_t* bc,
                        double* values, double* b, void* stream) {
   if (n <= 0 || !rowptr || !cols || !bc || (!values && !b)) PG_FAIL(PG_ERR_ARG, "bad arguments");
-  hipLaunchKernelGGL(k_dirichlet, dim3(nblocks(n, 256)), dim3(256), 0, as_stream(stream), n,
+  hipLaunchKernelGGL(k_dirichlet, dim3(nblocks(16 * n, 256)), dim3(256), 0, as_stream(stream), n,
                      rowptr, cols, bc, reinterpret_cast<double2*>(values),
                      reinterpret_cast<double2*>(b));
   PG_CHECK_LAUNCH();
@@ -777,9 +799,9 @@ int pg_adjoint_rhs(int p, int64_t npts, const double* pts, const int32_t* elem,
   return PG_OK;
 }
 
-int pg_adjoint_gradient(const void* tables, int64_t E, const void* ws, const int32_t* dofs,
-                        int nrhs, int64_t ndof, const double* x, const double* lam,
-                        const double* omega, double* g, void* stream) {
+static int adjoint_gradient(const void* tables, int64_t E, const void* ws, const int32_t* dofs,
+                            int nrhs, int64_t ndof, int64_t sd, int64_t sr, const double* x,
+                            const double* lam, const double* omega, double* g, void* stream) {
   if (!tables || !ws || !dofs || nrhs <= 0 || ndof <= 0 || !x || !lam || !omega || !g || E <= 0)
     PG_FAIL(PG_ERR_ARG, "bad arguments");
   if (nrhs > 4096) PG_FAIL(PG_ERR_ARG, "at most 4096 right-hand sides per call");
@@ -787,23 +809,35 @@ int pg_adjoint_gradient(const void* tables, int64_t E, const void* ws, const int
   std::vector<double> om(nrhs);
   for (int r = 0; r < nrhs; ++r) om[r] = omega[r] * kMu0;
   double* d_om = nullptr;
-  PG_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_om), nrhs * sizeof(double), as_stream(stream)));
   hipStream_t s = as_stream(stream);
+  PG_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_om), nrhs * sizeof(double), s));
   PG_HIP(hipMemcpyAsync(d_om, om.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s));
   void* w = const_cast<void*>(ws);
   const dim3 gr(nblocks(E, 256)), bl(256);
   const double2* xx = reinterpret_cast<const double2*>(x);
   const double2* ll = reinterpret_cast<const double2*>(lam);
   switch (t->p) {
-    case 1: hipLaunchKernelGGL(k_adjoint_grad<1>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, ndof, xx, ll, d_om, g); break;
-    case 2: hipLaunchKernelGGL(k_adjoint_grad<2>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, ndof, xx, ll, d_om, g); break;
-    case 3: hipLaunchKernelGGL(k_adjoint_grad<3>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, ndof, xx, ll, d_om, g); break;
+    case 1: hipLaunchKernelGGL(k_adjoint_grad<1>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, sd, sr, xx, ll, d_om, g); break;
+    case 2: hipLaunchKernelGGL(k_adjoint_grad<2>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, sd, sr, xx, ll, d_om, g); break;
+    case 3: hipLaunchKernelGGL(k_adjoint_grad<3>, gr, bl, 0, s, E, t->d_T, ws_coef(w), ws_orient(w, E), dofs, nrhs, sd, sr, xx, ll, d_om, g); break;
   }
   PG_CHECK_LAUNCH();
   // the host copy above is synchronous with respect to `om` only once the stream drains
   PG_HIP(hipStreamSynchronize(s));
   PG_HIP(hipFreeAsync(d_om, s));
   return PG_OK;
+}
+
+int pg_adjoint_gradient(const void* tables, int64_t E, const void* ws, const int32_t* dofs,
+                        int nrhs, int64_t ndof, const double* x, const double* lam,
+                        const double* omega, double* g, void* stream) {
+  return adjoint_gradient(tables, E, ws, dofs, nrhs, ndof, 1, ndof, x, lam, omega, g, stream);
+}
+
+int pg_adjoint_gradient_rows(const void* tables, int64_t E, const void* ws, const int32_t* dofs,
+                             int nrhs, int64_t ndof, const double* x, const double* lam,
+                             const double* omega, double* g, void* stream) {
+  return adjoint_gradient(tables, E, ws, dofs, nrhs, ndof, nrhs, 1, x, lam, omega, g, stream);
 }
 
 }  // extern "C"

@@ -22,8 +22,10 @@
 //  * every kernel returns immediately once the device-side `done` flag is
 //    set, so the host enqueues batches of iterations and reads the flags only
 //    between batches;
-//  * rows are split into one contiguous chunk per workgroup, workgroups
-//    renumbered so that each XCD owns a contiguous eighth of the rows;
+//  * each XCD owns a contiguous eighth of the rows; in the SpMV its
+//    workgroups sweep that eighth together (row-interleaved) so that the
+//    gathers of x stay in the XCD's L2, in the vector kernels each workgroup
+//    streams one contiguous chunk;
 //  * SpMV kernels (chosen per matrix by timing): "row" = G lanes per CSR row,
 //    "entity-row" = G lanes per mesh entity whose d rows share one column
 //    list (each index and gathered x value read once for d rows); each lane
@@ -102,6 +104,23 @@ __device__ inline void chunk(int64_t n, int64_t unit, int64_t* lo, int64_t* hi) 
   *hi = b < n ? b : n;
 }
 
+// Split for the SpMV, whose gathers of x have spatial reuse: XCD x (workgroup
+// b runs on XCD b % 8, the round-robin dispatch order) owns the x-th eighth of
+// the rows, and that XCD's workgroups sweep it together, interleaved by
+// `per` rows, so that at any moment the XCD's gathers fall in a narrow moving
+// window of x that stays in its L2.  (Contiguous per-workgroup chunks spread
+// the XCD's concurrent workgroups over its whole eighth and re-fetch x from
+// memory many times over.)
+struct Sweep {
+  int64_t first, end, stride;
+};
+__device__ inline Sweep xcd_sweep(int64_t n, int per) {
+  const int64_t b = blockIdx.x, nb = gridDim.x;
+  if (nb % 8 != 0) return {b * per, n, nb * per};
+  const int64_t x = b % 8, nbx = nb / 8;
+  return {n * x / 8 + (b / 8) * per, n * (x + 1) / 8, nbx * per};
+}
+
 // block reduction of NV doubles; result valid in thread 0
 template <int NV>
 __device__ inline void block_reduce(double (&v)[NV]) {
@@ -151,10 +170,38 @@ __device__ inline void sum_partials(const double* part, int np, double (&tot)[NV
   for (int k = 0; k < NV; ++k) tot[k] = res[k];
 }
 
+// Matrix loads.  (Non-temporal loads were measured slower here: without L2
+// allocation, the neighbouring halves of a cache line that other load
+// instructions of the same wave use are fetched from memory again.)
+template <typename T>
+__device__ inline T ld_stream(const T* p) {
+  return *p;
+}
+
 // ---------------------------------------------------------------------------
 // SpMV.  MODE 0: y = A x.  MODE 1: y = A x and partials of x^T y (COCG mu).
 // MODE 2: y = b - A x and partials of |y|^2 (true residual).
 // ---------------------------------------------------------------------------
+template <int MODE, typename XT>
+__device__ inline void spmv_row_out(int64_t row, XT acc, const XT* __restrict__ x,
+                                    XT* __restrict__ y, const double2* __restrict__ b,
+                                    double (&red)[2]) {
+  if (MODE == 2) {
+    const double2 bb = b[row];
+    const double2 rr = make_double2(bb.x - (double)acc.x, bb.y - (double)acc.y);
+    y[row] = from_d<XT>(rr);
+    red[0] += rr.x * rr.x + rr.y * rr.y;
+  } else {
+    y[row] = acc;
+    if (MODE == 1) {
+      const double2 pv = to_d(x[row]), qv = to_d(acc);
+      red[0] += pv.x * qv.x - pv.y * qv.y;
+      red[1] += pv.x * qv.y + pv.y * qv.x;
+    }
+  }
+}
+
+// G lanes per CSR row; the next row's extent is fetched before this row's work
 template <int G, int U, int MODE, typename VT, typename XT>
 __global__ void __launch_bounds__(kBS) k_spmv(const Scal* __restrict__ s, int64_t n,
                                               const int32_t* __restrict__ rowptr,
@@ -164,13 +211,23 @@ __global__ void __launch_bounds__(kBS) k_spmv(const Scal* __restrict__ s, int64_
                                               const double2* __restrict__ b, double* part) {
   if (MODE != 0 && s->done) return;
   constexpr int RPB = kBS / G;
-  int64_t lo, hi;
-  chunk(n, RPB, &lo, &hi);
+  const Sweep sw = xcd_sweep(n, RPB);
   const int lane = threadIdx.x % G;
   double red[2] = {0.0, 0.0};
-  for (int64_t row = lo + threadIdx.x / G; row < hi; row += RPB) {
+  int64_t row = sw.first + threadIdx.x / G;
+  int s0n = 0, s1n = 0;
+  if (row < sw.end) {
+    s0n = rowptr[row];
+    s1n = rowptr[row + 1];
+  }
+  for (; row < sw.end; row += sw.stride) {
+    const int s0 = s0n, s1 = s1n;
+    const int64_t nx = row + sw.stride;
+    if (nx < sw.end) {
+      s0n = rowptr[nx];
+      s1n = rowptr[nx + 1];
+    }
     XT acc = czero<XT>();
-    const int s0 = rowptr[row], s1 = rowptr[row + 1];
     for (int kb = s0 + lane; kb < s1; kb += G * U) {
       int c[U];
       VT a[U];
@@ -178,8 +235,8 @@ __global__ void __launch_bounds__(kBS) k_spmv(const Scal* __restrict__ s, int64_
 #pragma unroll
       for (int j = 0; j < U; ++j) {
         const int k = kb + j * G;
-        c[j] = k < s1 ? cols[k] : -1;
-        a[j] = k < s1 ? vals[k] : VT{0, 0};
+        c[j] = k < s1 ? ld_stream(cols + k) : -1;
+        a[j] = k < s1 ? ld_stream(vals + k) : VT{0, 0};
       }
 #pragma unroll
       for (int j = 0; j < U; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : czero<XT>();
@@ -188,62 +245,53 @@ __global__ void __launch_bounds__(kBS) k_spmv(const Scal* __restrict__ s, int64_
     }
     acc.x = group_sum<G>(acc.x);
     acc.y = group_sum<G>(acc.y);
-    if (lane == 0) {
-      if (MODE == 2) {
-        const double2 bb = b[row];
-        const double2 rr = make_double2(bb.x - (double)acc.x, bb.y - (double)acc.y);
-        y[row] = from_d<XT>(rr);
-        red[0] += rr.x * rr.x + rr.y * rr.y;
-      } else {
-        y[row] = acc;
-        if (MODE == 1) {
-          const double2 pv = to_d(x[row]), qv = to_d(acc);
-          red[0] += pv.x * qv.x - pv.y * qv.y;
-          red[1] += pv.x * qv.y + pv.y * qv.x;
-        }
-      }
-    }
+    if (lane == 0) spmv_row_out<MODE, XT>(row, acc, x, y, b, red);
   }
   if (MODE != 0) write_partials<2>(red, part);
 }
 
-// entity-row SpMV: the d rows of entity e (DOFs bptr[e]..bptr[e+1]) are
-// consecutive CSR rows of equal length L with the same column list
+// entity-row SpMV: the d rows of entity e (DOFs r0 .. r0+d) are consecutive CSR
+// rows of equal length L with the same column list starting at CSR slot
+// `base`; emeta[e] = {r0, d, base, L} (one 16-byte load, fetched one entity ahead)
 template <int G, int U, int D, int MODE, typename VT, typename XT>
 __global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int64_t nent,
-                                                 const int32_t* __restrict__ bptr,
-                                                 const int32_t* __restrict__ rowptr,
+                                                 const int4* __restrict__ emeta,
                                                  const int32_t* __restrict__ cols,
                                                  const VT* __restrict__ vals,
                                                  const XT* __restrict__ x, XT* __restrict__ y,
                                                  const double2* __restrict__ b, double* part) {
   if (MODE != 0 && s->done) return;
   constexpr int EPB = kBS / G;
-  int64_t lo, hi;
-  chunk(nent, EPB, &lo, &hi);
+  const Sweep sw = xcd_sweep(nent, EPB);
   const int lane = threadIdx.x % G;
   double red[2] = {0.0, 0.0};
-  for (int64_t ent = lo + threadIdx.x / G; ent < hi; ent += EPB) {
-    const int r0 = bptr[ent], d = bptr[ent + 1] - r0;
-    const int base = rowptr[r0];
-    const int L = rowptr[r0 + 1] - base;
+  int64_t ent = sw.first + threadIdx.x / G;
+  int4 mn = make_int4(0, 0, 0, 0);
+  if (ent < sw.end) mn = emeta[ent];
+  for (; ent < sw.end; ent += sw.stride) {
+    const int r0 = mn.x, d = mn.y, base = mn.z, L = mn.w;
+    const int64_t nx = ent + sw.stride;
+    if (nx < sw.end) mn = emeta[nx];
     XT acc[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) acc[k] = czero<XT>();
     for (int tb = lane; tb < L; tb += G * U) {
+      int c[U];
       XT xv[U];
 #pragma unroll
       for (int j = 0; j < U; ++j) {
         const int t = tb + j * G;
-        xv[j] = t < L ? x[cols[base + t]] : czero<XT>();
+        c[j] = t < L ? ld_stream(cols + base + t) : -1;
       }
+#pragma unroll
+      for (int j = 0; j < U; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : czero<XT>();
 #pragma unroll
       for (int k = 0; k < D; ++k)
         if (k < d)
 #pragma unroll
           for (int j = 0; j < U; ++j) {
             const int t = tb + j * G;
-            const VT a = t < L ? vals[base + k * L + t] : VT{0, 0};
+            const VT a = t < L ? ld_stream(vals + base + k * L + t) : VT{0, 0};
             acc[k] = cfma(XT{a.x, a.y}, xv[j], acc[k]);
           }
     }
@@ -255,22 +303,7 @@ __global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int
     if (lane == 0) {
 #pragma unroll
       for (int k = 0; k < D; ++k)
-        if (k < d) {
-          const int64_t row = r0 + k;
-          if (MODE == 2) {
-            const double2 bb = b[row];
-            const double2 rr = make_double2(bb.x - (double)acc[k].x, bb.y - (double)acc[k].y);
-            y[row] = from_d<XT>(rr);
-            red[0] += rr.x * rr.x + rr.y * rr.y;
-          } else {
-            y[row] = acc[k];
-            if (MODE == 1) {
-              const double2 pv = to_d(x[row]), qv = to_d(acc[k]);
-              red[0] += pv.x * qv.x - pv.y * qv.y;
-              red[1] += pv.x * qv.y + pv.y * qv.x;
-            }
-          }
-        }
+        if (k < d) spmv_row_out<MODE, XT>(r0 + k, acc[k], x, y, b, red);
     }
   }
   if (MODE != 0) write_partials<2>(red, part);
@@ -661,6 +694,15 @@ __global__ void k_check_entity_rows(int64_t nent, const int32_t* __restrict__ bp
       }
 }
 
+__global__ void k_entity_meta(int64_t nent, const int32_t* __restrict__ bptr,
+                              const int32_t* __restrict__ rowptr, int4* __restrict__ em) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nent) return;
+  const int r0 = bptr[e];
+  const int base = rowptr[r0];
+  em[e] = make_int4(r0, bptr[e + 1] - r0, base, rowptr[r0 + 1] - base);
+}
+
 __global__ void k_init_scalars(Scal* s, int maxit, double tol2) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Scal z{};
@@ -800,33 +842,33 @@ static void spmv_row(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t n,
 
 template <int MODE, typename VT, typename XT, int D>
 static void spmv_er_d(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t nent,
-                      const int32_t* bp, const int32_t* rp, const int32_t* ci, const VT* v,
-                      const XT* x, XT* y, const double2* b, double* part) {
+                      const int4* em, const int32_t* ci, const VT* v, const XT* x, XT* y,
+                      const double2* b, double* part) {
   const dim3 g(c.grid), bl(kBS);
 #define PG_ER(GG, UU)                                                                       \
   if (c.G == GG && c.U == UU) {                                                             \
-    hipLaunchKernelGGL((k_spmv_er<GG, UU, D, MODE, VT, XT>), g, bl, 0, st, s, nent, bp, rp, \
-                       ci, v, x, y, b, part);                                               \
+    hipLaunchKernelGGL((k_spmv_er<GG, UU, D, MODE, VT, XT>), g, bl, 0, st, s, nent, em, ci, \
+                       v, x, y, b, part);                                                   \
     return;                                                                                 \
   }
   PG_GU_LIST(PG_ER)
 #undef PG_ER
-  hipLaunchKernelGGL((k_spmv_er<16, 2, D, MODE, VT, XT>), g, bl, 0, st, s, nent, bp, rp, ci, v, x,
-                     y, b, part);
+  hipLaunchKernelGGL((k_spmv_er<16, 2, D, MODE, VT, XT>), g, bl, 0, st, s, nent, em, ci, v, x, y,
+                     b, part);
 }
 
 template <int MODE, typename VT, typename XT>
 static void spmv_launch(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t n, int64_t nent,
-                        const int32_t* bp, const int32_t* rp, const int32_t* ci, const VT* v,
+                        const int4* em, const int32_t* rp, const int32_t* ci, const VT* v,
                         const XT* x, XT* y, const double2* b, double* part) {
   if (c.kind == 0) {
     spmv_row<MODE, VT, XT>(c, st, s, n, rp, ci, v, x, y, b, part);
   } else if (c.D == 1) {
-    spmv_er_d<MODE, VT, XT, 1>(c, st, s, nent, bp, rp, ci, v, x, y, b, part);
+    spmv_er_d<MODE, VT, XT, 1>(c, st, s, nent, em, ci, v, x, y, b, part);
   } else if (c.D == 2) {
-    spmv_er_d<MODE, VT, XT, 2>(c, st, s, nent, bp, rp, ci, v, x, y, b, part);
+    spmv_er_d<MODE, VT, XT, 2>(c, st, s, nent, em, ci, v, x, y, b, part);
   } else {
-    spmv_er_d<MODE, VT, XT, 6>(c, st, s, nent, bp, rp, ci, v, x, y, b, part);
+    spmv_er_d<MODE, VT, XT, 6>(c, st, s, nent, em, ci, v, x, y, b, part);
   }
 }
 
@@ -854,6 +896,7 @@ struct Solver {
   int gb = 1, maxblk = 0;
   bool er_ok = false;
   int32_t *bptr = nullptr, *boff = nullptr;
+  int4* emeta = nullptr;  // entity-row SpMV descriptors {r0, d, base, L}
   double2* binv = nullptr;
   float2* binv32 = nullptr;
   // GMRES(m)
@@ -897,7 +940,7 @@ static int read_scalars(Solver* S, hipStream_t st) {
 template <int MODE, typename VT, typename XT>
 static void spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x, XT* y,
                  const double2* b) {
-  spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->bptr, S->rowptr, S->cols, v, x, y,
+  spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->emeta, S->rowptr, S->cols, v, x, y,
                             b, S->part);
 }
 
@@ -909,16 +952,16 @@ static int true_residual(Solver* S, hipStream_t st, const double2* b, const doub
   return PG_OK;
 }
 
+// average duration of the solve's own SpMV (MODE 1) over `reps` launches
 template <typename VT, typename XT>
 static float time_spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x,
-                       XT* y) {
+                       XT* y, int reps) {
   hipEvent_t a, b;
   hipEventCreate(&a);
   hipEventCreate(&b);
-  spmv<0, VT, XT>(S, c, st, v, x, y, nullptr);
+  spmv<1, VT, XT>(S, c, st, v, x, y, nullptr);
   hipEventRecord(a, st);
-  const int reps = 5;
-  for (int i = 0; i < reps; ++i) spmv<0, VT, XT>(S, c, st, v, x, y, nullptr);
+  for (int i = 0; i < reps; ++i) spmv<1, VT, XT>(S, c, st, v, x, y, nullptr);
   hipEventRecord(b, st);
   hipEventSynchronize(b);
   float ms = 0.f;
@@ -926,6 +969,28 @@ static float time_spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v,
   hipEventDestroy(a);
   hipEventDestroy(b);
   return ms / reps;
+}
+
+// fastest candidate: every candidate timed over 6 launches, the best three
+// again over 20 launches each
+template <typename VT, typename XT>
+static SpmvCfg pick_spmv(Solver* S, const std::vector<SpmvCfg>& cands, hipStream_t st,
+                         const VT* v, const XT* x, XT* y) {
+  std::vector<std::pair<float, int>> t;
+  for (size_t i = 0; i < cands.size(); ++i)
+    t.push_back({time_spmv<VT, XT>(S, cands[i], st, v, x, y, 6), (int)i});
+  std::sort(t.begin(), t.end());
+  const size_t top = std::min<size_t>(3, t.size());
+  float best = 1e30f;
+  int bi = t[0].second;
+  for (size_t k = 0; k < top; ++k) {
+    const float ms = time_spmv<VT, XT>(S, cands[t[k].second], st, v, x, y, 20);
+    if (ms < best) {
+      best = ms;
+      bi = t[k].second;
+    }
+  }
+  return cands[bi];
 }
 
 // preconditioned (re)start and iteration, point or entity-block Jacobi
@@ -1227,6 +1292,9 @@ static int autotune(Solver* S, hipStream_t st) {
     PG_HIP(hipMemcpyAsync(&S->h->iter, S->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
     S->er_ok = S->h->iter == 0;
+    if (S->er_ok)
+      hipLaunchKernelGGL(k_entity_meta, dim3(nblk(S->nblocks, kBS)), dim3(kBS), 0, st, S->nblocks,
+                         S->bptr, S->rowptr, S->emeta);
   }
   if (S->o.spmv == PG_SPMV_ENTITY && !S->er_ok)
     PG_FAIL(PG_ERR_ARG, "entity-row SpMV needs block_ptr ranges whose rows share one column list");
@@ -1247,26 +1315,14 @@ static int autotune(Solver* S, hipStream_t st) {
       cands.push_back(e);
     }
   }
-  // timing inputs: x = 1 (finite), y = FP64 scratch q (allocated in every mode)
+  // timing inputs: x = 1 (finite), y = FP64 scratch q (allocated in every mode);
+  // scalars cleared so that the MODE 1 kernels run to completion
+  hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, 1, 0.0);
   hipLaunchKernelGGL(k_ones, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->r64);
-  float best = 1e30f;
-  for (const SpmvCfg& c : cands) {
-    const float t = time_spmv<double2, double2>(S, c, st, S->vals, S->r64, S->q);
-    if (t < best) {
-      best = t;
-      S->c64 = c;
-    }
-  }
+  S->c64 = pick_spmv<double2, double2>(S, cands, st, S->vals, S->r64, S->q);
   if (mixed) {
     hipLaunchKernelGGL(k_ones32, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->p32);
-    best = 1e30f;
-    for (const SpmvCfg& c : cands) {
-      const float t = time_spmv<float2, float2>(S, c, st, S->vals32, S->p32, S->q32);
-      if (t < best) {
-        best = t;
-        S->c32 = c;
-      }
-    }
+    S->c32 = pick_spmv<float2, float2>(S, cands, st, S->vals32, S->p32, S->q32);
   }
   PG_CHECK_LAUNCH();
   return PG_OK;
@@ -1395,6 +1451,7 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
     S->gb = vgrid_for(S->nblocks);
     PG_A(S->bptr, S->nblocks + 1);
     PG_A(S->boff, S->nblocks + 1);
+    PG_A(S->emeta, S->nblocks);
     if (opts->pc == PG_PC_BJACOBI) {
       PG_A(S->binv, hboff[S->nblocks]);
       if (mixed) PG_A(S->binv32, hboff[S->nblocks]);

@@ -1,0 +1,90 @@
+"""Per-kernel timing of the forward hot path, for profilers.
+
+Runs each hot kernel a fixed number of times on one workload, so that a
+kernel-trace or hardware-counter profiler sees a short, regular program:
+
+* the solve's SpMV (``q = A p`` plus the ``p^T q`` partials) in complex128 and
+  complex64, with the configuration the solver's autotuner picked;
+* the numeric assembly (element geometry + fused CSR assembly);
+* a plain device copy of ``--copy-mib`` MiB (torch's copy kernel), whose
+  counter readings calibrate the HBM byte counters (bytes read and written
+  by a copy are known exactly).
+
+Usage::
+
+    python -m petgem_amd.kernel_bench --config c2 --reps 20 [--pc jacobi] [--copy-mib 1024]
+
+Prints one JSON line with the HIP-event average of each kernel.
+"""
+import argparse
+import json
+import sys
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--pc", default="jacobi", choices=["jacobi", "bjacobi", "none"])
+    ap.add_argument("--spmv", default="auto", choices=["auto", "row", "entity"])
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    args = ap.parse_args(argv)
+
+    import torch
+    from . import core
+    from .synthetic import CONFIGS, Workload
+
+    w = Workload(**CONFIGS[args.config], seed=2026)
+    T = w.topo
+    dev = torch.device("cuda")
+    mesh = core.DeviceMesh.from_topology(T, dev)
+    asm = core.Assembler(mesh)
+    plan = asm.plan
+    sigma = torch.from_numpy(w.sigma).to(dev)
+    _, dof_bc = mesh.entity_mask(T.boundary_dofs)
+    omega = core.omega_of(w.frequency)
+    values = torch.empty((plan.nnz, 2), dtype=torch.float64, device=dev)
+    asm.geometry(sigma)
+    asm.assemble(omega, out=values)
+    core.apply_dirichlet(plan, values, dof_bc)
+    out = dict(config=args.config, ndof=plan.ndof, nnz=plan.nnz, reps=args.reps)
+    bptr = mesh.block_ptr()
+    for prec in ("fp64", "mixed"):
+        S = core.KrylovSolver(plan, values, precision=prec, pc=args.pc, spmv=args.spmv,
+                              block_ptr=bptr)
+        prof = S.profile(reps=args.reps)
+        if prec == "fp64":
+            out["spmv_c128_ms"], out["spmv_c128_cfg"] = prof["ms64"], prof["g64"]
+        else:
+            out["spmv_c64_ms"], out["spmv_c64_cfg"] = prof["ms32"], prof["g32"]
+        del S
+    torch.cuda.synchronize()
+
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.reps):
+        asm.geometry(sigma)
+        asm.assemble(omega, out=values)
+    e1.record()
+    torch.cuda.synchronize()
+    out["assembly_ms"] = e0.elapsed_time(e1) / args.reps
+
+    nbytes = args.copy_mib << 20
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0.record()
+    for _ in range(args.reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.reps
+    out["copy_bytes_each_way"] = nbytes
+    out["copy_ms"] = ms
+    out["copy_GBps_rw"] = 2 * nbytes / (ms * 1e-3) / 1e9
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -27,7 +27,7 @@ class _CSR:
         self.cols = torch.from_numpy(A.indices.astype(np.int32)).to(DEV)
 
 
-@pytest.mark.parametrize("ksp,restart", [("gmres", 30), ("gmres", 15)])
+@pytest.mark.parametrize("ksp,restart", [("gmres", 30), ("gmres", 60)])
 def test_gmres_on_reference_matrix_fixture(ksp, restart):
     """tests/data/matrix-A.dat + vector-b.dat of the reference (4184 DOFs, p=1, BC rows)."""
     ip, ix, d, sh = petsc_io.read_mat(os.path.join(GOLDEN, "matrix-A.dat"))
