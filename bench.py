@@ -35,10 +35,26 @@ def spmv_bytes(nnz, n, bv):
     return nnz * (bv + 4) + 4 * (n + 1) + 2 * n * bv
 
 
-def spmv_name(code, vt):
-    """Kernel name from the solver's configuration code kind*1000 + G*10 + U."""
+def spmv_key(code, dmax, vt):
+    """Kernel name (as the profile summaries print it) of the solve's SpMV from the
+    solver's configuration code kind*1000 + G*10 + U."""
     kind, g, u = code // 1000, (code % 1000) // 10, code % 10
-    return (f"k_spmv_er<G={g},U={u}> (entity rows)" if kind else f"k_spmv<G={g},U={u}>") + f" {vt}"
+    if kind:
+        d = 1 if dmax <= 1 else (2 if dmax <= 2 else 6)
+        return f"k_spmv_er<{g},{u},{d},1,{vt},{vt}>"
+    return f"k_spmv<{g},{u},1,{vt},{vt}>"
+
+
+def measured_traffic(config):
+    """HBM bytes per launch of each SpMV configuration, from the committed
+    FETCH_SIZE / WRITE_SIZE profile of this workload ({} if there is none)."""
+    from petgem_amd.profsum import parse_traffic_table
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"spmv_traffic_{config}.md")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return parse_traffic_table(f.read())
 
 
 def cpu_baseline(w, plan_rowptr, plan_cols, values_c128, b_c128, iters_fp64, budget_s):
@@ -79,6 +95,105 @@ def cpu_baseline(w, plan_rowptr, plan_cols, values_c128, b_c128, iters_fp64, bud
                 assembly_s=t_asm, per_iteration_s=t_it)
 
 
+def batched_leg(args, log):
+    """Batched multi-RHS solve: 64 HED positions (100 m apart along x at z = -950 m)
+    x 4 frequencies on the 30^3 p=2 mesh, solved in chunks of --batch-width columns."""
+    import torch
+    from petgem_amd import core
+    from petgem_amd.synthetic import CONFIGS, Workload
+
+    w = Workload(**CONFIGS["c4"], seed=2026)
+    T = w.topo
+    dev = torch.device("cuda")
+    mesh = core.DeviceMesh.from_topology(T, dev)
+    asm = core.Assembler(mesh)
+    ent_bc, dof_bc = mesh.entity_mask(T.boundary_dofs)
+    sigma = torch.from_numpy(w.sigma).to(dev)
+    K = torch.empty(asm.plan.nnz, dtype=torch.float64, device=dev)
+    M = torch.empty_like(K)
+    xs = -3150.0 + 100.0 * np.arange(64)
+    srcs = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
+    src_el = core.locate_points(mesh, srcs).cpu().numpy()
+    freqs = [0.1, 0.25, 0.5, 1.0]
+    cols = [(si, f) for f in freqs for si in range(64)]
+    width = args.batch_width
+    chunks = [cols[i:i + width] for i in range(0, len(cols), width)]
+    asm.geometry(sigma)
+    core.assemble_km(asm, ent_bc, out=(K, M))
+    bptr, order = mesh.block_ptr(), mesh.entity_order()
+    solvers = [core.BatchSolver(asm.plan, K, M, [f for _, f in ch], rtol=args.rtol,
+                                block_ptr=bptr, entity_order=order) for ch in chunks]
+    B = torch.zeros((T.ndof, width, 2), dtype=torch.float64, device=dev)
+    times, iters = [], []
+    for it in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        asm.geometry(sigma)
+        core.assemble_km(asm, ent_bc, out=(K, M))
+        its, worst = [], 0.0
+        for ch, bs in zip(chunks, solvers):
+            bs.refresh()
+            for c, (si, f) in enumerate(ch):
+                b = core.source_vector(mesh, int(src_el[si]), srcs[si], (1.0, 0.0, 0.0),
+                                       core.omega_of(f))
+                core.apply_dirichlet(asm.plan, None, dof_bc, b)
+                B[:, c, :] = b
+            X, info = bs.solve(B)
+            if (info["status"] != 0).any():
+                raise RuntimeError(f"batched solve did not converge: {info}")
+            its.append(info["iterations"])
+            worst = max(worst, float(info["relres"].max()))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"batched step {it}: {dt*1e3:.1f} ms for {len(cols)} RHS, iterations per chunk "
+            f"{[int(i.max()) for i in its]}, worst relres {worst:.2e}")
+        if it >= args.warmup:
+            times.append(dt)
+            iters = [int(i.max()) for i in its]
+    t = float(np.median(times))
+    return {"rhs_per_s": round(len(cols) / t, 3), "s_per_batch": round(t, 4), "nrhs": len(cols),
+            "width": width, "chunk_iterations": iters,
+            "workload": (f"c4: p=2, 30^3 Kuhn x 6 = {T.n_elems} tets, {T.ndof} DOFs, "
+                         f"{asm.plan.nnz} nnz; 64 HED x-positions x {freqs} Hz, rtol {args.rtol}, "
+                         "COCG + Jacobi per column, K/M assembled once per batch")}
+
+
+def adjoint_leg(args, log):
+    """Adjoint gradient: 8 HED sources x 2 frequencies, 101 receivers, 30^3 p=2 mesh;
+    observations from the model with the resistive block, gradient at the model
+    without it, w.r.t. every element's log-conductivity."""
+    import torch
+    from petgem_amd.adjoint import InversionProblem
+    from petgem_amd.synthetic import Workload, receiver_line
+
+    w_true = Workload(30, 2, block=True, n_rx=101, seed=2026)
+    w0 = Workload(30, 2, block=False, n_rx=101, seed=2026)
+    T = w_true.topo
+    xs = np.linspace(-1750.0, 1750.0, 8)
+    src = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
+    rx = receiver_line(101, z=-1000.0)
+    prob = InversionProblem(T, src, [0.25, 1.0], rx, rtol=args.rtol, batched=True,
+                            device=torch.device("cuda"))
+    prob.observed = prob.forward(np.log(w_true.sigma[:, 0]))
+    m0 = torch.tensor(np.log(w0.sigma[:, 0]), dtype=torch.float64, device="cuda")
+    times = []
+    for it in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        phi, g = prob.misfit_and_gradient(m0)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"adjoint step {it}: {dt*1e3:.1f} ms, misfit {phi:.4e}, |g|max {float(g.abs().max()):.3e}")
+        if it >= args.warmup:
+            times.append(dt)
+    t = float(np.median(times))
+    return {"grad_per_s": round(1.0 / t, 4), "s_per_gradient": round(t, 4),
+            "n_params": int(T.n_elems), "solves_per_gradient": 32,
+            "workload": (f"c5: p=2, 30^3 Kuhn x 6 = {T.n_elems} tets, {T.ndof} DOFs; 8 HED x "
+                         "2 frequencies (0.25, 1 Hz), 101 receivers; forward + adjoint as two "
+                         f"batched 16-column solves, rtol {args.rtol}, + element contraction")}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,11 +203,14 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of host work for the CPU baseline sample (0 = skip)")
     ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--pc", default="jacobi", choices=["jacobi", "bjacobi"])
+    ap.add_argument("--pc", default="bjacobi", choices=["jacobi", "bjacobi"])
+    ap.add_argument("--legs", default="solve,batch,adjoint",
+                    help="comma list of: solve (headline), batch (RHS/s), adjoint (grad/s)")
+    ap.add_argument("--batch-width", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU sharding of the "
-                          "path is described in DESIGN.md but not implemented; run --gpus 1"}))
+                          "path is designed but not implemented; run --gpus 1"}))
         sys.exit(2)
 
     import torch
@@ -129,8 +247,9 @@ def main():
 
     assemble()
     bptr = mesh.block_ptr()
+    order = mesh.entity_order()
     solvers = {prec: core.KrylovSolver(plan, values, precision=prec, rtol=args.rtol, pc=args.pc,
-                                       block_ptr=bptr)
+                                       block_ptr=bptr, entity_order=order)
                for prec in ("fp64", "mixed")}
     results = {}
     for prec in ("fp64", "mixed"):
@@ -151,7 +270,9 @@ def main():
                 times.append(dt)
                 infos.append(info)
             log(f"{prec} step {it}: {dt*1e3:.1f} ms, {info}")
-        results[prec] = dict(t=float(np.median(times)), times=times, info=infos[-1], x=x)
+        results[prec] = dict(t=float(np.median(times)), times=times, info=infos[-1], x=x,
+                             spmv_ms=float(np.mean([i["spmv_ms"] for i in infos])),
+                             spmv_samples=int(np.sum([i["spmv_samples"] for i in infos])))
 
     # assembly alone (HIP events on the current stream)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -168,17 +289,23 @@ def main():
     f32 = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, results["mixed"]["x"], omega))
     field_diff = float(np.abs(f64 - f32).max() / np.abs(f64).max())
 
-    # roofline of the dominant kernel: the complex64 SpMV of the mixed solve
+    # roofline of the dominant kernel: the complex64 SpMV of the mixed solve.  Its
+    # average duration is sampled with HIP events on the solver's stream inside
+    # the timed solves; the standalone profile() timing is kept as a cross-check.
     prof = solvers["mixed"].profile(reps=20)
     prof64 = solvers["fp64"].profile(reps=20)
     n, nnz = plan.ndof, plan.nnz
+    dmax = int(np.diff(bptr).max())
     by32 = spmv_bytes(nnz, n, 8)
     by64 = spmv_bytes(nnz, n, 16)
-    ach32 = by32 / (prof["ms32"] * 1e-3) / 1e9
-    ach64 = by64 / (prof64["ms64"] * 1e-3) / 1e9
-    k32, k64 = spmv_name(prof["g32"], "complex64"), spmv_name(prof64["g64"], "complex128")
-    log(f"SpMV {k32} {prof['ms32']:.3f} ms ({ach32:.0f} GB/s), "
-        f"{k64} {prof64['ms64']:.3f} ms ({ach64:.0f} GB/s); assembly {t_asm_ms:.2f} ms")
+    ms32, ms64 = results["mixed"]["spmv_ms"], results["fp64"]["spmv_ms"]
+    ach32 = by32 / (ms32 * 1e-3) / 1e9
+    ach64 = by64 / (ms64 * 1e-3) / 1e9
+    k32, k64 = spmv_key(prof["g32"], dmax, "float2"), spmv_key(prof64["g64"], dmax, "double2")
+    traffic = measured_traffic(args.config)
+    log(f"SpMV {k32} {ms32:.3f} ms in the solve ({ach32:.0f} GB/s; standalone "
+        f"{prof['ms32']:.3f} ms), {k64} {ms64:.3f} ms ({ach64:.0f} GB/s); "
+        f"assembly {t_asm_ms:.2f} ms; traffic {traffic.get(k32)} B")
 
     cpu = None
     if args.cpu_budget > 0:
@@ -188,6 +315,12 @@ def main():
         cpu = cpu_baseline(w, plan.h_rowptr, plan.h_cols, vals_h, b_h,
                            results["fp64"]["info"]["iterations"], args.cpu_budget)
         log(f"cpu baseline {cpu['value']:.1f} s")
+
+    legs = set(args.legs.split(","))
+    del solvers, values
+    torch.cuda.empty_cache()
+    batched = batched_leg(args, log) if "batch" in legs else None
+    adjoint = adjoint_leg(args, log) if "adjoint" in legs else None
 
     t_mixed, t_fp64 = results["mixed"]["t"], results["fp64"]["t"]
     line = {
@@ -221,13 +354,18 @@ def main():
         "receiver_field_diff_mixed_vs_fp64": field_diff,
         "assembly_ms": round(t_asm_ms, 3),
         "roofline": {
-            "bound": "hbm", "kernel": f"{k32} (SpMV + p^T q partials)",
+            "bound": "hbm", "kernel": f"{k32} (q = A p + p^T q partials)",
             "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": None,
-            "algorithmic_bytes": by32, "avg_ms": round(prof["ms32"], 5),
-            "fp64_spmv": {"kernel": k64, "achieved": round(ach64, 1), "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                          "avg_ms": round(prof64["ms64"], 5), "algorithmic_bytes": by64},
+            "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic.get(k32),
+            "algorithmic_bytes": by32, "avg_ms": round(ms32, 5),
+            "avg_ms_samples": results["mixed"]["spmv_samples"],
+            "avg_ms_standalone": round(prof["ms32"], 5),
+            "fp64_spmv": {"kernel": k64, "achieved": round(ach64, 1),
+                          "frac": round(ach64 / HBM_PEAK_GBS, 4), "avg_ms": round(ms64, 5),
+                          "algorithmic_bytes": by64, "traffic": traffic.get(k64)},
         },
+        "batched": batched,
+        "adjoint": adjoint,
         "cpu_baseline": None if cpu is None else {
             k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "host": platform.processor() or platform.machine(),

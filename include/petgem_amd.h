@@ -161,7 +161,7 @@ int pg_receiver_fields(int p, int64_t n_points, const double *points, const int3
 
 /* ------------------------------------------------------------------------
  * Adjoint gradient of a receiver misfit with respect to per-element
- * log-conductivity (no counterpart in the reference; BASELINE extension).
+ * log-conductivity (an extension with no counterpart in the reference).
  * ---------------------------------------------------------------------- */
 /* Adjoint source f[dofs_e] += sum_c coef[r][c] N_c(x_r) for located receivers
  * (coef: (n_points,3) complex128 = weight * conj(E_pred - E_obs)). */
@@ -220,6 +220,10 @@ typedef struct pg_solve_opts {
   int64_t nblocks;
   int32_t spmv;       /* PG_SPMV_*: SpMV kernel family (AUTO = fastest by timing) */
   int32_t pad;
+  const int32_t *entity_order; /* optional host permutation of the nblocks entities: the
+                                  order in which the entity-row SpMV visits them (a
+                                  spatially local order keeps the gathers of x in cache);
+                                  NULL = natural order */
 } pg_solve_opts;
 
 typedef struct pg_solve_info {
@@ -228,6 +232,10 @@ typedef struct pg_solve_info {
   int32_t status;           /* PG_OK, PG_ERR_BREAKDOWN, or -1 = not converged */
   int32_t pad;
   double relres;            /* true FP64 ||b - A x|| / ||b|| at exit */
+  double spmv_ms;           /* COCG: mean duration of the iterations' SpMV kernel in this
+                               solve (HIP events around one launch per host check) */
+  int32_t spmv_samples;     /* launches sampled for spmv_ms */
+  int32_t pad2;
 } pg_solve_info;
 
 /* Create a solver for an n x n complex symmetric CSR matrix on the device
@@ -249,7 +257,7 @@ int pg_solver_profile(void *solver, int reps, double *ms64, double *ms32, int32_
 
 /* ------------------------------------------------------------------------
  * Batched multi-RHS solve on one mesh (many sources and frequencies in one
- * launch; BASELINE extension, no reference counterpart -- the reference
+ * launch; an extension with no counterpart in the reference, which
  * creates one KSP per right-hand side, petgem/solver.py:584-590).
  * A_r = K - i*omega_r*mu0*M from pg_assemble_km; vectors are row-major
  * (ndof, nrhs) complex128.  nrhs: power of two < 64, or 64/128/192/256.
@@ -262,6 +270,16 @@ int pg_batch_create(int64_t n, int64_t nnz, const int32_t *rowptr, const int32_t
  * PG_ERR_BREAKDOWN) into host arrays (each nullable). */
 int pg_batch_solve(void *batch, const double *b, double *x, int32_t *iterations, double *relres,
                    int32_t *status, void *stream);
+/* Optional: the mesh entities' DOF ranges (host array, nblocks+1 entries, sizes
+ * 1..6; the rows of an entity must share one column list, as the assembly
+ * plan's CSR guarantees) and the order in which to visit them (host
+ * permutation, NULL = natural).  Switches the solve to the entity-row SpMM, in
+ * which each gathered row of the search directions serves all rows of an
+ * entity; the batch keeps schedule-order copies of the column lists, K and M. */
+int pg_batch_set_entities(void *batch, const int32_t *block_ptr, int64_t nblocks,
+                          const int32_t *order, void *stream);
+/* Re-derive the Jacobi diagonals after the caller has reassembled K and M in place. */
+int pg_batch_refresh(void *batch, void *stream);
 int pg_batch_destroy(void *batch);
 
 #ifdef __cplusplus

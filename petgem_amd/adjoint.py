@@ -19,6 +19,11 @@ mass matrices are recomputed on the device inside the contraction kernel,
 never stored.  Boundary rows/columns are not differentiated (the PEC rows
 are fixed), which the device code realises by zeroing the adjoint source on
 boundary DOFs.
+
+With ``batched=True`` every (source, frequency) pair becomes one column of the
+batched multi-RHS solver (one pass over K and M per iteration serves all
+columns), for the forward and for the adjoint solves, and the contraction
+reads the solutions in the solver's (ndof, nrhs) layout.
 """
 import math
 
@@ -29,11 +34,25 @@ from . import _native as nat
 from . import core
 
 
+def batch_width(r):
+    """Smallest width of the batched solver holding r columns (powers of two
+    below 64, multiples of 64 up to 256); extra columns are zero."""
+    if not 1 <= r <= 256:
+        raise ValueError("a batch holds 1..256 columns")
+    if r <= 64:
+        w = 1
+        while w < r:
+            w *= 2
+        return w
+    return -(-r // 64) * 64
+
+
 class InversionProblem:
     """Forward fields, misfit and adjoint gradient for many sources and frequencies."""
 
     def __init__(self, topo, source_positions, frequencies, receivers, observed=None,
-                 rtol=1e-10, precision="fp64", moment=(1.0, 0.0, 0.0), device=None):
+                 rtol=1e-10, precision="fp64", moment=(1.0, 0.0, 0.0), device=None,
+                 batched=False, max_batch=64):
         self.topo = topo
         self.mesh = core.DeviceMesh.from_topology(topo, device)
         self.asm = core.Assembler(self.mesh)
@@ -51,9 +70,21 @@ class InversionProblem:
             raise ValueError("a receiver lies outside the mesh")
         self.rx_dev = torch.from_numpy(self.rx).to(self.mesh.device)
         nnz = self.asm.plan.nnz
-        self.values = [torch.empty((nnz, 2), dtype=torch.float64, device=self.mesh.device)
-                       for _ in self.freqs]
-        self.solvers = [None] * len(self.freqs)
+        self.batched = bool(batched)
+        if self.batched:
+            if precision != "fp64":
+                raise ValueError("the batched solver runs in FP64")
+            self.K = torch.empty(nnz, dtype=torch.float64, device=self.mesh.device)
+            self.M = torch.empty_like(self.K)
+            cols = [(si, fi) for fi in range(len(self.freqs)) for si in range(len(self.sources))]
+            mb = max(1, min(int(max_batch), 256))
+            self.chunks = [cols[i:i + mb] for i in range(0, len(cols), mb)]
+            self.batches = [None] * len(self.chunks)
+            self.X = None
+        else:
+            self.values = [torch.empty((nnz, 2), dtype=torch.float64, device=self.mesh.device)
+                           for _ in self.freqs]
+            self.solvers = [None] * len(self.freqs)
         self.observed = None if observed is None else np.asarray(observed)
         self.x = None
         self.stats = dict(solves=0, iterations=0)
@@ -64,6 +95,19 @@ class InversionProblem:
             np.asarray(log_sigma, dtype=np.float64))
         sig = torch.exp(ls.to(self.mesh.device, torch.float64))
         self.asm.geometry(torch.stack([sig, sig], dim=1).contiguous())
+        if self.batched:
+            core.assemble_km(self.asm, self.ent_bc, out=(self.K, self.M))
+            for ci, ch in enumerate(self.chunks):
+                if self.batches[ci] is None:
+                    w = batch_width(len(ch))
+                    fr = [self.freqs[fi] for _, fi in ch] + [self.freqs[ch[0][1]]] * (w - len(ch))
+                    self.batches[ci] = core.BatchSolver(self.asm.plan, self.K, self.M, fr,
+                                                        rtol=self.rtol,
+                                                        block_ptr=self.mesh.block_ptr(),
+                                                        entity_order=self.mesh.entity_order())
+                else:
+                    self.batches[ci].refresh()
+            return
         for fi, f in enumerate(self.freqs):
             self.asm.assemble(core.omega_of(f), ent_bc=self.ent_bc, out=self.values[fi])
             if self.solvers[fi] is None:
@@ -80,6 +124,41 @@ class InversionProblem:
         self.stats["iterations"] += info["iterations"]
         return x
 
+    def _solve_batch(self, ci, column):
+        """Solve chunk ci with right-hand side column(si, fi) -> (ndof, 2) per pair."""
+        bs = self.batches[ci]
+        ch = self.chunks[ci]
+        B = torch.zeros((self.mesh.ndof, bs.nrhs, 2), dtype=torch.float64, device=self.mesh.device)
+        for c, (si, fi) in enumerate(ch):
+            B[:, c, :] = column(si, fi)
+        X, info = bs.solve(B)
+        st = info["status"][:len(ch)]
+        if (st != 0).any():
+            raise RuntimeError(f"batched Krylov solve failed: status {st}, "
+                               f"relres {info['relres'][:len(ch)]}")
+        self.stats["solves"] += len(ch)
+        self.stats["iterations"] += int(info["iterations"][:len(ch)].sum())
+        return X
+
+    def _source(self, si, fi):
+        b = core.source_vector(self.mesh, int(self.src_el[si]), self.sources[si], self.moment,
+                               core.omega_of(self.freqs[fi]))
+        self._mask(b)
+        return b
+
+    def _adjoint_source(self, si, fi, coef):
+        m = self.mesh
+        cdev = core.from_complex(coef, m.device)
+        rhs = torch.zeros((m.ndof, 2), dtype=torch.float64, device=m.device)
+        nat.check(nat.lib().pg_adjoint_rhs(m.p, self.rx.shape[0], nat.ptr(self.rx_dev),
+                                           nat.ptr(self.rx_el), nat.ptr(m.coords),
+                                           nat.ptr(m.elems_nodes), nat.ptr(m.elems_edges),
+                                           nat.ptr(m.edges_nodes), nat.ptr(m.faces_edges),
+                                           nat.ptr(m.dofs), nat.ptr(cdev), nat.ptr(rhs),
+                                           nat.stream_handle()))
+        self._mask(rhs)
+        return rhs
+
     def _mask(self, v):
         nat.check(nat.lib().pg_apply_dirichlet(self.mesh.ndof, nat.ptr(self.asm.plan.rowptr),
                                                nat.ptr(self.asm.plan.cols), nat.ptr(self.dof_bc),
@@ -89,8 +168,19 @@ class InversionProblem:
         """Receiver E fields (S, F, R, 3) complex128 and the solutions kept on the device."""
         self._set_model(log_sigma)
         S, F = self.sources.shape[0], len(self.freqs)
-        self.x = [[None] * F for _ in range(S)]
         out = np.zeros((S, F, self.rx.shape[0], 3), dtype=np.complex128)
+        if self.batched:
+            self.X = []
+            for ci, ch in enumerate(self.chunks):
+                X = self._solve_batch(ci, self._source)
+                self.X.append(X)
+                for c, (si, fi) in enumerate(ch):
+                    fld = core.receiver_fields(self.mesh, self.rx, self.rx_el,
+                                               X[:, c, :].contiguous(),
+                                               core.omega_of(self.freqs[fi]))
+                    out[si, fi] = core.as_complex(fld)[:, 0:3]
+            return out
+        self.x = [[None] * F for _ in range(S)]
         for fi, f in enumerate(self.freqs):
             om = core.omega_of(f)
             for si in range(S):
@@ -118,20 +208,27 @@ class InversionProblem:
         g = torch.zeros(m.n_elems, dtype=torch.float64, device=m.device)
         w = 1.0 / np.maximum(np.abs(self.observed) ** 2, 1e-300)
         L = nat.lib()
+
+        def adjoint_source(si, fi):
+            coef = w[si, fi] * np.conj(fields[si, fi] - self.observed[si, fi])
+            return self._adjoint_source(si, fi, coef)
+
+        if self.batched:
+            for ci, ch in enumerate(self.chunks):
+                lam = self._solve_batch(ci, adjoint_source)
+                width = lam.shape[1]
+                omega = np.zeros(width, dtype=np.float64)
+                omega[:len(ch)] = [core.omega_of(self.freqs[fi]) for _, fi in ch]
+                nat.check(L.pg_adjoint_gradient_rows(self.asm.tables.handle, m.n_elems,
+                                                     nat.ptr(self.asm.workspace), nat.ptr(m.dofs),
+                                                     width, m.ndof, nat.ptr(self.X[ci]),
+                                                     nat.ptr(lam), nat.ptr(omega), nat.ptr(g),
+                                                     nat.stream_handle()))
+            return phi, g
         for fi, f in enumerate(self.freqs):
             om = core.omega_of(f)
             for si in range(self.sources.shape[0]):
-                coef = w[si, fi] * np.conj(fields[si, fi] - self.observed[si, fi])
-                cdev = core.from_complex(coef, m.device)
-                rhs = torch.zeros((m.ndof, 2), dtype=torch.float64, device=m.device)
-                nat.check(L.pg_adjoint_rhs(m.p, self.rx.shape[0], nat.ptr(self.rx_dev),
-                                           nat.ptr(self.rx_el), nat.ptr(m.coords),
-                                           nat.ptr(m.elems_nodes), nat.ptr(m.elems_edges),
-                                           nat.ptr(m.edges_nodes), nat.ptr(m.faces_edges),
-                                           nat.ptr(m.dofs), nat.ptr(cdev), nat.ptr(rhs),
-                                           nat.stream_handle()))
-                self._mask(rhs)
-                lam = self._solve(fi, rhs)
+                lam = self._solve(fi, adjoint_source(si, fi))
                 omega = np.array([om], dtype=np.float64)
                 nat.check(L.pg_adjoint_gradient(self.asm.tables.handle, m.n_elems,
                                                 nat.ptr(self.asm.workspace), nat.ptr(m.dofs), 1,

@@ -44,6 +44,20 @@ def dipole_moment(source):
     return il * rot
 
 
+def _spread3(v):
+    """Spread the low 10 bits of v so that two zero bits follow each one."""
+    v = v.astype(np.uint64) & np.uint64(0x3FF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x030000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x0300F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x030C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x09249249)
+    return v
+
+
+def _morton3(x, y, z):
+    return _spread3(x) | (_spread3(y) << np.uint64(1)) | (_spread3(z) << np.uint64(2))
+
+
 class DeviceMesh:
     """Per-element arrays in the reference's scratch layout, resident on the device."""
 
@@ -64,9 +78,11 @@ class DeviceMesh:
         self.edges_nodes = _i32(np.asarray(edges_nodes).reshape(self.n_elems, 12), dev)
         self.faces_edges = _i32(np.asarray(faces_edges).reshape(self.n_elems, 12), dev)
         self.dofs = _i32(dofs, dev)
-        # host copies needed by the (host-side, integer) assembly plan
+        # host copies needed by the (host-side, integer) assembly plan and the
+        # entity schedule
         self.h_elems_edges = np.ascontiguousarray(elems_edges, dtype=np.int32)
         self.h_elems_faces = np.ascontiguousarray(elems_faces, dtype=np.int32)
+        self.h_centroids = np.asarray(coords, dtype=np.float64).reshape(self.n_elems, 4, 3).mean(1)
 
     @classmethod
     def from_topology(cls, topo, device=None):
@@ -83,6 +99,28 @@ class DeviceMesh:
         if dv:
             sizes.append(np.full(self.n_elems, dv))
         return np.concatenate([[0], np.cumsum(np.concatenate(sizes))]).astype(np.int32)
+
+    def entity_order(self):
+        """The DOF-carrying entities (block_ptr order: edges, faces, volumes) sorted
+        by the 3-D Morton code of their position (the centroid of an element that
+        contains them, on a 1024^3 grid over the mesh's bounding box).  Visiting
+        the entity rows in this order keeps the SpMV's gathers spatially local."""
+        de, df, dv, _ = dof_counts(self.p)
+        c = self.h_centroids
+        parts = []
+        pe = np.empty((self.n_edges, 3))
+        pe[self.h_elems_edges.ravel()] = np.repeat(c, 6, axis=0)
+        parts.append(pe)
+        if df:
+            pf = np.empty((self.n_faces, 3))
+            pf[self.h_elems_faces.ravel()] = np.repeat(c, 4, axis=0)
+            parts.append(pf)
+        if dv:
+            parts.append(c)
+        pos = np.concatenate(parts)
+        lo, hi = pos.min(0), pos.max(0)
+        q = np.clip(((pos - lo) / np.maximum(hi - lo, 1e-300) * 1023.0).astype(np.int64), 0, 1023)
+        return np.argsort(_morton3(q[:, 0], q[:, 1], q[:, 2]), kind="stable").astype(np.int32)
 
     def entity_mask(self, boundary_dofs):
         """(per-entity, per-DOF) uint8 boundary masks from a list of boundary DOFs."""
@@ -201,11 +239,15 @@ class Assembler:
         return out
 
 
-def assemble_km(assembler, ent_bc=None, stream=None):
-    """Stiffness and mass parts of the assembled system, each (nnz,) float64."""
+def assemble_km(assembler, ent_bc=None, out=None, stream=None):
+    """Stiffness and mass parts of the assembled system, each (nnz,) float64
+    (written into ``out = (K, M)`` when given)."""
     pl = assembler.plan
-    K = torch.empty(pl.nnz, dtype=torch.float64, device=assembler.mesh.device)
-    M = torch.empty_like(K)
+    if out is None:
+        K = torch.empty(pl.nnz, dtype=torch.float64, device=assembler.mesh.device)
+        M = torch.empty_like(K)
+    else:
+        K, M = out
     nat.check(nat.lib().pg_assemble_km(assembler.tables.handle, ctypes.byref(pl.dev_struct),
                                        nat.ptr(assembler.workspace), nat.ptr(ent_bc), nat.ptr(K),
                                        nat.ptr(M), nat.stream_handle(stream)))
@@ -215,7 +257,8 @@ def assemble_km(assembler, ent_bc=None, stream=None):
 class BatchSolver:
     """Many right-hand sides (each with its own frequency) in one batched COCG."""
 
-    def __init__(self, plan, K, M, frequencies, rtol=1e-8, maxit=100000, stream=None):
+    def __init__(self, plan, K, M, frequencies, rtol=1e-8, maxit=100000, block_ptr=None,
+                 entity_order=None, stream=None):
         self.plan, self.K, self.M = plan, K, M
         self.omega = np.array([omega_of(f) for f in frequencies], dtype=np.float64)
         self.nrhs = self.omega.size
@@ -225,6 +268,18 @@ class BatchSolver:
                                             self.nrhs, nat.ptr(self.omega), float(rtol), int(maxit),
                                             ctypes.byref(h), nat.stream_handle(stream)))
         self.handle = h
+        if block_ptr is not None:
+            bp = np.ascontiguousarray(block_ptr, dtype=np.int32)
+            od = None if entity_order is None else np.ascontiguousarray(entity_order, np.int32)
+            if od is not None and od.size != bp.size - 1:
+                raise ValueError("entity_order needs one entry per block_ptr entity")
+            nat.check(nat.lib().pg_batch_set_entities(self.handle, bp.ctypes.data, bp.size - 1,
+                                                      None if od is None else od.ctypes.data,
+                                                      nat.stream_handle(stream)))
+
+    def refresh(self, stream=None):
+        """Re-derive the Jacobi diagonals after K and M were reassembled in place."""
+        nat.check(nat.lib().pg_batch_refresh(self.handle, nat.stream_handle(stream)))
 
     def solve(self, B, stream=None):
         """B: (ndof, nrhs, 2) float64 (complex interleaved, row-major); returns X, info."""
@@ -313,7 +368,7 @@ class KrylovSolver:
 
     def __init__(self, plan, values, ksp="cocg", pc="jacobi", precision="fp64", rtol=1e-8,
                  maxit=100000, check_every=25, inner_rtol=1e-4, restart=30, block_ptr=None,
-                 spmv="auto", stream=None):
+                 spmv="auto", entity_order=None, stream=None):
         self.plan = plan
         self.values = values
         if pc == "bjacobi" and block_ptr is None:
@@ -321,10 +376,16 @@ class KrylovSolver:
         # block_ptr (the mesh entities' DOF ranges) also enables the entity-row SpMV
         self._bptr = None if block_ptr is None else np.ascontiguousarray(block_ptr, np.int32)
         nb = 0 if self._bptr is None else self._bptr.size - 1
+        # entity_order (DeviceMesh.entity_order()): the entity-row SpMV's visiting order
+        self._order = None if entity_order is None else np.ascontiguousarray(entity_order,
+                                                                            np.int32)
+        if self._order is not None and self._order.size != nb:
+            raise ValueError("entity_order needs one entry per block_ptr entity")
         self.opts = nat.SolveOpts(_KSP[ksp], _PC[pc], _PREC[precision], int(restart), int(maxit),
                                   int(check_every), float(rtol), float(inner_rtol),
                                   None if self._bptr is None else self._bptr.ctypes.data, nb,
-                                  _SPMV[spmv], 0)
+                                  _SPMV[spmv], 0,
+                                  None if self._order is None else self._order.ctypes.data)
         h = ctypes.c_void_p()
         nat.check(nat.lib().pg_solver_create(plan.ndof, plan.nnz, nat.ptr(plan.rowptr),
                                              nat.ptr(plan.cols), nat.ptr(values),
@@ -352,7 +413,8 @@ class KrylovSolver:
         nat.check(nat.lib().pg_solver_solve(self.handle, nat.ptr(b), nat.ptr(x),
                                             ctypes.byref(info), nat.stream_handle(stream)))
         return x, dict(iterations=info.iterations, refinements=info.refinements,
-                       status=info.status, relres=info.relres)
+                       status=info.status, relres=info.relres, spmv_ms=info.spmv_ms,
+                       spmv_samples=info.spmv_samples)
 
     def __del__(self):
         try:

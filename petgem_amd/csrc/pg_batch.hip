@@ -166,6 +166,138 @@ __global__ void __launch_bounds__(kBS) k_spmm(const Ctl* ctl, int64_t n, int R,
   column_partials<CT, 2, TMAX>(R, v, part);
 }
 
+// Entity-row SpMM over the batch's entity schedule: the d rows of an entity
+// (DOFs r0 .. r0+d) share one column list of length L; column lists and the
+// d x L blocks of K and M are kept in schedule order (from cbase / vbase), so
+// they stream sequentially, and each gathered row of P serves all d rows.
+// em[i] = {r0, d | L << 8, cbase, vbase}.  Lanes c of an entity slot cover the
+// columns; U column indices are loaded before their gathers, and the next
+// entity's descriptor is fetched one step ahead.
+template <int CT, int TMAX, int D>
+__global__ void __launch_bounds__(kBS) k_spmm_er(const Ctl* ctl, int64_t nent, int R,
+                                                 const int4* __restrict__ em,
+                                                 const int32_t* __restrict__ cols,
+                                                 const double* __restrict__ K,
+                                                 const double* __restrict__ M,
+                                                 const double* __restrict__ omu,
+                                                 const double2* __restrict__ P,
+                                                 double2* __restrict__ Q, double* part) {
+  if (ctl->all_done) return;
+  constexpr int EPB = kBS / CT;
+  constexpr int U = 4;
+  const int c = threadIdx.x % CT, es = threadIdx.x / CT;
+  const int T = R / CT;
+  const Sweep sw = xcd_sweep(nent, EPB);
+  double v[TMAX][2];
+  double om[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    v[t][0] = v[t][1] = 0.0;
+    om[t] = t < T ? omu[c + t * CT] : 0.0;
+  }
+  int64_t ent = sw.first + es;
+  int4 mn = make_int4(0, 0, 0, 0);
+  if (ent < sw.end) mn = em[ent];
+  for (; ent < sw.end; ent += sw.stride) {
+    const int r0 = mn.x, d = mn.y & 0xff, L = mn.y >> 8, cbase = mn.z, vbase = mn.w;
+    const int64_t nx = ent + sw.stride;
+    if (nx < sw.end) mn = em[nx];
+    double2 acc[D][TMAX];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+      for (int t = 0; t < TMAX; ++t) acc[k][t] = make_double2(0.0, 0.0);
+    for (int j0 = 0; j0 < L; j0 += U) {
+      int64_t cj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) cj[u] = j0 + u < L ? (int64_t)cols[cbase + j0 + u] * R : -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (cj[u] < 0) continue;
+        double2 x[TMAX];
+#pragma unroll
+        for (int t = 0; t < TMAX; ++t)
+          x[t] = t < T ? P[cj[u] + c + t * CT] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+          if (k < d) {
+            const double kv = K[vbase + k * L + j0 + u], mv = M[vbase + k * L + j0 + u];
+#pragma unroll
+            for (int t = 0; t < TMAX; ++t)
+              if (t < T) {
+                const double ai = -om[t] * mv;
+                acc[k][t].x += kv * x[t].x - ai * x[t].y;
+                acc[k][t].y += kv * x[t].y + ai * x[t].x;
+              }
+          }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      if (k < d)
+#pragma unroll
+        for (int t = 0; t < TMAX; ++t)
+          if (t < T) {
+            const int64_t idx = (int64_t)(r0 + k) * R + c + t * CT;
+            Q[idx] = acc[k][t];
+            const double2 pv = P[idx];
+            v[t][0] += pv.x * acc[k][t].x - pv.y * acc[k][t].y;
+            v[t][1] += pv.x * acc[k][t].y + pv.y * acc[k][t].x;
+          }
+  }
+  column_partials<CT, 2, TMAX>(R, v, part);
+}
+
+// schedule-order copies: entity i's column list and its d x L blocks of K, M
+// (contiguous in the CSR from slot src[i])
+__global__ void __launch_bounds__(kBS) k_sched_cols(int64_t nent, const int4* __restrict__ em,
+                                                    const int32_t* __restrict__ src,
+                                                    const int32_t* __restrict__ cols,
+                                                    int32_t* __restrict__ cols_s) {
+  constexpr int G = 32;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (i >= nent) return;
+  const int4 m = em[i];
+  const int L = m.y >> 8, base = src[i];
+  for (int t = threadIdx.x % G; t < L; t += G) cols_s[m.z + t] = cols[base + t];
+}
+__global__ void __launch_bounds__(kBS) k_sched_km(int64_t nent, const int4* __restrict__ em,
+                                                  const int32_t* __restrict__ src,
+                                                  const double* __restrict__ K,
+                                                  const double* __restrict__ M,
+                                                  double* __restrict__ Ks, double* __restrict__ Ms) {
+  constexpr int G = 32;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (i >= nent) return;
+  const int4 m = em[i];
+  const int len = (m.y & 0xff) * (m.y >> 8), base = src[i];
+  for (int t = threadIdx.x % G; t < len; t += G) {
+    Ks[m.w + t] = K[base + t];
+    Ms[m.w + t] = M[base + t];
+  }
+}
+
+// entity rows must be consecutive rows of equal length sharing one column list
+__global__ void k_check_entities(int64_t nent, const int32_t* __restrict__ bptr,
+                                 const int32_t* __restrict__ rowptr,
+                                 const int32_t* __restrict__ cols, int32_t* bad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nent) return;
+  const int r0 = bptr[e], d = bptr[e + 1] - r0;
+  const int base = rowptr[r0], L = rowptr[r0 + 1] - base;
+  for (int k = 1; k <= d; ++k)
+    if (rowptr[r0 + k] != base + k * L) {
+      bad[0] = 1;
+      return;
+    }
+  for (int k = 1; k < d; ++k)
+    for (int t = 0; t < L; ++t)
+      if (cols[base + k * L + t] != cols[base + t]) {
+        bad[0] = 1;
+        return;
+      }
+}
+
 // finish mu_r and alpha_r = rho_r / mu_r (one workgroup per column)
 __global__ void __launch_bounds__(kBS) k_fin_mu(const Ctl* ctl, Col* col, int R,
                                                 const double* part, int np) {
@@ -413,6 +545,86 @@ __global__ void __launch_bounds__(kBS) k_fin_true(Col* col, int R, const double*
   if (threadIdx.x == 0) col[blockIdx.x].res2 = tot[0];
 }
 
+// restart from the current iterate: r = b - A x, z = r / diag, p = 0;
+// partials of rho = r^T z and |r|^2 per column
+template <int CT, int TMAX>
+__global__ void __launch_bounds__(kBS) k_reinit(int64_t n, int R,
+                                                const int32_t* __restrict__ rowptr,
+                                                const int32_t* __restrict__ cols,
+                                                const double* __restrict__ K,
+                                                const double* __restrict__ M,
+                                                const double* __restrict__ dK,
+                                                const double* __restrict__ dM,
+                                                const double* __restrict__ omu,
+                                                const double2* __restrict__ X,
+                                                const double2* __restrict__ B,
+                                                double2* __restrict__ Rr, double2* __restrict__ Z,
+                                                double2* __restrict__ P, double* part) {
+  constexpr int RPB = kBS / CT;
+  const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
+  const int T = R / CT;
+  const Sweep sw = xcd_sweep(n, RPB);
+  double v[TMAX][3];
+  double om[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    v[t][0] = v[t][1] = v[t][2] = 0.0;
+    om[t] = t < T ? omu[c + t * CT] : 0.0;
+  }
+  for (int64_t row = sw.first + rs; row < sw.end; row += sw.stride) {
+    double2 acc[TMAX];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t) acc[t] = make_double2(0.0, 0.0);
+    for (int k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+      const double kv = K[k], mv = M[k];
+      const int64_t cj = (int64_t)cols[k] * R;
+#pragma unroll
+      for (int t = 0; t < TMAX; ++t)
+        if (t < T) {
+          const double2 x = X[cj + c + t * CT];
+          const double ai = -om[t] * mv;
+          acc[t].x += kv * x.x - ai * x.y;
+          acc[t].y += kv * x.y + ai * x.x;
+        }
+    }
+    const double kd = dK[row], md = dM[row];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int64_t idx = row * R + c + t * CT;
+        const double2 bv = B[idx];
+        const double2 rv = make_double2(bv.x - acc[t].x, bv.y - acc[t].y);
+        Rr[idx] = rv;
+        const double2 zv = zdiv(rv, make_double2(kd, -om[t] * md));
+        Z[idx] = zv;
+        P[idx] = make_double2(0.0, 0.0);
+        v[t][0] += rv.x * zv.x - rv.y * zv.y;
+        v[t][1] += rv.x * zv.y + rv.y * zv.x;
+        v[t][2] += rv.x * rv.x + rv.y * rv.y;
+      }
+  }
+  column_partials<CT, 3, TMAX>(R, v, part);
+}
+
+// finish a restart: columns whose true residual already meets the tolerance
+// (or that stopped on breakdown or the iteration limit) stay done
+__global__ void __launch_bounds__(kBS) k_fin_reinit(const Ctl* ctl, Col* col, int R,
+                                                    const double* part, int np) {
+  double tot[3];
+  column_totals<3>(part, R, np, tot);
+  if (threadIdx.x != 0) return;
+  Col& cc = col[blockIdx.x];
+  cc.rho = make_double2(tot[0], tot[1]);
+  cc.res2 = tot[2];
+  cc.beta = make_double2(0.0, 0.0);
+  cc.alpha = make_double2(0.0, 0.0);
+  cc.done = (cc.status != 0 || cc.iter >= ctl->maxit || tot[2] <= ctl->tol2 * cc.bnorm2) ? 1 : 0;
+}
+
+__global__ void k_ctl_resume(Ctl* ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl->all_done = 0;
+}
+
 __global__ void k_diag(int64_t n, const int32_t* __restrict__ rowptr,
                        const int32_t* __restrict__ cols, const double* __restrict__ K,
                        const double* __restrict__ M, double* __restrict__ dK,
@@ -464,6 +676,13 @@ struct Batch {
   int32_t* bad = nullptr;
   double rtol = 1e-8;
   int maxit = 100000, check = 25;
+  // entity rows (optional): schedule descriptors {r0, d | L << 8, cbase, vbase},
+  // each scheduled entity's CSR slot, the scheduled column lists and K, M
+  int64_t nent = 0;
+  int4* em = nullptr;
+  int32_t *esrc = nullptr, *cols_s = nullptr;
+  double *Ks = nullptr, *Ms = nullptr;
+  int dmax = 0, egrid = 1;
   std::vector<void*> owned;
   ~Batch() {
     for (void* v : owned) hipFree(v);
@@ -485,9 +704,24 @@ template <int CT, int TM>
 static void launch_iter(Batch* B, hipStream_t st, double2* X) {
   const dim3 g(B->grid), b(kBS), gr(B->R);
   hipLaunchKernelGGL((k_pupdate<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->Z, B->P);
-  hipLaunchKernelGGL((k_spmm<CT, TM>), g, b, 0, st, B->ctl, B->n, B->R, B->rowptr, B->cols, B->K,
-                     B->M, B->omu, B->P, B->Q, B->part);
-  hipLaunchKernelGGL(k_fin_mu, gr, b, 0, st, B->ctl, B->col, B->R, B->part, B->grid);
+  int np = B->grid;
+  if (B->em) {
+    const dim3 ge(B->egrid);
+    np = B->egrid;
+    if (B->dmax <= 1)
+      hipLaunchKernelGGL((k_spmm_er<CT, TM, 1>), ge, b, 0, st, B->ctl, B->nent, B->R, B->em,
+                         B->cols_s, B->Ks, B->Ms, B->omu, B->P, B->Q, B->part);
+    else if (B->dmax <= 2)
+      hipLaunchKernelGGL((k_spmm_er<CT, TM, 2>), ge, b, 0, st, B->ctl, B->nent, B->R, B->em,
+                         B->cols_s, B->Ks, B->Ms, B->omu, B->P, B->Q, B->part);
+    else
+      hipLaunchKernelGGL((k_spmm_er<CT, TM, 6>), ge, b, 0, st, B->ctl, B->nent, B->R, B->em,
+                         B->cols_s, B->Ks, B->Ms, B->omu, B->P, B->Q, B->part);
+  } else {
+    hipLaunchKernelGGL((k_spmm<CT, TM>), g, b, 0, st, B->ctl, B->n, B->R, B->rowptr, B->cols,
+                       B->K, B->M, B->omu, B->P, B->Q, B->part);
+  }
+  hipLaunchKernelGGL(k_fin_mu, gr, b, 0, st, B->ctl, B->col, B->R, B->part, np);
   hipLaunchKernelGGL((k_update<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK, B->dM,
                      B->omu, X, B->Rr, B->P, B->Q, B->Z, B->part);
   hipLaunchKernelGGL(k_fin_update, gr, b, 0, st, B->ctl, B->col, B->R, B->part, B->grid);
@@ -506,6 +740,34 @@ static void launch_true(Batch* B, hipStream_t st, const double2* Bv, const doubl
   hipLaunchKernelGGL((k_true_res<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->rowptr,
                      B->cols, B->K, B->M, B->omu, X, Bv, B->part);
   hipLaunchKernelGGL(k_fin_true, dim3(B->R), dim3(kBS), 0, st, B->col, B->R, B->part, B->grid);
+}
+
+// Jacobi diagonals K_ii, M_ii from the caller's K and M, and the schedule-order
+// copies of K and M (at creation and after the caller has reassembled them in place)
+static int refresh_diag(Batch* B, hipStream_t st) {
+  if (B->em)
+    hipLaunchKernelGGL(k_sched_km, dim3((32 * B->nent + kBS - 1) / kBS), dim3(kBS), 0, st, B->nent,
+                       B->em, B->esrc, B->K, B->M, B->Ks, B->Ms);
+  PG_HIP(hipMemsetAsync(B->bad, 0, sizeof(int32_t), st));
+  hipLaunchKernelGGL(k_diag, dim3((B->n + kBS - 1) / kBS), dim3(kBS), 0, st, B->n, B->rowptr,
+                     B->cols, B->K, B->M, B->dK, B->dM, B->bad);
+  PG_CHECK_LAUNCH();
+  PG_HIP(hipMemcpyAsync(&B->hctl->maxit, B->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  if (B->hctl->maxit != 0)
+    PG_FAIL(PG_ERR_ARG, "zero diagonal entry: Jacobi preconditioner undefined");
+  return PG_OK;
+}
+
+template <int CT, int TM>
+static void launch_reinit(Batch* B, hipStream_t st, const double2* Bv, const double2* X) {
+  hipLaunchKernelGGL((k_reinit<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->rowptr,
+                     B->cols, B->K, B->M, B->dK, B->dM, B->omu, X, Bv, B->Rr, B->Z, B->P,
+                     B->part);
+  hipLaunchKernelGGL(k_fin_reinit, dim3(B->R), dim3(kBS), 0, st, B->ctl, B->col, B->R, B->part,
+                     B->grid);
+  hipLaunchKernelGGL(k_ctl_resume, dim3(1), dim3(64), 0, st, B->ctl);
+  hipLaunchKernelGGL(k_fin_all, dim3(1), dim3(kBS), 0, st, B->ctl, B->col, B->R);
 }
 
 // (CT, TM) configurations: R = CT * T with T <= TM
@@ -588,19 +850,10 @@ int pg_batch_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t
   std::vector<double> om(R);
   for (int r = 0; r < R; ++r) om[r] = omega[r] * 4.0 * 3.14159265358979323846 * 1e-7;
   hipMemcpy(B->omu, om.data(), R * sizeof(double), hipMemcpyHostToDevice);
-  hipMemsetAsync(B->bad, 0, sizeof(int32_t), st);
-  hipLaunchKernelGGL(k_diag, dim3((n + kBS - 1) / kBS), dim3(kBS), 0, st, n, rowptr, cols, K, M,
-                     B->dK, B->dM, B->bad);
-  int32_t bad = 0;
-  hipMemcpyAsync(&B->hctl->maxit, B->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st);
-  if (hipStreamSynchronize(st) != hipSuccess) {
+  const int rc_diag = refresh_diag(B, st);
+  if (rc_diag != PG_OK) {
     delete B;
-    PG_FAIL(PG_ERR_HIP, "batch setup failed");
-  }
-  bad = B->hctl->maxit;
-  if (bad) {
-    delete B;
-    PG_FAIL(PG_ERR_ARG, "zero diagonal entry: Jacobi preconditioner undefined");
+    return rc_diag;
   }
   *out = B;
   return PG_OK;
@@ -617,17 +870,30 @@ int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations,
                      B->R);
   PG_BATCH_DISPATCH(launch_init, B, st, Bv, X);
   PG_CHECK_LAUNCH();
-  while (true) {
-    for (int k = 0; k < B->check; ++k) PG_BATCH_DISPATCH(launch_iter, B, st, X);
+  // iterate until every column's recursion has converged, then check the true
+  // residuals; columns above the tolerance restart from their current iterate
+  for (int restart = 0;; ++restart) {
+    while (true) {
+      for (int k = 0; k < B->check; ++k) PG_BATCH_DISPATCH(launch_iter, B, st, X);
+      PG_CHECK_LAUNCH();
+      PG_HIP(hipMemcpyAsync(B->hctl, B->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+      PG_HIP(hipStreamSynchronize(st));
+      if (B->hctl->all_done) break;
+    }
+    PG_BATCH_DISPATCH(launch_true, B, st, Bv, X);
     PG_CHECK_LAUNCH();
-    PG_HIP(hipMemcpyAsync(B->hctl, B->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipMemcpyAsync(B->hcol, B->col, B->R * sizeof(Col), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
-    if (B->hctl->all_done) break;
+    bool again = false;
+    for (int r = 0; r < B->R; ++r) {
+      const Col& c = B->hcol[r];
+      if (c.status == 0 && c.iter < B->maxit && c.res2 > B->rtol * B->rtol * c.bnorm2)
+        again = true;
+    }
+    if (!again || restart >= 20) break;
+    PG_BATCH_DISPATCH(launch_reinit, B, st, Bv, X);
+    PG_CHECK_LAUNCH();
   }
-  PG_BATCH_DISPATCH(launch_true, B, st, Bv, X);
-  PG_CHECK_LAUNCH();
-  PG_HIP(hipMemcpyAsync(B->hcol, B->col, B->R * sizeof(Col), hipMemcpyDeviceToHost, st));
-  PG_HIP(hipStreamSynchronize(st));
   for (int r = 0; r < B->R; ++r) {
     const Col& c = B->hcol[r];
     const double rel = c.bnorm2 > 0 ? std::sqrt(c.res2 / c.bnorm2) : 0.0;
@@ -636,6 +902,84 @@ int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations,
     if (status) status[r] = c.status != 0 ? c.status : (rel <= B->rtol * 1.0000001 ? 0 : -1);
   }
   return PG_OK;
+}
+
+int pg_batch_set_entities(void* batch, const int32_t* block_ptr, int64_t nblocks,
+                          const int32_t* order, void* stream) {
+  if (!batch || !block_ptr || nblocks <= 0) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  Batch* B = static_cast<Batch*>(batch);
+  if (B->em) PG_FAIL(PG_ERR_ARG, "entities already set");
+  if (block_ptr[0] != 0 || block_ptr[nblocks] != B->n)
+    PG_FAIL(PG_ERR_ARG, "block_ptr must start at 0 and end at n");
+  int dmax = 0;
+  for (int64_t k = 0; k < nblocks; ++k) {
+    const int d = block_ptr[k + 1] - block_ptr[k];
+    if (d < 1 || d > 6) PG_FAIL(PG_ERR_ARG, "entity sizes must be in 1..6");
+    dmax = std::max(dmax, d);
+  }
+  if (order) {
+    std::vector<char> seen(nblocks, 0);
+    for (int64_t i = 0; i < nblocks; ++i) {
+      if (order[i] < 0 || order[i] >= nblocks || seen[order[i]])
+        PG_FAIL(PG_ERR_ARG, "order must be a permutation of 0..nblocks-1");
+      seen[order[i]] = 1;
+    }
+  }
+  hipStream_t st = as_stream(stream);
+  int32_t* bp = nullptr;
+  int rc;
+  if ((rc = B->alloc(&bp, nblocks + 1)) != PG_OK) return rc;
+  PG_HIP(hipMemcpyAsync(bp, block_ptr, (nblocks + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  PG_HIP(hipMemsetAsync(B->bad, 0, sizeof(int32_t), st));
+  hipLaunchKernelGGL(k_check_entities, dim3((nblocks + kBS - 1) / kBS), dim3(kBS), 0, st, nblocks,
+                     bp, B->rowptr, B->cols, B->bad);
+  PG_CHECK_LAUNCH();
+  PG_HIP(hipMemcpyAsync(&B->hctl->maxit, B->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  if (B->hctl->maxit != 0)
+    PG_FAIL(PG_ERR_ARG, "the rows of an entity must be consecutive with one shared column list");
+  // schedule descriptors on the host
+  std::vector<int32_t> rp(B->n + 1);
+  PG_HIP(hipMemcpy(rp.data(), B->rowptr, (B->n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<int4> em(nblocks);
+  std::vector<int32_t> src(nblocks);
+  int64_t cb = 0, vb = 0;
+  for (int64_t i = 0; i < nblocks; ++i) {
+    const int e = order ? order[i] : (int)i;
+    const int r0 = block_ptr[e], d = block_ptr[e + 1] - r0;
+    const int base = rp[r0], L = rp[r0 + 1] - base;
+    if (L >= (1 << 23)) PG_FAIL(PG_ERR_ARG, "entity row too long");
+    em[i] = make_int4(r0, d | (L << 8), (int)cb, (int)vb);
+    src[i] = base;
+    cb += L;
+    vb += (int64_t)d * L;
+    if (cb >= (1LL << 31) || vb >= (1LL << 31)) PG_FAIL(PG_ERR_ARG, "matrix too large");
+  }
+  if (vb != B->nnz) PG_FAIL(PG_ERR_ARG, "the entity rows do not cover the matrix");
+  int4* dem = nullptr;
+  if ((rc = B->alloc(&dem, nblocks)) != PG_OK || (rc = B->alloc(&B->esrc, nblocks)) != PG_OK ||
+      (rc = B->alloc(&B->cols_s, cb)) != PG_OK || (rc = B->alloc(&B->Ks, B->nnz)) != PG_OK ||
+      (rc = B->alloc(&B->Ms, B->nnz)) != PG_OK)
+    return rc;
+  PG_HIP(hipMemcpy(dem, em.data(), nblocks * sizeof(int4), hipMemcpyHostToDevice));
+  PG_HIP(hipMemcpy(B->esrc, src.data(), nblocks * sizeof(int32_t), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_sched_cols, dim3((32 * nblocks + kBS - 1) / kBS), dim3(kBS), 0, st,
+                     nblocks, dem, B->esrc, B->cols, B->cols_s);
+  PG_CHECK_LAUNCH();
+  // partials are sized for kMaxGrid workgroups
+  const int64_t epb = kBS / B->CT;
+  int64_t g = std::min<int64_t>(kMaxGrid, (nblocks + epb - 1) / epb);
+  if (g >= 64) g = (g / 8) * 8;
+  B->egrid = (int)std::max<int64_t>(1, g);
+  B->nent = nblocks;
+  B->dmax = dmax;
+  B->em = dem;
+  return refresh_diag(B, st);
+}
+
+int pg_batch_refresh(void* batch, void* stream) {
+  if (!batch) PG_FAIL(PG_ERR_ARG, "batch is NULL");
+  return refresh_diag(static_cast<Batch*>(batch), as_stream(stream));
 }
 
 int pg_batch_destroy(void* batch) {

@@ -250,9 +250,12 @@ __global__ void __launch_bounds__(kBS) k_spmv(const Scal* __restrict__ s, int64_
   if (MODE != 0) write_partials<2>(red, part);
 }
 
-// entity-row SpMV: the d rows of entity e (DOFs r0 .. r0+d) are consecutive CSR
-// rows of equal length L with the same column list starting at CSR slot
-// `base`; emeta[e] = {r0, d, base, L} (one 16-byte load, fetched one entity ahead)
+// Entity-row SpMV over the solver's entity schedule.  The d rows of an entity
+// (DOFs r0 .. r0+d) share one column list of length L; the solver keeps the
+// column lists and the d x L value blocks in schedule order (cols_s from
+// cbase, vals_s from vbase), so both stream sequentially while the gathered x
+// stays spatially local.  em[i] = {r0, d | L << 8, cbase, vbase} (one 16-byte
+// load, fetched one entity ahead).
 template <int G, int U, int D, int MODE, typename VT, typename XT>
 __global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int64_t nent,
                                                  const int4* __restrict__ emeta,
@@ -269,7 +272,7 @@ __global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int
   int4 mn = make_int4(0, 0, 0, 0);
   if (ent < sw.end) mn = emeta[ent];
   for (; ent < sw.end; ent += sw.stride) {
-    const int r0 = mn.x, d = mn.y, base = mn.z, L = mn.w;
+    const int r0 = mn.x, d = mn.y & 0xff, L = mn.y >> 8, cbase = mn.z, vbase = mn.w;
     const int64_t nx = ent + sw.stride;
     if (nx < sw.end) mn = emeta[nx];
     XT acc[D];
@@ -281,7 +284,7 @@ __global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int
 #pragma unroll
       for (int j = 0; j < U; ++j) {
         const int t = tb + j * G;
-        c[j] = t < L ? ld_stream(cols + base + t) : -1;
+        c[j] = t < L ? ld_stream(cols + cbase + t) : -1;
       }
 #pragma unroll
       for (int j = 0; j < U; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : czero<XT>();
@@ -291,7 +294,7 @@ __global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int
 #pragma unroll
           for (int j = 0; j < U; ++j) {
             const int t = tb + j * G;
-            const VT a = t < L ? ld_stream(vals + base + k * L + t) : VT{0, 0};
+            const VT a = t < L ? ld_stream(vals + vbase + k * L + t) : VT{0, 0};
             acc[k] = cfma(XT{a.x, a.y}, xv[j], acc[k]);
           }
     }
@@ -307,6 +310,32 @@ __global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int
     }
   }
   if (MODE != 0) write_partials<2>(red, part);
+}
+
+// scheduled copies: entity i's column list (L entries from CSR slot src[i]) and
+// its d x L value block (contiguous in the CSR) into the schedule-order arrays
+__global__ void __launch_bounds__(kBS) k_sched_cols(int64_t nent, const int4* __restrict__ em,
+                                                    const int32_t* __restrict__ src,
+                                                    const int32_t* __restrict__ cols,
+                                                    int32_t* __restrict__ cols_s) {
+  constexpr int G = 32;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (i >= nent) return;
+  const int4 m = em[i];
+  const int L = m.y >> 8, base = src[i];
+  for (int t = threadIdx.x % G; t < L; t += G) cols_s[m.z + t] = cols[base + t];
+}
+template <typename VT>
+__global__ void __launch_bounds__(kBS) k_sched_vals(int64_t nent, const int4* __restrict__ em,
+                                                    const int32_t* __restrict__ src,
+                                                    const double2* __restrict__ vals,
+                                                    VT* __restrict__ vals_s) {
+  constexpr int G = 32;
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (i >= nent) return;
+  const int4 m = em[i];
+  const int len = (m.y & 0xff) * (m.y >> 8), base = src[i];
+  for (int t = threadIdx.x % G; t < len; t += G) vals_s[m.w + t] = from_d<VT>(vals[base + t]);
 }
 
 // finish mu = p^T q, alpha = rho / mu
@@ -694,15 +723,6 @@ __global__ void k_check_entity_rows(int64_t nent, const int32_t* __restrict__ bp
       }
 }
 
-__global__ void k_entity_meta(int64_t nent, const int32_t* __restrict__ bptr,
-                              const int32_t* __restrict__ rowptr, int4* __restrict__ em) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nent) return;
-  const int r0 = bptr[e];
-  const int base = rowptr[r0];
-  em[e] = make_int4(r0, bptr[e + 1] - r0, base, rowptr[r0 + 1] - base);
-}
-
 __global__ void k_init_scalars(Scal* s, int maxit, double tol2) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   Scal z{};
@@ -896,7 +916,12 @@ struct Solver {
   int gb = 1, maxblk = 0;
   bool er_ok = false;
   int32_t *bptr = nullptr, *boff = nullptr;
-  int4* emeta = nullptr;  // entity-row SpMV descriptors {r0, d, base, L}
+  // entity-row SpMV in schedule order: descriptors {r0, d | L << 8, cbase, vbase},
+  // each scheduled entity's CSR slot, and the scheduled column lists and values
+  int4* emeta = nullptr;
+  int32_t* esrc = nullptr;
+  int32_t* cols_s = nullptr;
+  double2* vals_s64 = nullptr;
   double2* binv = nullptr;
   float2* binv32 = nullptr;
   // GMRES(m)
@@ -906,6 +931,7 @@ struct Solver {
   double* partg = nullptr;
   double* dnorm = nullptr;
   int restart = 30;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // SpMV sampling inside the solve
   std::vector<void*> owned;
 
   template <typename T>
@@ -919,6 +945,8 @@ struct Solver {
     return PG_OK;
   }
   ~Solver() {
+    for (hipEvent_t e : ev)
+      if (e) hipEventDestroy(e);
     for (void* v : owned) hipFree(v);
     if (h) hipHostFree(h);
     if (hhost) hipHostFree(hhost);
@@ -937,11 +965,20 @@ static int read_scalars(Solver* S, hipStream_t st) {
   return PG_OK;
 }
 
+// the entity-row kind reads the scheduled copies (complex128: vals_s64;
+// complex64: vals32, which is kept in schedule order whenever entity rows exist)
+static inline const double2* sched_values(Solver* S, const double2*) { return S->vals_s64; }
+static inline const float2* sched_values(Solver* S, const float2*) { return S->vals32; }
+
 template <int MODE, typename VT, typename XT>
 static void spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x, XT* y,
                  const double2* b) {
-  spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->emeta, S->rowptr, S->cols, v, x, y,
-                            b, S->part);
+  if (c.kind == 1)
+    spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->emeta, S->rowptr, S->cols_s,
+                              sched_values(S, v), x, y, b, S->part);
+  else
+    spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, nullptr, S->rowptr, S->cols, v, x, y,
+                              b, S->part);
 }
 
 // r64 = b - A x (FP64) and |r64|^2 into s->res64
@@ -1012,9 +1049,12 @@ static void launch_reset(Solver* S, hipStream_t st, XT* r, const XT* dinv, const
 
 template <typename VT, typename XT>
 static void launch_iteration(Solver* S, hipStream_t st, const SpmvCfg& c, const VT* vals, XT* x,
-                             XT* r, XT* p, XT* q, const XT* dinv, const XT* binv, XT* z) {
+                             XT* r, XT* p, XT* q, const XT* dinv, const XT* binv, XT* z,
+                             bool sample) {
   hipLaunchKernelGGL(k_pupdate<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, z, p);
+  if (sample) hipEventRecord(S->ev[0], st);
   spmv<1, VT, XT>(S, c, st, vals, p, q, nullptr);
+  if (sample) hipEventRecord(S->ev[1], st);
   hipLaunchKernelGGL(k_fin_mu, dim3(1), dim3(kBS), 0, st, S->s, S->part, c.grid);
   int np;
   if (S->o.pc == PG_PC_BJACOBI) {
@@ -1045,8 +1085,12 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
   PG_TRY(read_scalars(S, st));
   const double bnorm2 = S->h->bnorm2;
   int refinements = 0, restarts = 0;
+  double spmv_ms = 0.0;
+  int spmv_n = 0;
   info->status = -1;
   info->refinements = 0;
+  info->spmv_ms = 0.0;
+  info->spmv_samples = 0;
   if (bnorm2 == 0.0) {
     PG_HIP(hipMemsetAsync(x, 0, n * sizeof(double2), st));
     info->iterations = 0;
@@ -1065,16 +1109,25 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
   if (mixed) hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
   const int check = std::max(1, S->o.check_every);
   while (true) {
+    // the first iteration of every batch runs in full (the flags were clear
+    // when the batch was enqueued): time its SpMV with events on this stream
     for (int k = 0; k < check; ++k) {
       if (mixed)
         launch_iteration<float2, float2>(S, st, S->c32, S->vals32, S->x32, S->r32, S->p32, S->q32,
-                                         S->dinv32, S->binv32, S->z32);
+                                         S->dinv32, S->binv32, S->z32, k == 0);
       else
         launch_iteration<double2, double2>(S, st, S->c64, S->vals, x, S->r, S->p, S->q, S->dinv,
-                                           S->binv, S->z);
+                                           S->binv, S->z, k == 0);
     }
     PG_CHECK_LAUNCH();
     PG_TRY(read_scalars(S, st));
+    {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, S->ev[0], S->ev[1]) == hipSuccess) {
+        spmv_ms += ms;
+        ++spmv_n;
+      }
+    }
     const Scal hs = *S->h;
     if (hs.status == PG_ERR_BREAKDOWN) {
       info->status = PG_ERR_BREAKDOWN;
@@ -1110,6 +1163,8 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
   PG_TRY(read_scalars(S, st));
   info->iterations = S->h->iter;
   info->refinements = refinements;
+  info->spmv_samples = spmv_n;
+  info->spmv_ms = spmv_n ? spmv_ms / spmv_n : 0.0;
   info->relres = std::sqrt(res64 / bnorm2);
   return PG_OK;
 }
@@ -1239,15 +1294,24 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
 }
 
 // (Re)build what depends on the caller's matrix values: Jacobi or block
-// inverses and, for mixed precision, the complex64 copy of A.
+// inverses, the schedule-order copies of A (entity rows) and, for mixed
+// precision, the complex64 copy of A.
 static int refresh_values(Solver* S, hipStream_t st) {
   const int64_t n = S->n, nnz = S->nnz;
   const bool mixed = S->o.precision == PG_PREC_MIXED;
   PG_HIP(hipMemsetAsync(S->bad, 0, sizeof(int32_t), st));
-  if (mixed)
+  if (S->er_ok) {
+    const dim3 g(nblk(32 * S->nblocks, kBS));
+    hipLaunchKernelGGL(k_sched_vals<double2>, g, dim3(kBS), 0, st, S->nblocks, S->emeta, S->esrc,
+                       S->vals, S->vals_s64);
+    if (mixed)
+      hipLaunchKernelGGL(k_sched_vals<float2>, g, dim3(kBS), 0, st, S->nblocks, S->emeta, S->esrc,
+                         S->vals, S->vals32);
+  } else if (mixed) {
     hipLaunchKernelGGL(k_demote, dim3(nblk(2 * nnz, kBS)), dim3(kBS), 0, st, 2 * nnz,
                        reinterpret_cast<const double*>(S->vals),
                        reinterpret_cast<float*>(S->vals32));
+  }
   if (S->o.pc == PG_PC_BJACOBI) {
     hipLaunchKernelGGL(k_bjacobi_setup, dim3(nblk(S->nblocks, kBS)), dim3(kBS), 0, st,
                        S->nblocks, S->bptr, S->boff, S->rowptr, S->cols, S->vals, S->binv,
@@ -1282,20 +1346,63 @@ static int refresh_values(Solver* S, hipStream_t st) {
   return PG_OK;
 }
 
+// Entity rows: check that the rows of every entity (block_ptr range) share one
+// column list, then lay out the schedule (entity_order, or natural order):
+// descriptors, column lists and value blocks in schedule order.
+static int setup_entities(Solver* S, const int32_t* bp, const int32_t* order, hipStream_t st) {
+  const int64_t nb = S->nblocks;
+  if (S->o.spmv == PG_SPMV_ROW) {  // CSR-row kernels only: no schedule copies
+    S->er_ok = false;
+    return PG_OK;
+  }
+  PG_HIP(hipMemsetAsync(S->bad, 0, sizeof(int32_t), st));
+  hipLaunchKernelGGL(k_check_entity_rows, dim3(nblk(nb, kBS)), dim3(kBS), 0, st, nb, S->bptr,
+                     S->rowptr, S->cols, S->bad);
+  PG_CHECK_LAUNCH();
+  PG_HIP(hipMemcpyAsync(&S->h->iter, S->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  S->er_ok = S->h->iter == 0;
+  if (!S->er_ok) return PG_OK;
+  std::vector<int32_t> rp(S->n + 1);
+  PG_HIP(hipMemcpy(rp.data(), S->rowptr, (S->n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (order) {
+    std::vector<char> seen(nb, 0);
+    for (int64_t i = 0; i < nb; ++i) {
+      if (order[i] < 0 || order[i] >= nb || seen[order[i]])
+        PG_FAIL(PG_ERR_ARG, "entity_order must be a permutation of 0..nblocks-1");
+      seen[order[i]] = 1;
+    }
+  }
+  std::vector<int4> em(nb);
+  std::vector<int32_t> src(nb);
+  int64_t cb = 0, vb = 0;
+  for (int64_t i = 0; i < nb; ++i) {
+    const int e = order ? order[i] : (int)i;
+    const int r0 = bp[e], d = bp[e + 1] - r0;
+    const int base = rp[r0], L = rp[r0 + 1] - base;
+    if (L >= (1 << 23)) PG_FAIL(PG_ERR_ARG, "entity row too long");
+    em[i] = make_int4(r0, d | (L << 8), (int)cb, (int)vb);
+    src[i] = base;
+    cb += L;
+    vb += (int64_t)d * L;
+    if (cb >= (1LL << 31) || vb >= (1LL << 31)) PG_FAIL(PG_ERR_ARG, "matrix too large");
+  }
+  if (vb != S->nnz) PG_FAIL(PG_ERR_ARG, "the entity rows do not cover the matrix");
+  int rc;
+  if ((rc = S->alloc(&S->esrc, nb)) != PG_OK || (rc = S->alloc(&S->cols_s, cb)) != PG_OK ||
+      (rc = S->alloc(&S->vals_s64, S->nnz)) != PG_OK)
+    return rc;
+  PG_HIP(hipMemcpy(S->emeta, em.data(), nb * sizeof(int4), hipMemcpyHostToDevice));
+  PG_HIP(hipMemcpy(S->esrc, src.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_sched_cols, dim3(nblk(32 * nb, kBS)), dim3(kBS), 0, st, nb, S->emeta,
+                     S->esrc, S->cols, S->cols_s);
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
 // pick the fastest SpMV configuration for this matrix (row / entity-row, G, U)
 static int autotune(Solver* S, hipStream_t st) {
   const bool mixed = S->o.precision == PG_PREC_MIXED;
-  if (S->bptr) {
-    PG_HIP(hipMemsetAsync(S->bad, 0, sizeof(int32_t), st));
-    hipLaunchKernelGGL(k_check_entity_rows, dim3(nblk(S->nblocks, kBS)), dim3(kBS), 0, st,
-                       S->nblocks, S->bptr, S->rowptr, S->cols, S->bad);
-    PG_HIP(hipMemcpyAsync(&S->h->iter, S->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    PG_HIP(hipStreamSynchronize(st));
-    S->er_ok = S->h->iter == 0;
-    if (S->er_ok)
-      hipLaunchKernelGGL(k_entity_meta, dim3(nblk(S->nblocks, kBS)), dim3(kBS), 0, st, S->nblocks,
-                         S->bptr, S->rowptr, S->emeta);
-  }
   if (S->o.spmv == PG_SPMV_ENTITY && !S->er_ok)
     PG_FAIL(PG_ERR_ARG, "entity-row SpMV needs block_ptr ranges whose rows share one column list");
   std::vector<SpmvCfg> cands;
@@ -1321,8 +1428,13 @@ static int autotune(Solver* S, hipStream_t st) {
   hipLaunchKernelGGL(k_ones, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->r64);
   S->c64 = pick_spmv<double2, double2>(S, cands, st, S->vals, S->r64, S->q);
   if (mixed) {
+    // the complex64 copy has one layout: schedule order when entity rows exist
+    std::vector<SpmvCfg> c32;
+    for (const SpmvCfg& c : cands)
+      if ((c.kind == 1) == S->er_ok) c32.push_back(c);
+    if (c32.empty()) PG_FAIL(PG_ERR_ARG, "no SpMV kernel for the complex64 layout");
     hipLaunchKernelGGL(k_ones32, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->p32);
-    S->c32 = pick_spmv<float2, float2>(S, cands, st, S->vals32, S->p32, S->q32);
+    S->c32 = pick_spmv<float2, float2>(S, c32, st, S->vals32, S->p32, S->q32);
   }
   PG_CHECK_LAUNCH();
   return PG_OK;
@@ -1410,7 +1522,8 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   S->n = n;
   S->nnz = nnz;
   S->o = *opts;
-  S->o.block_ptr = nullptr;  // the host pointer is not kept
+  S->o.block_ptr = nullptr;  // the host pointers are not kept
+  S->o.entity_order = nullptr;
   if (S->o.maxit <= 0) S->o.maxit = 100000;
   if (S->o.check_every <= 0) S->o.check_every = 25;
   if (!(S->o.inner_rtol > 0.0) || S->o.inner_rtol >= 1.0) S->o.inner_rtol = 1e-4;
@@ -1476,11 +1589,17 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
     }
   }
 #undef PG_A
-  if (hipHostMalloc(reinterpret_cast<void**>(&S->h), sizeof(Scal), 0) != hipSuccess) {
+  if (hipHostMalloc(reinterpret_cast<void**>(&S->h), sizeof(Scal), 0) != hipSuccess ||
+      hipEventCreate(&S->ev[0]) != hipSuccess || hipEventCreate(&S->ev[1]) != hipSuccess) {
     delete S;
-    PG_FAIL(PG_ERR_HIP, "hipHostMalloc failed");
+    PG_FAIL(PG_ERR_HIP, "hipHostMalloc / hipEventCreate failed");
   }
   hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, 0, 0.0);
+  if (S->nblocks > 0 &&
+      (rc = setup_entities(S, opts->block_ptr, opts->entity_order, st)) != PG_OK) {
+    delete S;
+    return rc;
+  }
   if ((rc = refresh_values(S, st)) != PG_OK || (rc = autotune(S, st)) != PG_OK) {
     delete S;
     return rc;
@@ -1501,6 +1620,7 @@ int pg_solver_refresh(void* solver, void* stream) {
 int pg_solver_solve(void* solver, const double* b, double* x, pg_solve_info* info, void* stream) {
   if (!solver || !b || !x || !info) PG_FAIL(PG_ERR_ARG, "bad arguments");
   Solver* S = static_cast<Solver*>(solver);
+  *info = pg_solve_info{};
   if (S->o.ksp == PG_KSP_GMRES)
     return gmres_solve(S, reinterpret_cast<const double2*>(b), reinterpret_cast<double2*>(x),
                        info, as_stream(stream));

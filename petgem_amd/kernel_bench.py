@@ -28,6 +28,12 @@ def main(argv=None):
     ap.add_argument("--pc", default="jacobi", choices=["jacobi", "bjacobi", "none"])
     ap.add_argument("--spmv", default="auto", choices=["auto", "row", "entity"])
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--natural-order", action="store_true",
+                    help="entity-row SpMV in natural entity order (default: Morton order)")
+    ap.add_argument("--batch-width", type=int, default=0,
+                    help="also run the batched solver on the 30^3 p=2 mesh with this many "
+                         "columns for --batch-iters iterations (0 = skip)")
+    ap.add_argument("--batch-iters", type=int, default=50)
     args = ap.parse_args(argv)
 
     import torch
@@ -49,9 +55,10 @@ def main(argv=None):
     core.apply_dirichlet(plan, values, dof_bc)
     out = dict(config=args.config, ndof=plan.ndof, nnz=plan.nnz, reps=args.reps)
     bptr = mesh.block_ptr()
+    order = None if args.natural_order else mesh.entity_order()
     for prec in ("fp64", "mixed"):
         S = core.KrylovSolver(plan, values, precision=prec, pc=args.pc, spmv=args.spmv,
-                              block_ptr=bptr)
+                              block_ptr=bptr, entity_order=order)
         prof = S.profile(reps=args.reps)
         if prec == "fp64":
             out["spmv_c128_ms"], out["spmv_c128_cfg"] = prof["ms64"], prof["g64"]
@@ -82,8 +89,52 @@ def main(argv=None):
     out["copy_bytes_each_way"] = nbytes
     out["copy_ms"] = ms
     out["copy_GBps_rw"] = 2 * nbytes / (ms * 1e-3) / 1e9
+    if args.batch_width:
+        out.update(batch_run(args.batch_width, args.batch_iters))
     print(json.dumps(out), flush=True)
     return 0
+
+
+def batch_run(width, iters):
+    """Fixed number of batched COCG iterations (the iteration limit stops them)."""
+    import numpy as np
+    import torch
+    from . import core
+    from .synthetic import CONFIGS, Workload
+
+    w = Workload(**CONFIGS["c4"], seed=2026)
+    T = w.topo
+    dev = torch.device("cuda")
+    mesh = core.DeviceMesh.from_topology(T, dev)
+    asm = core.Assembler(mesh)
+    ent_bc, dof_bc = mesh.entity_mask(T.boundary_dofs)
+    asm.geometry(torch.from_numpy(w.sigma).to(dev))
+    K, M = core.assemble_km(asm, ent_bc)
+    xs = np.linspace(-3000.0, 3000.0, width)
+    srcs = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
+    el = core.locate_points(mesh, srcs).cpu().numpy()
+    B = torch.zeros((T.ndof, width, 2), dtype=torch.float64, device=dev)
+    for c in range(width):
+        b = core.source_vector(mesh, int(el[c]), srcs[c], (1.0, 0.0, 0.0), core.omega_of(1.0))
+        core.apply_dirichlet(asm.plan, None, dof_bc, b)
+        B[:, c, :] = b
+    out = {}
+    bptr = mesh.block_ptr()
+    for name, bp, od in (("row", None, None), ("entity_natural", bptr, None),
+                         ("entity_morton", bptr, mesh.entity_order())):
+        bs = core.BatchSolver(asm.plan, K, M, [1.0] * width, rtol=1e-30, maxit=iters, block_ptr=bp,
+                              entity_order=od)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        X, info = bs.solve(B)
+        e1.record()
+        torch.cuda.synchronize()
+        out[f"batch_{name}_ms_per_iter"] = e0.elapsed_time(e1) / iters
+        del bs
+    out["batch_width"] = width
+    out["batch_ndof"] = T.ndof
+    return out
 
 
 if __name__ == "__main__":

@@ -109,7 +109,8 @@ class Solver:
         solver = core.KrylovSolver(self.assembler.plan, self.A, ksp=o.ksp_type, pc=o.pc_type,
                                    precision=o.precision, rtol=o.rtol, maxit=o.max_it,
                                    restart=o.restart, inner_rtol=o.inner_rtol,
-                                   block_ptr=self.mesh.block_ptr())
+                                   block_ptr=self.mesh.block_ptr(),
+                                   entity_order=self.mesh.entity_order())
         self.x, self.info = solver.solve(self.b, self.x)
         device_sync()
         Timers()["Solver"].stop()

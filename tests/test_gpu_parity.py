@@ -245,8 +245,10 @@ def _forward_gpu(w, precision="fp64", rtol=1e-8, ksp="cocg", pc="jacobi", spmv="
     el = int(core.locate_points(mesh, np.array(w.source["position"])[None]).cpu()[0])
     b = core.source_vector(mesh, el, w.source["position"], core.dipole_moment(w.source), omega)
     core.apply_dirichlet(asm.plan, vals, dof_bc, b)
+    order = None if spmv == "entity_natural" else mesh.entity_order()
     solver = core.KrylovSolver(asm.plan, vals, ksp=ksp, precision=precision, rtol=rtol, pc=pc,
-                               spmv=spmv, block_ptr=mesh.block_ptr())
+                               spmv="entity" if spmv == "entity_natural" else spmv,
+                               block_ptr=mesh.block_ptr(), entity_order=order)
     x, info = solver.solve(b)
     rx_el = core.locate_points(mesh, w.receivers)
     f = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, x, omega))
@@ -267,7 +269,8 @@ def _oracle_forward(w):
 @pytest.mark.parametrize("precision,pc,spmv", [("fp64", "jacobi", "row"),
                                                ("fp64", "bjacobi", "entity"),
                                                ("mixed", "jacobi", "entity"),
-                                               ("mixed", "bjacobi", "row")])
+                                               ("mixed", "bjacobi", "row"),
+                                               ("mixed", "bjacobi", "entity_natural")])
 def test_forward_solve_vs_superlu(p, n, precision, pc, spmv):
     w = Workload(n, p, noise=0.1, seed=7, n_rx=21)
     g = _forward_gpu(w, precision=precision, pc=pc, spmv=spmv)

@@ -129,7 +129,7 @@ def test_errors_are_reported():
                                   nat.ptr(np.zeros((1, 4), np.int32)), ctypes.byref(h)))
 
 
-def test_config_sizes_match_baseline_table():
+def test_config_sizes_match_workload_table():
     """Exact sizes of the benchmark workloads (N, nnz, boundary DOFs)."""
     for n, p, ndof, nnz, nb in [(20, 1, 59660, 938060, 7200), (40, 2, 2480240, 106797280, 96000),
                                 (30, 2, 1053180, 45133560, 54000)]:
@@ -146,3 +146,19 @@ def test_config_sizes_match_baseline_table():
         L.pg_plan_info(h, nat.ptr(inf))
         L.pg_plan_destroy(h)
         assert (T.ndof, int(inf[1]), T.boundary_dofs.size) == (ndof, nnz, nb)
+
+
+def test_entity_order_is_morton_permutation():
+    """DeviceMesh.entity_order: a permutation of the block_ptr entities whose
+    positions follow the Morton curve (spatially local)."""
+    from petgem_amd.core import DeviceMesh, _morton3
+    for p in (1, 2, 3):
+        w = Workload(4, p)
+        m = DeviceMesh.from_topology(w.topo, "cpu")
+        order = m.entity_order()
+        nb = m.block_ptr().size - 1
+        assert order.dtype == np.int32 and order.size == nb
+        assert np.array_equal(np.sort(order), np.arange(nb))
+    # the Morton key interleaves x, y, z bits (x lowest)
+    assert list(_morton3(np.array([1, 0, 0, 3]), np.array([0, 1, 0, 0]),
+                         np.array([0, 0, 1, 0]))) == [1, 2, 4, 9]
