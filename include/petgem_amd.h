@@ -282,6 +282,27 @@ int pg_batch_set_entities(void *batch, const int32_t *block_ptr, int64_t nblocks
 int pg_batch_refresh(void *batch, void *stream);
 int pg_batch_destroy(void *batch);
 
+/* ------------------------------------------------------------------------
+ * Multi-mesh batch: independent meshes of uneven size (each its own assembled
+ * complex128 CSR system, e.g. from pg_assemble + pg_apply_dirichlet, and one
+ * right-hand side) solved together, one COCG + Jacobi iteration of every mesh
+ * per set of launches, each mesh converging on its own.  An extension with no
+ * counterpart in the reference (one KSP per problem, petgem/solver.py:584-590).
+ * The batch keeps a concatenated copy of the systems; with use_graph != 0 the
+ * check_every iterations between host convergence checks are captured once
+ * into a HIP graph and replayed.  Arrays of device pointers are host arrays.
+ * ---------------------------------------------------------------------- */
+int pg_multi_create(int nmesh, const int64_t *n, const int64_t *nnz,
+                    const int32_t *const *rowptr, const int32_t *const *cols,
+                    const double *const *values, double rtol, int maxit, int check_every,
+                    int use_graph, void **batch, void *stream);
+/* Zero initial guesses; per-mesh iterations, true relres and status (0 ok, -1
+ * not converged, PG_ERR_BREAKDOWN) into host arrays (nullable); *launches =
+ * graph launches (or iteration batches without a graph). */
+int pg_multi_solve(void *batch, const double *const *b, double *const *x, int32_t *iterations,
+                   double *relres, int32_t *status, int32_t *launches, void *stream);
+int pg_multi_destroy(void *batch);
+
 #ifdef __cplusplus
 }
 #endif

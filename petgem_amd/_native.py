@@ -83,6 +83,10 @@ SIGNATURES = {
     "pg_batch_set_entities": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
     "pg_batch_destroy": (ctypes.c_int, [_P]),
     "pg_solver_profile": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P]),
+    "pg_multi_create": (ctypes.c_int, [ctypes.c_int, _P, _P, _P, _P, _P, _D, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P), _P]),
+    "pg_multi_solve": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "pg_multi_destroy": (ctypes.c_int, [_P]),
 }
 
 

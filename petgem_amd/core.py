@@ -302,6 +302,56 @@ class BatchSolver:
             pass
 
 
+class MultiMeshSolver:
+    """Independent meshes of uneven size solved together (one COCG + Jacobi per
+    mesh, all advancing in the same launches; the iterations between host
+    checks replayed from a captured HIP graph)."""
+
+    def __init__(self, systems, rtol=1e-8, maxit=100000, check_every=25, use_graph=True,
+                 stream=None):
+        """systems: list of (plan, values) with values (nnz, 2) float64 on the device."""
+        k = len(systems)
+        self.n = np.array([pl.ndof for pl, _ in systems], dtype=np.int64)
+        nnz = np.array([pl.nnz for pl, _ in systems], dtype=np.int64)
+        arr = lambda ts: (ctypes.c_void_p * k)(*[nat.ptr(t) for t in ts])  # noqa: E731
+        rp = arr([pl.rowptr for pl, _ in systems])
+        ci = arr([pl.cols for pl, _ in systems])
+        va = arr([v for _, v in systems])
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pg_multi_create(k, self.n.ctypes.data, nnz.ctypes.data, rp, ci, va,
+                                            float(rtol), int(maxit), int(check_every),
+                                            1 if use_graph else 0, ctypes.byref(h),
+                                            nat.stream_handle(stream)))
+        self.handle = h
+        self.k = k
+
+    def solve(self, bs, stream=None):
+        """bs: list of (ndof_i, 2) float64 right-hand sides; returns xs, info."""
+        if len(bs) != self.k:
+            raise ValueError(f"need {self.k} right-hand sides")
+        for b, n in zip(bs, self.n):
+            if tuple(b.shape) != (int(n), 2) or b.dtype != torch.float64 or not b.is_contiguous():
+                raise ValueError("each b must be a contiguous (ndof, 2) float64 tensor")
+        xs = [torch.empty_like(b) for b in bs]
+        arr = lambda ts: (ctypes.c_void_p * self.k)(*[nat.ptr(t) for t in ts])  # noqa: E731
+        it = np.zeros(self.k, dtype=np.int32)
+        rel = np.zeros(self.k, dtype=np.float64)
+        st = np.zeros(self.k, dtype=np.int32)
+        launches = ctypes.c_int32()
+        nat.check(nat.lib().pg_multi_solve(self.handle, arr(bs), arr(xs), nat.ptr(it),
+                                           nat.ptr(rel), nat.ptr(st), ctypes.byref(launches),
+                                           nat.stream_handle(stream)))
+        return xs, dict(iterations=it, relres=rel, status=st, launches=launches.value)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                nat.lib().pg_multi_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 def apply_dirichlet(plan, values, dof_bc, b=None, stream=None):
     nat.check(nat.lib().pg_apply_dirichlet(plan.ndof, nat.ptr(plan.rowptr), nat.ptr(plan.cols),
                                            nat.ptr(dof_bc), nat.ptr(values), nat.ptr(b),

@@ -1,0 +1,650 @@
+// Multi-mesh batched solve for gfx950: many independent meshes of uneven size
+// (each with its own assembled complex system and right-hand side) advance
+// together, one COCG iteration of every mesh per set of kernel launches.
+//
+// No counterpart in the reference (one KSP per problem, petgem/solver.py:584-590).
+// The systems are concatenated into one block-diagonal CSR.  Workgroups are
+// dealt to the meshes in proportion to their rows (at least one each), so
+// every workgroup belongs to exactly one mesh: its partial sums go to that
+// mesh's slots, and a one-workgroup-per-mesh finish kernel reduces them in a
+// fixed order (bitwise reproducible, no device-scope fences).  Each mesh runs
+// its own COCG + Jacobi recurrence (its own alpha, beta, rho, convergence) and
+// drops out of the work as soon as it has converged.
+//
+// Persistence: the `check` iterations between host convergence checks (six
+// kernels each) are captured once into a HIP graph at creation and replayed
+// on the batch's own stream until every mesh has converged, so a batch of
+// small meshes costs one graph launch per `check` iterations instead of
+// 6 * check kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pg_common.h"
+
+namespace pg {
+namespace multi {
+
+constexpr int kBS = 256;
+constexpr int kRW = 256;  // rows per workgroup
+constexpr int kG = 16;    // lanes per row in the SpMV
+
+struct Seg {  // per-mesh scalars
+  double2 rho, mu, alpha, beta;
+  double res2, bnorm2;
+  int32_t iter, done, status, pad;
+};
+
+struct Ctl {
+  double tol2;
+  int32_t maxit, nseg, all_done, pad;
+};
+
+// the rows of workgroup w: segment wseg[w], rows [lo, hi)
+struct Span {
+  int s;
+  int64_t lo, hi;
+};
+__device__ inline Span span_of(const int32_t* __restrict__ wseg, const int32_t* __restrict__ wg0,
+                               const int64_t* __restrict__ row0) {
+  const int w = blockIdx.x;
+  const int s = wseg[w];
+  const int64_t lo = row0[s] + (int64_t)(w - wg0[s]) * kRW;
+  const int64_t hi = lo + kRW < row0[s + 1] ? lo + kRW : row0[s + 1];
+  return {s, lo, hi};
+}
+
+// this workgroup's NV partial sums -> part[k * W + w]
+template <int NV>
+__device__ inline void write_partials(double (&v)[NV], double* part) {
+  __shared__ double sm[NV][kBS / kWave];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sm[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < kBS / kWave; ++i) s += sm[k][i];
+      part[(int64_t)k * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// in a finish kernel (one workgroup per segment s = blockIdx.x): fixed-order
+// totals of the partials of workgroups [wg0[s], wg0[s+1]) out of W
+template <int NV>
+__device__ inline void seg_totals(const double* part, int W, int w0, int w1, double (&tot)[NV]) {
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int i = w0 + threadIdx.x; i < w1; i += kBS)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += part[(int64_t)k * W + i];
+  __shared__ double sm[NV][kBS / kWave];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sm[k][w] = acc[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int i = 0; i < kBS / kWave; ++i) s += sm[k][i];
+    tot[k] = s;
+  }
+}
+
+__device__ inline double2 cfma(double2 a, double2 b, double2 c) {
+  return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
+}
+
+// y = A x over this workgroup's rows (kG lanes per row); MODE 0: y = A x and
+// partials of x^T y; MODE 1: y = b - A x and partials of |y|^2
+template <int MODE>
+__device__ inline void spmv_rows(const Span& sp, const int32_t* __restrict__ rowptr,
+                                 const int32_t* __restrict__ cols,
+                                 const double2* __restrict__ vals, const double2* __restrict__ x,
+                                 double2* __restrict__ y, const double2* __restrict__ b,
+                                 double (&red)[2]) {
+  const int lane = threadIdx.x % kG;
+  for (int64_t row = sp.lo + threadIdx.x / kG; row < sp.hi; row += kBS / kG) {
+    double2 acc = make_double2(0.0, 0.0);
+    const int s0 = rowptr[row], s1 = rowptr[row + 1];
+    for (int kb = s0 + lane; kb < s1; kb += 2 * kG) {
+      const int k1 = kb + kG;
+      const int c0 = cols[kb], c1 = k1 < s1 ? cols[k1] : -1;
+      const double2 a0 = vals[kb], a1 = k1 < s1 ? vals[k1] : make_double2(0.0, 0.0);
+      const double2 x0 = x[c0], x1 = c1 >= 0 ? x[c1] : make_double2(0.0, 0.0);
+      acc = cfma(a0, x0, acc);
+      acc = cfma(a1, x1, acc);
+    }
+    acc.x = group_sum<kG>(acc.x);
+    acc.y = group_sum<kG>(acc.y);
+    if (lane == 0) {
+      if (MODE == 1) {
+        const double2 r = make_double2(b[row].x - acc.x, b[row].y - acc.y);
+        y[row] = r;
+        red[0] += r.x * r.x + r.y * r.y;
+      } else {
+        y[row] = acc;
+        const double2 p = x[row];
+        red[0] += p.x * acc.x - p.y * acc.y;
+        red[1] += p.x * acc.y + p.y * acc.x;
+      }
+    }
+  }
+}
+
+// p = z + beta_s p
+__global__ void __launch_bounds__(kBS) k_mpupdate(const Ctl* ctl, const Seg* seg,
+                                                  const int32_t* __restrict__ wseg,
+                                                  const int32_t* __restrict__ wg0,
+                                                  const int64_t* __restrict__ row0,
+                                                  const double2* __restrict__ z,
+                                                  double2* __restrict__ p) {
+  if (ctl->all_done) return;
+  const Span sp = span_of(wseg, wg0, row0);
+  if (seg[sp.s].done) return;
+  const double2 be = seg[sp.s].beta;
+  for (int64_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBS) p[i] = cfma(be, p[i], z[i]);
+}
+
+// q = A p, partials of p^T q
+__global__ void __launch_bounds__(kBS) k_mspmv(const Ctl* ctl, const Seg* seg,
+                                               const int32_t* __restrict__ wseg,
+                                               const int32_t* __restrict__ wg0,
+                                               const int64_t* __restrict__ row0,
+                                               const int32_t* __restrict__ rowptr,
+                                               const int32_t* __restrict__ cols,
+                                               const double2* __restrict__ vals,
+                                               const double2* __restrict__ p,
+                                               double2* __restrict__ q, double* part) {
+  if (ctl->all_done) return;
+  const Span sp = span_of(wseg, wg0, row0);
+  if (seg[sp.s].done) return;
+  double red[2] = {0.0, 0.0};
+  spmv_rows<0>(sp, rowptr, cols, vals, p, q, nullptr, red);
+  write_partials<2>(red, part);
+}
+
+__global__ void __launch_bounds__(kBS) k_mfin_mu(const Ctl* ctl, Seg* seg,
+                                                 const int32_t* __restrict__ wg0,
+                                                 const double* part, int W) {
+  if (ctl->all_done) return;
+  const int s = blockIdx.x;
+  if (seg[s].done) return;
+  double tot[2];
+  seg_totals<2>(part, W, wg0[s], wg0[s + 1], tot);
+  if (threadIdx.x != 0) return;
+  Seg& c = seg[s];
+  const double2 mu = make_double2(tot[0], tot[1]);
+  c.mu = mu;
+  const double2 al = zdiv(c.rho, mu);
+  if (!(mu.x != 0.0 || mu.y != 0.0) || !isfinite(al.x) || !isfinite(al.y)) {
+    c.status = PG_ERR_BREAKDOWN;
+    c.done = 1;
+    c.alpha = make_double2(0.0, 0.0);
+  } else {
+    c.alpha = al;
+  }
+}
+
+// x += alpha p; r -= alpha q; z = dinv r; partials of r^T z, |r|^2
+__global__ void __launch_bounds__(kBS) k_mupdate(const Ctl* ctl, const Seg* seg,
+                                                 const int32_t* __restrict__ wseg,
+                                                 const int32_t* __restrict__ wg0,
+                                                 const int64_t* __restrict__ row0,
+                                                 double2* __restrict__ x, double2* __restrict__ r,
+                                                 const double2* __restrict__ p,
+                                                 const double2* __restrict__ q,
+                                                 const double2* __restrict__ dinv,
+                                                 double2* __restrict__ z, double* part) {
+  if (ctl->all_done) return;
+  const Span sp = span_of(wseg, wg0, row0);
+  if (seg[sp.s].done) return;
+  const double2 a = seg[sp.s].alpha;
+  const double2 na = make_double2(-a.x, -a.y);
+  double red[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBS) {
+    x[i] = cfma(a, p[i], x[i]);
+    const double2 ri = cfma(na, q[i], r[i]);
+    r[i] = ri;
+    const double2 zi = zmul(dinv[i], ri);
+    z[i] = zi;
+    red[0] += ri.x * zi.x - ri.y * zi.y;
+    red[1] += ri.x * zi.y + ri.y * zi.x;
+    red[2] += ri.x * ri.x + ri.y * ri.y;
+  }
+  write_partials<3>(red, part);
+}
+
+__global__ void __launch_bounds__(kBS) k_mfin_update(const Ctl* ctl, Seg* seg,
+                                                     const int32_t* __restrict__ wg0,
+                                                     const double* part, int W) {
+  if (ctl->all_done) return;
+  const int s = blockIdx.x;
+  if (seg[s].done) return;
+  double tot[3];
+  seg_totals<3>(part, W, wg0[s], wg0[s + 1], tot);
+  if (threadIdx.x != 0) return;
+  Seg& c = seg[s];
+  const double2 rho = make_double2(tot[0], tot[1]);
+  const double2 beta = zdiv(rho, c.rho);
+  c.beta = beta;
+  c.rho = rho;
+  c.res2 = tot[2];
+  c.iter += 1;
+  if (!isfinite(tot[2]) || !isfinite(beta.x) || !isfinite(beta.y)) {
+    c.status = PG_ERR_BREAKDOWN;
+    c.done = 1;
+  } else if (tot[2] <= ctl->tol2 * c.bnorm2 || c.iter >= ctl->maxit) {
+    c.done = 1;
+  }
+}
+
+// all_done = every mesh done (one workgroup)
+__global__ void __launch_bounds__(kBS) k_mfin_all(Ctl* ctl, const Seg* seg, int nseg) {
+  if (ctl->all_done) return;
+  __shared__ int notdone;
+  if (threadIdx.x == 0) notdone = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int s = threadIdx.x; s < nseg; s += kBS) mine |= seg[s].done ? 0 : 1;
+  if (mine) notdone = 1;  // every writer stores the same value
+  __syncthreads();
+  if (threadIdx.x == 0) ctl->all_done = notdone ? 0 : 1;
+}
+
+// (re)start from the current x: r = b - A x, z = dinv r, p = 0; partials of r^T z, |r|^2
+__global__ void __launch_bounds__(kBS) k_mrestart(const int32_t* __restrict__ wseg,
+                                                  const int32_t* __restrict__ wg0,
+                                                  const int64_t* __restrict__ row0,
+                                                  const int32_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ cols,
+                                                  const double2* __restrict__ vals,
+                                                  const double2* __restrict__ x,
+                                                  const double2* __restrict__ b,
+                                                  double2* __restrict__ r,
+                                                  const double2* __restrict__ dinv,
+                                                  double2* __restrict__ z, double2* __restrict__ p,
+                                                  double* part) {
+  const Span sp = span_of(wseg, wg0, row0);
+  double red2[2] = {0.0, 0.0};
+  spmv_rows<1>(sp, rowptr, cols, vals, x, r, b, red2);
+  __syncthreads();  // r of this workgroup's rows is complete
+  double red[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBS) {
+    const double2 ri = r[i];
+    const double2 zi = zmul(dinv[i], ri);
+    z[i] = zi;
+    p[i] = make_double2(0.0, 0.0);
+    red[0] += ri.x * zi.x - ri.y * zi.y;
+    red[1] += ri.x * zi.y + ri.y * zi.x;
+    red[2] += ri.x * ri.x + ri.y * ri.y;
+  }
+  write_partials<3>(red, part);
+}
+
+// finish a (re)start: rho, |r|^2 (and |b|^2 the first time); meshes that meet
+// the tolerance, stopped on breakdown or used up their iterations stay done
+__global__ void __launch_bounds__(kBS) k_mfin_restart(Ctl* ctl, Seg* seg,
+                                                      const int32_t* __restrict__ wg0,
+                                                      const double* part, int W, int first) {
+  const int s = blockIdx.x;
+  double tot[3];
+  seg_totals<3>(part, W, wg0[s], wg0[s + 1], tot);
+  if (threadIdx.x != 0) return;
+  Seg& c = seg[s];
+  if (first) {
+    c = Seg{};
+    c.bnorm2 = tot[2];  // x = 0 on the first start
+  }
+  c.rho = make_double2(tot[0], tot[1]);
+  c.res2 = tot[2];
+  c.beta = make_double2(0.0, 0.0);
+  c.alpha = make_double2(0.0, 0.0);
+  c.done = (c.status != 0 || c.iter >= ctl->maxit || tot[2] <= ctl->tol2 * c.bnorm2) ? 1 : 0;
+  if (s == 0) ctl->all_done = 0;
+}
+
+// |b - A x|^2 per mesh (after the solve)
+__global__ void __launch_bounds__(kBS) k_mtrue(const int32_t* __restrict__ wseg,
+                                               const int32_t* __restrict__ wg0,
+                                               const int64_t* __restrict__ row0,
+                                               const int32_t* __restrict__ rowptr,
+                                               const int32_t* __restrict__ cols,
+                                               const double2* __restrict__ vals,
+                                               const double2* __restrict__ x,
+                                               const double2* __restrict__ b,
+                                               double2* __restrict__ r, double* part) {
+  const Span sp = span_of(wseg, wg0, row0);
+  double red[2] = {0.0, 0.0};
+  spmv_rows<1>(sp, rowptr, cols, vals, x, r, b, red);
+  double v[1] = {red[0]};
+  write_partials<1>(v, part);
+}
+__global__ void __launch_bounds__(kBS) k_mfin_true(Seg* seg, const int32_t* __restrict__ wg0,
+                                                   const double* part, int W) {
+  const int s = blockIdx.x;
+  double tot[1];
+  seg_totals<1>(part, W, wg0[s], wg0[s + 1], tot);
+  if (threadIdx.x == 0) seg[s].res2 = tot[0];
+}
+
+__global__ void k_mctl_init(Ctl* ctl, int maxit, double tol2, int nseg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Ctl c{};
+  c.tol2 = tol2;
+  c.maxit = maxit;
+  c.nseg = nseg;
+  *ctl = c;
+}
+
+// concatenation: rowptr of segment s shifted by its first non-zero, columns by its first row
+__global__ void k_shift_i32(int64_t n, const int32_t* __restrict__ src, int32_t add,
+                            int32_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i] + add;
+}
+
+// Jacobi: dinv = 1 / A_ii (the diagonal of the concatenated, column-shifted CSR)
+__global__ void k_mjacobi(int64_t n, const int32_t* __restrict__ rowptr,
+                          const int32_t* __restrict__ cols, const double2* __restrict__ vals,
+                          double2* __restrict__ dinv, int32_t* bad) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int lo = rowptr[r], hi = rowptr[r + 1] - 1;
+  double2 d = make_double2(0.0, 0.0);
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = cols[mid];
+    if (c == r) {
+      d = vals[mid];
+      break;
+    }
+    if (c < r) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  if (d.x == 0.0 && d.y == 0.0) {
+    bad[0] = 1;
+    dinv[r] = make_double2(1.0, 0.0);
+    return;
+  }
+  dinv[r] = zdiv(make_double2(1.0, 0.0), d);
+}
+
+struct Multi {
+  int nseg = 0, W = 0, check = 25, maxit = 100000;
+  int64_t n = 0, nnz = 0;
+  double rtol = 1e-8;
+  std::vector<int64_t> hrow0;
+  int32_t *rowptr = nullptr, *cols = nullptr, *wseg = nullptr, *wg0 = nullptr, *bad = nullptr;
+  int64_t* row0 = nullptr;
+  double2 *vals = nullptr, *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *q = nullptr,
+          *z = nullptr, *dinv = nullptr;
+  double* part = nullptr;
+  Ctl* ctl = nullptr;
+  Seg* seg = nullptr;
+  Ctl* hctl = nullptr;
+  Seg* hseg = nullptr;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void*> owned;
+  ~Multi() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (st) hipStreamDestroy(st);
+    for (void* v : owned) hipFree(v);
+    if (hctl) hipHostFree(hctl);
+    if (hseg) hipHostFree(hseg);
+  }
+  template <typename T>
+  int alloc(T** ptr, size_t count) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess)
+      PG_FAIL(PG_ERR_HIP, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+    owned.push_back(v);
+    *ptr = static_cast<T*>(v);
+    return PG_OK;
+  }
+};
+
+// one COCG iteration of every mesh (six launches on the batch's stream)
+static void enqueue_iteration(Multi* B) {
+  const dim3 g(B->W), b(kBS), gs(B->nseg);
+  hipLaunchKernelGGL(k_mpupdate, g, b, 0, B->st, B->ctl, B->seg, B->wseg, B->wg0, B->row0, B->z,
+                     B->p);
+  hipLaunchKernelGGL(k_mspmv, g, b, 0, B->st, B->ctl, B->seg, B->wseg, B->wg0, B->row0, B->rowptr,
+                     B->cols, B->vals, B->p, B->q, B->part);
+  hipLaunchKernelGGL(k_mfin_mu, gs, b, 0, B->st, B->ctl, B->seg, B->wg0, B->part, B->W);
+  hipLaunchKernelGGL(k_mupdate, g, b, 0, B->st, B->ctl, B->seg, B->wseg, B->wg0, B->row0, B->x,
+                     B->r, B->p, B->q, B->dinv, B->z, B->part);
+  hipLaunchKernelGGL(k_mfin_update, gs, b, 0, B->st, B->ctl, B->seg, B->wg0, B->part, B->W);
+  hipLaunchKernelGGL(k_mfin_all, dim3(1), b, 0, B->st, B->ctl, B->seg, B->nseg);
+}
+
+static void enqueue_restart(Multi* B, int first) {
+  hipLaunchKernelGGL(k_mrestart, dim3(B->W), dim3(kBS), 0, B->st, B->wseg, B->wg0, B->row0,
+                     B->rowptr, B->cols, B->vals, B->x, B->b, B->r, B->dinv, B->z, B->p, B->part);
+  hipLaunchKernelGGL(k_mfin_restart, dim3(B->nseg), dim3(kBS), 0, B->st, B->ctl, B->seg, B->wg0,
+                     B->part, B->W, first);
+  hipLaunchKernelGGL(k_mfin_all, dim3(1), dim3(kBS), 0, B->st, B->ctl, B->seg, B->nseg);
+}
+
+}  // namespace multi
+}  // namespace pg
+
+using namespace pg;
+using namespace pg::multi;
+
+extern "C" {
+
+int pg_multi_create(int nseg, const int64_t* n, const int64_t* nnz, const int32_t* const* rowptr,
+                    const int32_t* const* cols, const double* const* values, double rtol,
+                    int maxit, int check_every, int use_graph, void** out, void* stream) {
+  if (!out || nseg <= 0 || !n || !nnz || !rowptr || !cols || !values)
+    PG_FAIL(PG_ERR_ARG, "bad arguments");
+  int64_t N = 0, Z = 0;
+  std::vector<int64_t> row0(nseg + 1), nz0(nseg + 1);
+  std::vector<int32_t> wg0(nseg + 1);
+  int64_t W = 0;
+  for (int s = 0; s < nseg; ++s) {
+    if (n[s] <= 0 || nnz[s] <= 0 || !rowptr[s] || !cols[s] || !values[s])
+      PG_FAIL(PG_ERR_ARG, "mesh %d: bad arguments", s);
+    row0[s] = N;
+    nz0[s] = Z;
+    wg0[s] = (int32_t)W;
+    N += n[s];
+    Z += nnz[s];
+    W += (n[s] + kRW - 1) / kRW;
+  }
+  row0[nseg] = N;
+  nz0[nseg] = Z;
+  wg0[nseg] = (int32_t)W;
+  if (N >= (1LL << 31) || Z >= (1LL << 31) || W >= (1LL << 31))
+    PG_FAIL(PG_ERR_ARG, "the concatenated system is too large for 32-bit indices");
+  std::vector<int32_t> wseg(W);
+  for (int s = 0; s < nseg; ++s)
+    for (int32_t w = wg0[s]; w < wg0[s + 1]; ++w) wseg[w] = s;
+  hipStream_t cst = as_stream(stream);
+  Multi* B = new Multi();
+  B->nseg = nseg;
+  B->n = N;
+  B->nnz = Z;
+  B->W = (int)W;
+  B->rtol = rtol > 0 ? rtol : 1e-8;
+  B->maxit = maxit > 0 ? maxit : 100000;
+  B->check = check_every > 0 ? check_every : 25;
+  B->hrow0 = row0;
+  int rc;
+#define PG_MA(ptr, cnt)                          \
+  if ((rc = B->alloc(&(ptr), (cnt))) != PG_OK) { \
+    delete B;                                    \
+    return rc;                                   \
+  }
+  PG_MA(B->rowptr, N + 1);
+  PG_MA(B->cols, Z);
+  PG_MA(B->vals, Z);
+  PG_MA(B->row0, nseg + 1);
+  PG_MA(B->wg0, nseg + 1);
+  PG_MA(B->wseg, W);
+  PG_MA(B->x, N);
+  PG_MA(B->b, N);
+  PG_MA(B->r, N);
+  PG_MA(B->p, N);
+  PG_MA(B->q, N);
+  PG_MA(B->z, N);
+  PG_MA(B->dinv, N);
+  PG_MA(B->part, 3 * W);
+  PG_MA(B->ctl, 1);
+  PG_MA(B->seg, nseg);
+  PG_MA(B->bad, 1);
+#undef PG_MA
+  if (hipHostMalloc(reinterpret_cast<void**>(&B->hctl), sizeof(Ctl), 0) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&B->hseg), nseg * sizeof(Seg), 0) != hipSuccess ||
+      hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking) != hipSuccess) {
+    delete B;
+    PG_FAIL(PG_ERR_HIP, "host buffers / stream creation failed");
+  }
+  // the caller's arrays are complete once its stream has drained
+  if (hipStreamSynchronize(cst) != hipSuccess) {
+    delete B;
+    PG_FAIL(PG_ERR_HIP, "stream synchronisation failed");
+  }
+  hipStream_t st = B->st;
+  for (int s = 0; s < nseg; ++s) {
+    const unsigned gr = (unsigned)((n[s] + 1 + kBS - 1) / kBS), gc = (unsigned)((nnz[s] + kBS - 1) / kBS);
+    hipLaunchKernelGGL(k_shift_i32, dim3(gr), dim3(kBS), 0, st, n[s] + 1, rowptr[s],
+                       (int32_t)nz0[s], B->rowptr + row0[s]);
+    hipLaunchKernelGGL(k_shift_i32, dim3(gc), dim3(kBS), 0, st, nnz[s], cols[s], (int32_t)row0[s],
+                       B->cols + nz0[s]);
+    if (hipMemcpyAsync(B->vals + nz0[s], values[s], nnz[s] * sizeof(double2),
+                       hipMemcpyDeviceToDevice, st) != hipSuccess) {
+      delete B;
+      PG_FAIL(PG_ERR_HIP, "value copy failed");
+    }
+  }
+  // row pointers of consecutive meshes overlap in one entry (equal values): the
+  // later mesh's first entry is written last, in stream order
+  if (hipMemcpyAsync(B->row0, row0.data(), (nseg + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                     st) != hipSuccess ||
+      hipMemcpyAsync(B->wg0, wg0.data(), (nseg + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
+                     st) != hipSuccess ||
+      hipMemcpyAsync(B->wseg, wseg.data(), W * sizeof(int32_t), hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipMemsetAsync(B->bad, 0, sizeof(int32_t), st) != hipSuccess) {
+    delete B;
+    PG_FAIL(PG_ERR_HIP, "upload failed");
+  }
+  hipLaunchKernelGGL(k_mjacobi, dim3((unsigned)((N + kBS - 1) / kBS)), dim3(kBS), 0, st, N,
+                     B->rowptr, B->cols, B->vals, B->dinv, B->bad);
+  hipLaunchKernelGGL(k_mctl_init, dim3(1), dim3(64), 0, st, B->ctl, B->maxit,
+                     B->rtol * B->rtol, nseg);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(&B->hctl->maxit, B->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    delete B;
+    PG_FAIL(PG_ERR_HIP, "multi-mesh setup failed");
+  }
+  if (B->hctl->maxit != 0) {
+    delete B;
+    PG_FAIL(PG_ERR_ARG, "zero diagonal entry: Jacobi preconditioner undefined");
+  }
+  if (use_graph) {
+    // capture `check` iterations once; replayed by every solve
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      delete B;
+      PG_FAIL(PG_ERR_HIP, "graph capture failed to start");
+    }
+    for (int k = 0; k < B->check; ++k) enqueue_iteration(B);
+    if (hipStreamEndCapture(st, &B->graph) != hipSuccess ||
+        hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess) {
+      delete B;
+      PG_FAIL(PG_ERR_HIP, "graph capture failed");
+    }
+  }
+  *out = B;
+  return PG_OK;
+}
+
+int pg_multi_solve(void* batch, const double* const* b, double* const* x, int32_t* iterations,
+                   double* relres, int32_t* status, int32_t* launches, void* stream) {
+  if (!batch || !b || !x) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  Multi* B = static_cast<Multi*>(batch);
+  hipStream_t cst = as_stream(stream);
+  PG_HIP(hipStreamSynchronize(cst));  // the caller's right-hand sides are ready
+  hipStream_t st = B->st;
+  for (int s = 0; s < B->nseg; ++s) {
+    if (!b[s] || !x[s]) PG_FAIL(PG_ERR_ARG, "mesh %d: NULL b or x", s);
+    const int64_t ns = B->hrow0[s + 1] - B->hrow0[s];
+    PG_HIP(hipMemcpyAsync(B->b + B->hrow0[s], b[s], ns * sizeof(double2), hipMemcpyDeviceToDevice,
+                          st));
+  }
+  PG_HIP(hipMemsetAsync(B->x, 0, B->n * sizeof(double2), st));
+  hipLaunchKernelGGL(k_mctl_init, dim3(1), dim3(64), 0, st, B->ctl, B->maxit, B->rtol * B->rtol,
+                     B->nseg);
+  enqueue_restart(B, 1);
+  PG_CHECK_LAUNCH();
+  int nlaunch = 0;
+  for (int restart = 0;; ++restart) {
+    while (true) {
+      if (B->exec) {
+        PG_HIP(hipGraphLaunch(B->exec, st));
+      } else {
+        for (int k = 0; k < B->check; ++k) enqueue_iteration(B);
+        PG_CHECK_LAUNCH();
+      }
+      ++nlaunch;
+      PG_HIP(hipMemcpyAsync(B->hctl, B->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+      PG_HIP(hipStreamSynchronize(st));
+      if (B->hctl->all_done) break;
+    }
+    // true residuals; meshes above the tolerance restart from their iterate
+    hipLaunchKernelGGL(k_mtrue, dim3(B->W), dim3(kBS), 0, st, B->wseg, B->wg0, B->row0, B->rowptr,
+                       B->cols, B->vals, B->x, B->b, B->r, B->part);
+    hipLaunchKernelGGL(k_mfin_true, dim3(B->nseg), dim3(kBS), 0, st, B->seg, B->wg0, B->part,
+                       B->W);
+    PG_CHECK_LAUNCH();
+    PG_HIP(hipMemcpyAsync(B->hseg, B->seg, B->nseg * sizeof(Seg), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    bool again = false;
+    for (int s = 0; s < B->nseg; ++s) {
+      const Seg& c = B->hseg[s];
+      if (c.status == 0 && c.iter < B->maxit && c.res2 > B->rtol * B->rtol * c.bnorm2) again = true;
+    }
+    if (!again || restart >= 20) break;
+    enqueue_restart(B, 0);
+    PG_CHECK_LAUNCH();
+  }
+  for (int s = 0; s < B->nseg; ++s) {
+    const Seg& c = B->hseg[s];
+    const int64_t ns = B->hrow0[s + 1] - B->hrow0[s];
+    PG_HIP(hipMemcpyAsync(x[s], B->x + B->hrow0[s], ns * sizeof(double2), hipMemcpyDeviceToDevice,
+                          st));
+    const double rel = c.bnorm2 > 0 ? std::sqrt(c.res2 / c.bnorm2) : 0.0;
+    if (iterations) iterations[s] = c.iter;
+    if (relres) relres[s] = rel;
+    if (status) status[s] = c.status != 0 ? c.status : (rel <= B->rtol * 1.0000001 ? 0 : -1);
+  }
+  PG_HIP(hipStreamSynchronize(st));
+  if (launches) *launches = nlaunch;
+  return PG_OK;
+}
+
+int pg_multi_destroy(void* batch) {
+  delete static_cast<Multi*>(batch);
+  return PG_OK;
+}
+
+}  // extern "C"
