@@ -123,7 +123,8 @@ def batched_leg(args, log):
     core.assemble_km(asm, ent_bc, out=(K, M))
     bptr, order = mesh.block_ptr(), mesh.entity_order()
     solvers = [core.BatchSolver(asm.plan, K, M, [f for _, f in ch], rtol=args.rtol,
-                                block_ptr=bptr, entity_order=order) for ch in chunks]
+                                block_ptr=bptr, entity_order=order,
+                                precision=args.batch_precision) for ch in chunks]
     B = torch.zeros((T.ndof, width, 2), dtype=torch.float64, device=dev)
     times, iters = [], []
     for it in range(1):
@@ -151,12 +152,34 @@ def batched_leg(args, log):
         times.append(dt)
         iters = [int(i.max()) for i in its]
     t = float(np.median(times))
+    # the same right-hand sides one after another through the single-RHS solver
+    # (mixed precision, entity-block Jacobi), one solver per frequency
+    seq_t = 0.0
+    vals = torch.empty((asm.plan.nnz, 2), dtype=torch.float64, device=dev)
+    for f in freqs:
+        asm.assemble(core.omega_of(f), ent_bc=ent_bc, out=vals)
+        S = core.KrylovSolver(asm.plan, vals, precision="mixed", pc="bjacobi", rtol=args.rtol,
+                              block_ptr=bptr, entity_order=order)
+        for si in range(len(xs)):
+            b = core.source_vector(mesh, int(src_el[si]), srcs[si], (1.0, 0.0, 0.0),
+                                   core.omega_of(f))
+            core.apply_dirichlet(asm.plan, None, dof_bc, b)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, inf = S.solve(b)
+            torch.cuda.synchronize()
+            seq_t += time.perf_counter() - t0
+            if inf["status"] != 0:
+                raise RuntimeError(f"single solve did not converge: {inf}")
+        del S
+    log(f"batched: {t:.1f} s per batch; one after another (mixed, block Jacobi) {seq_t:.1f} s")
     return {"rhs_per_s": round(len(cols) / t, 3), "s_per_batch": round(t, 4), "nrhs": len(cols),
             "width": width, "chunk_iterations": iters,
+            "sequential_mixed_bjacobi_rhs_per_s": round(len(cols) / seq_t, 3),
             "workload": (f"c4: p=2, 30^3 Kuhn x 6 = {T.n_elems} tets, {T.ndof} DOFs, "
                          f"{asm.plan.nnz} nnz; {len(xs)} of the 64 HED x-positions (100 m apart "
                          f"at z=-950 m) x {freqs} Hz, rtol {args.rtol}, COCG + Jacobi per "
-                         "column, K/M assembled once per batch")}
+                         f"column ({args.batch_precision}), K/M assembled once per batch")}
 
 
 def multimesh_leg(args, log):
@@ -238,7 +261,7 @@ def adjoint_leg(args, log):
     src = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
     rx = receiver_line(101, z=-1000.0)
     prob = InversionProblem(T, src, [0.25, 1.0], rx, rtol=args.rtol, batched=True,
-                            device=torch.device("cuda"))
+                            precision=args.batch_precision, device=torch.device("cuda"))
     prob.observed = prob.forward(np.log(w_true.sigma[:, 0]))
     m0 = torch.tensor(np.log(w0.sigma[:, 0]), dtype=torch.float64, device="cuda")
     times = []
@@ -255,7 +278,8 @@ def adjoint_leg(args, log):
             "n_params": int(T.n_elems), "solves_per_gradient": 32,
             "workload": (f"c5: p=2, 30^3 Kuhn x 6 = {T.n_elems} tets, {T.ndof} DOFs; 8 HED x "
                          "2 frequencies (0.25, 1 Hz), 101 receivers; forward + adjoint as two "
-                         f"batched 16-column solves, rtol {args.rtol}, + element contraction")}
+                         f"batched 16-column solves ({args.batch_precision}), rtol {args.rtol}, "
+                         "+ element contraction")}
 
 
 def main():
@@ -272,6 +296,8 @@ def main():
                     help="comma list of: solve (headline), batch (RHS/s), multimesh (RHS/s), "
                          "adjoint (grad/s)")
     ap.add_argument("--batch-width", type=int, default=64)
+    ap.add_argument("--batch-precision", default="fp64", choices=["fp64", "mixed"],
+                    help="batched and adjoint legs: FP64, or complex64 under FP64 refinement")
     ap.add_argument("--batch-positions", type=int, default=16,
                     help="HED positions of the batched leg (of 64; x 4 frequencies)")
     args = ap.parse_args()

@@ -278,7 +278,13 @@ int pg_batch_solve(void *batch, const double *b, double *x, int32_t *iterations,
  * entity; the batch keeps schedule-order copies of the column lists, K and M. */
 int pg_batch_set_entities(void *batch, const int32_t *block_ptr, int64_t nblocks,
                           const int32_t *order, void *stream);
-/* Re-derive the Jacobi diagonals after the caller has reassembled K and M in place. */
+/* Optional, after pg_batch_set_entities: complex64 COCG per column under FP64
+ * iterative refinement (the FP64 residual b - A x is formed whenever a column's
+ * complex64 residual has dropped by inner_rtol, and the column's complex64
+ * recursion restarts from it).  Keeps complex64 vectors and float K, M. */
+int pg_batch_set_mixed(void *batch, double inner_rtol, void *stream);
+/* Re-derive the Jacobi diagonals (and the schedule-order copies of K and M)
+ * after the caller has reassembled K and M in place. */
 int pg_batch_refresh(void *batch, void *stream);
 int pg_batch_destroy(void *batch);
 

@@ -72,8 +72,6 @@ class InversionProblem:
         nnz = self.asm.plan.nnz
         self.batched = bool(batched)
         if self.batched:
-            if precision != "fp64":
-                raise ValueError("the batched solver runs in FP64")
             self.K = torch.empty(nnz, dtype=torch.float64, device=self.mesh.device)
             self.M = torch.empty_like(self.K)
             cols = [(si, fi) for fi in range(len(self.freqs)) for si in range(len(self.sources))]
@@ -104,7 +102,8 @@ class InversionProblem:
                     self.batches[ci] = core.BatchSolver(self.asm.plan, self.K, self.M, fr,
                                                         rtol=self.rtol,
                                                         block_ptr=self.mesh.block_ptr(),
-                                                        entity_order=self.mesh.entity_order())
+                                                        entity_order=self.mesh.entity_order(),
+                                                        precision=self.precision)
                 else:
                     self.batches[ci].refresh()
             return

@@ -258,7 +258,7 @@ class BatchSolver:
     """Many right-hand sides (each with its own frequency) in one batched COCG."""
 
     def __init__(self, plan, K, M, frequencies, rtol=1e-8, maxit=100000, block_ptr=None,
-                 entity_order=None, stream=None):
+                 entity_order=None, precision="fp64", inner_rtol=1e-4, stream=None):
         self.plan, self.K, self.M = plan, K, M
         self.omega = np.array([omega_of(f) for f in frequencies], dtype=np.float64)
         self.nrhs = self.omega.size
@@ -276,6 +276,13 @@ class BatchSolver:
             nat.check(nat.lib().pg_batch_set_entities(self.handle, bp.ctypes.data, bp.size - 1,
                                                       None if od is None else od.ctypes.data,
                                                       nat.stream_handle(stream)))
+        if precision == "mixed":
+            if block_ptr is None:
+                raise ValueError("the mixed-precision batch needs block_ptr (entity rows)")
+            nat.check(nat.lib().pg_batch_set_mixed(self.handle, float(inner_rtol),
+                                                   nat.stream_handle(stream)))
+        elif precision != "fp64":
+            raise ValueError(f"precision must be fp64 or mixed, not {precision}")
 
     def refresh(self, stream=None):
         """Re-derive the Jacobi diagonals after K and M were reassembled in place."""

@@ -29,8 +29,20 @@ constexpr int kMaxR = 1024;
 struct Col {  // per-column scalars
   double2 rho, mu, alpha, beta;
   double res2, bnorm2;
-  int32_t iter, done, status, pad;
+  double ref2;    // mixed precision: inner target |r|^2 (0 in FP64)
+  double res64;   // mixed precision: FP64 |b - A x|^2 at the last refinement
+  int32_t iter, done, status, outer;  // outer: converged (mixed precision)
 };
+
+// complex helpers for the two vector precisions
+__device__ inline double2 to_d(double2 v) { return v; }
+__device__ inline double2 to_d(float2 v) { return make_double2(v.x, v.y); }
+template <typename T>
+__device__ inline T from_d(double2 v);
+template <>
+__device__ inline double2 from_d<double2>(double2 v) { return v; }
+template <>
+__device__ inline float2 from_d<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
 struct Ctl {
   double tol2;
@@ -173,15 +185,16 @@ __global__ void __launch_bounds__(kBS) k_spmm(const Ctl* ctl, int64_t n, int R,
 // em[i] = {r0, d | L << 8, cbase, vbase}.  Lanes c of an entity slot cover the
 // columns; U column indices are loaded before their gathers, and the next
 // entity's descriptor is fetched one step ahead.
-template <int CT, int TMAX, int D>
+template <int CT, int TMAX, int D, typename VT, typename MT>
 __global__ void __launch_bounds__(kBS) k_spmm_er(const Ctl* ctl, int64_t nent, int R,
                                                  const int4* __restrict__ em,
                                                  const int32_t* __restrict__ cols,
-                                                 const double* __restrict__ K,
-                                                 const double* __restrict__ M,
+                                                 const MT* __restrict__ K,
+                                                 const MT* __restrict__ M,
                                                  const double* __restrict__ omu,
-                                                 const double2* __restrict__ P,
-                                                 double2* __restrict__ Q, double* part) {
+                                                 const VT* __restrict__ P,
+                                                 VT* __restrict__ Q, double* part) {
+  using RT = decltype(VT::x);
   if (ctl->all_done) return;
   constexpr int EPB = kBS / CT;
   constexpr int U = 4;
@@ -189,11 +202,11 @@ __global__ void __launch_bounds__(kBS) k_spmm_er(const Ctl* ctl, int64_t nent, i
   const int T = R / CT;
   const Sweep sw = xcd_sweep(nent, EPB);
   double v[TMAX][2];
-  double om[TMAX];
+  RT om[TMAX];
 #pragma unroll
   for (int t = 0; t < TMAX; ++t) {
     v[t][0] = v[t][1] = 0.0;
-    om[t] = t < T ? omu[c + t * CT] : 0.0;
+    om[t] = t < T ? (RT)omu[c + t * CT] : (RT)0;
   }
   int64_t ent = sw.first + es;
   int4 mn = make_int4(0, 0, 0, 0);
@@ -202,11 +215,11 @@ __global__ void __launch_bounds__(kBS) k_spmm_er(const Ctl* ctl, int64_t nent, i
     const int r0 = mn.x, d = mn.y & 0xff, L = mn.y >> 8, cbase = mn.z, vbase = mn.w;
     const int64_t nx = ent + sw.stride;
     if (nx < sw.end) mn = em[nx];
-    double2 acc[D][TMAX];
+    VT acc[D][TMAX];
 #pragma unroll
     for (int k = 0; k < D; ++k)
 #pragma unroll
-      for (int t = 0; t < TMAX; ++t) acc[k][t] = make_double2(0.0, 0.0);
+      for (int t = 0; t < TMAX; ++t) acc[k][t] = VT{0, 0};
     for (int j0 = 0; j0 < L; j0 += U) {
       int64_t cj[U];
 #pragma unroll
@@ -214,18 +227,17 @@ __global__ void __launch_bounds__(kBS) k_spmm_er(const Ctl* ctl, int64_t nent, i
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (cj[u] < 0) continue;
-        double2 x[TMAX];
+        VT x[TMAX];
 #pragma unroll
-        for (int t = 0; t < TMAX; ++t)
-          x[t] = t < T ? P[cj[u] + c + t * CT] : make_double2(0.0, 0.0);
+        for (int t = 0; t < TMAX; ++t) x[t] = t < T ? P[cj[u] + c + t * CT] : VT{0, 0};
 #pragma unroll
         for (int k = 0; k < D; ++k)
           if (k < d) {
-            const double kv = K[vbase + k * L + j0 + u], mv = M[vbase + k * L + j0 + u];
+            const RT kv = K[vbase + k * L + j0 + u], mv = M[vbase + k * L + j0 + u];
 #pragma unroll
             for (int t = 0; t < TMAX; ++t)
               if (t < T) {
-                const double ai = -om[t] * mv;
+                const RT ai = -om[t] * mv;
                 acc[k][t].x += kv * x[t].x - ai * x[t].y;
                 acc[k][t].y += kv * x[t].y + ai * x[t].x;
               }
@@ -240,9 +252,9 @@ __global__ void __launch_bounds__(kBS) k_spmm_er(const Ctl* ctl, int64_t nent, i
           if (t < T) {
             const int64_t idx = (int64_t)(r0 + k) * R + c + t * CT;
             Q[idx] = acc[k][t];
-            const double2 pv = P[idx];
-            v[t][0] += pv.x * acc[k][t].x - pv.y * acc[k][t].y;
-            v[t][1] += pv.x * acc[k][t].y + pv.y * acc[k][t].x;
+            const double2 pv = to_d(P[idx]), qv = to_d(acc[k][t]);
+            v[t][0] += pv.x * qv.x - pv.y * qv.y;
+            v[t][1] += pv.x * qv.y + pv.y * qv.x;
           }
   }
   column_partials<CT, 2, TMAX>(R, v, part);
@@ -261,19 +273,20 @@ __global__ void __launch_bounds__(kBS) k_sched_cols(int64_t nent, const int4* __
   const int L = m.y >> 8, base = src[i];
   for (int t = threadIdx.x % G; t < L; t += G) cols_s[m.z + t] = cols[base + t];
 }
+template <typename MT>
 __global__ void __launch_bounds__(kBS) k_sched_km(int64_t nent, const int4* __restrict__ em,
                                                   const int32_t* __restrict__ src,
                                                   const double* __restrict__ K,
                                                   const double* __restrict__ M,
-                                                  double* __restrict__ Ks, double* __restrict__ Ms) {
+                                                  MT* __restrict__ Ks, MT* __restrict__ Ms) {
   constexpr int G = 32;
   const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   if (i >= nent) return;
   const int4 m = em[i];
   const int len = (m.y & 0xff) * (m.y >> 8), base = src[i];
   for (int t = threadIdx.x % G; t < len; t += G) {
-    Ks[m.w + t] = K[base + t];
-    Ms[m.w + t] = M[base + t];
+    Ks[m.w + t] = (MT)K[base + t];
+    Ms[m.w + t] = (MT)M[base + t];
   }
 }
 
@@ -323,15 +336,15 @@ __global__ void __launch_bounds__(kBS) k_fin_mu(const Ctl* ctl, Col* col, int R,
 }
 
 // x += alpha p; r -= alpha q; z = r / diag_r; partials of rho = r^T z and |r|^2
-template <int CT, int TMAX>
+template <int CT, int TMAX, typename VT>
 __global__ void __launch_bounds__(kBS) k_update(const Ctl* ctl, const Col* col, int64_t n, int R,
                                                 const double* __restrict__ dK,
                                                 const double* __restrict__ dM,
                                                 const double* __restrict__ omu,
-                                                double2* __restrict__ X, double2* __restrict__ Rr,
-                                                const double2* __restrict__ P,
-                                                const double2* __restrict__ Q,
-                                                double2* __restrict__ Z, double* part) {
+                                                VT* __restrict__ X, VT* __restrict__ Rr,
+                                                const VT* __restrict__ P,
+                                                const VT* __restrict__ Q,
+                                                VT* __restrict__ Z, double* part) {
   if (ctl->all_done) return;
   constexpr int RPB = kBS / CT;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
@@ -339,12 +352,12 @@ __global__ void __launch_bounds__(kBS) k_update(const Ctl* ctl, const Col* col, 
   int64_t lo, hi;
   rows_of_block(n, &lo, &hi);
   double v[TMAX][3];
-  double2 al[TMAX];
+  VT al[TMAX];
   double om[TMAX];
 #pragma unroll
   for (int t = 0; t < TMAX; ++t) {
     v[t][0] = v[t][1] = v[t][2] = 0.0;
-    al[t] = t < T ? col[c + t * CT].alpha : make_double2(0.0, 0.0);
+    al[t] = from_d<VT>(t < T ? col[c + t * CT].alpha : make_double2(0.0, 0.0));
     om[t] = t < T ? omu[c + t * CT] : 0.0;
   }
   for (int64_t row = lo + rs; row < hi; row += RPB) {
@@ -353,19 +366,21 @@ __global__ void __launch_bounds__(kBS) k_update(const Ctl* ctl, const Col* col, 
     for (int t = 0; t < TMAX; ++t)
       if (t < T) {
         const int64_t idx = row * R + c + t * CT;
-        const double2 pv = P[idx], qv = Q[idx];
-        double2 xv = X[idx], rv = Rr[idx];
+        const VT pv = P[idx], qv = Q[idx];
+        VT xv = X[idx], rv = Rr[idx];
         xv.x += al[t].x * pv.x - al[t].y * pv.y;
         xv.y += al[t].x * pv.y + al[t].y * pv.x;
         rv.x -= al[t].x * qv.x - al[t].y * qv.y;
         rv.y -= al[t].x * qv.y + al[t].y * qv.x;
         X[idx] = xv;
         Rr[idx] = rv;
-        const double2 zv = zdiv(rv, make_double2(kd, -om[t] * md));
+        const double2 rd = to_d(rv);
+        const VT zv = from_d<VT>(zdiv(rd, make_double2(kd, -om[t] * md)));
         Z[idx] = zv;
-        v[t][0] += rv.x * zv.x - rv.y * zv.y;
-        v[t][1] += rv.x * zv.y + rv.y * zv.x;
-        v[t][2] += rv.x * rv.x + rv.y * rv.y;
+        const double2 zd = to_d(zv);
+        v[t][0] += rd.x * zd.x - rd.y * zd.y;
+        v[t][1] += rd.x * zd.y + rd.y * zd.x;
+        v[t][2] += rd.x * rd.x + rd.y * rd.y;
       }
   }
   column_partials<CT, 3, TMAX>(R, v, part);
@@ -392,8 +407,8 @@ __global__ void __launch_bounds__(kBS) k_fin_update(const Ctl* ctl, Col* col, in
   if (!isfinite(tot[2]) || !isfinite(beta.x) || !isfinite(beta.y)) {
     cc.status = PG_ERR_BREAKDOWN;
     cc.done = 1;
-  } else if (tot[2] <= ctl->tol2 * cc.bnorm2 || cc.iter >= ctl->maxit) {
-    cc.done = 1;
+  } else if (tot[2] <= ctl->tol2 * cc.bnorm2 || tot[2] <= cc.ref2 || cc.iter >= ctl->maxit) {
+    cc.done = 1;  // converged, refinement point (mixed), or iteration limit
   }
 }
 
@@ -411,27 +426,26 @@ __global__ void __launch_bounds__(kBS) k_fin_all(Ctl* ctl, const Col* col, int R
 }
 
 // p = z + beta p (done columns keep p; their alpha is 0)
-template <int CT, int TMAX>
+template <int CT, int TMAX, typename VT>
 __global__ void __launch_bounds__(kBS) k_pupdate(const Ctl* ctl, const Col* col, int64_t n, int R,
-                                                 const double2* __restrict__ Z,
-                                                 double2* __restrict__ P) {
+                                                 const VT* __restrict__ Z, VT* __restrict__ P) {
   if (ctl->all_done) return;
   constexpr int RPB = kBS / CT;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
   const int T = R / CT;
   int64_t lo, hi;
   rows_of_block(n, &lo, &hi);
-  double2 be[TMAX];
+  VT be[TMAX];
 #pragma unroll
-  for (int t = 0; t < TMAX; ++t) be[t] = t < T ? col[c + t * CT].beta : make_double2(0.0, 0.0);
+  for (int t = 0; t < TMAX; ++t)
+    be[t] = from_d<VT>(t < T ? col[c + t * CT].beta : make_double2(0.0, 0.0));
   for (int64_t row = lo + rs; row < hi; row += RPB)
 #pragma unroll
     for (int t = 0; t < TMAX; ++t)
       if (t < T) {
         const int64_t idx = row * R + c + t * CT;
-        const double2 pv = P[idx], zv = Z[idx];
-        P[idx] = make_double2(zv.x + be[t].x * pv.x - be[t].y * pv.y,
-                              zv.y + be[t].x * pv.y + be[t].y * pv.x);
+        const VT pv = P[idx], zv = Z[idx];
+        P[idx] = VT{zv.x + be[t].x * pv.x - be[t].y * pv.y, zv.y + be[t].x * pv.y + be[t].y * pv.x};
       }
 }
 
@@ -481,13 +495,10 @@ __global__ void __launch_bounds__(kBS) k_fin_init(Col* col, int R, const double*
   column_totals<3>(part, R, np, tot);
   if (threadIdx.x != 0) return;
   Col& cc = col[blockIdx.x];
+  cc = Col{};  // every field, including the refinement target (0: none in FP64)
   cc.rho = make_double2(tot[0], tot[1]);
   cc.res2 = tot[2];
   cc.bnorm2 = tot[2];
-  cc.beta = make_double2(0.0, 0.0);
-  cc.alpha = make_double2(0.0, 0.0);
-  cc.iter = 0;
-  cc.status = 0;
   cc.done = (cc.bnorm2 == 0.0) ? 1 : 0;
 }
 
@@ -625,6 +636,104 @@ __global__ void k_ctl_resume(Ctl* ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctl->all_done = 0;
 }
 
+// Mixed precision, refinement step: r64 = b - A x (FP64, the caller's K and M),
+// then the complex64 recursion restarts from it: r = demote(r64), z = r / diag,
+// p = 0, x32 = 0.  Partials of rho = r^T z (complex64 values) and |r64|^2.
+template <int CT, int TMAX>
+__global__ void __launch_bounds__(kBS) k_refine(int64_t n, int R,
+                                                const int32_t* __restrict__ rowptr,
+                                                const int32_t* __restrict__ cols,
+                                                const double* __restrict__ K,
+                                                const double* __restrict__ M,
+                                                const double* __restrict__ dK,
+                                                const double* __restrict__ dM,
+                                                const double* __restrict__ omu,
+                                                const double2* __restrict__ X,
+                                                const double2* __restrict__ B,
+                                                float2* __restrict__ Rr, float2* __restrict__ Z,
+                                                float2* __restrict__ P, float2* __restrict__ X32,
+                                                double* part) {
+  constexpr int RPB = kBS / CT;
+  const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
+  const int T = R / CT;
+  const Sweep sw = xcd_sweep(n, RPB);
+  double v[TMAX][3];
+  double om[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    v[t][0] = v[t][1] = v[t][2] = 0.0;
+    om[t] = t < T ? omu[c + t * CT] : 0.0;
+  }
+  for (int64_t row = sw.first + rs; row < sw.end; row += sw.stride) {
+    double2 acc[TMAX];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t) acc[t] = make_double2(0.0, 0.0);
+    for (int k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+      const double kv = K[k], mv = M[k];
+      const int64_t cj = (int64_t)cols[k] * R;
+#pragma unroll
+      for (int t = 0; t < TMAX; ++t)
+        if (t < T) {
+          const double2 x = X[cj + c + t * CT];
+          const double ai = -om[t] * mv;
+          acc[t].x += kv * x.x - ai * x.y;
+          acc[t].y += kv * x.y + ai * x.x;
+        }
+    }
+    const double kd = dK[row], md = dM[row];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int64_t idx = row * R + c + t * CT;
+        const double2 bv = B[idx];
+        const double2 r64 = make_double2(bv.x - acc[t].x, bv.y - acc[t].y);
+        const float2 rv = from_d<float2>(r64);
+        Rr[idx] = rv;
+        const float2 zv = from_d<float2>(zdiv(to_d(rv), make_double2(kd, -om[t] * md)));
+        Z[idx] = zv;
+        P[idx] = make_float2(0.f, 0.f);
+        X32[idx] = make_float2(0.f, 0.f);
+        const double2 rd = to_d(rv), zd = to_d(zv);
+        v[t][0] += rd.x * zd.x - rd.y * zd.y;
+        v[t][1] += rd.x * zd.y + rd.y * zd.x;
+        v[t][2] += r64.x * r64.x + r64.y * r64.y;
+      }
+  }
+  column_partials<CT, 3, TMAX>(R, v, part);
+}
+
+// finish a refinement step: first time |b|^2 (x = 0); a column whose FP64
+// residual meets the tolerance (or that stopped) is finished for good, the
+// others restart the complex64 recursion with target |r|^2 <= res64 * delta2
+__global__ void __launch_bounds__(kBS) k_fin_refine(const Ctl* ctl, Col* col, int R,
+                                                    const double* part, int np, int first,
+                                                    double delta2) {
+  double tot[3];
+  column_totals<3>(part, R, np, tot);
+  if (threadIdx.x != 0) return;
+  Col& cc = col[blockIdx.x];
+  if (first) {
+    cc = Col{};
+    cc.bnorm2 = tot[2];
+  }
+  cc.res64 = tot[2];
+  cc.res2 = tot[2];
+  cc.rho = make_double2(tot[0], tot[1]);
+  cc.beta = make_double2(0.0, 0.0);
+  cc.alpha = make_double2(0.0, 0.0);
+  cc.ref2 = tot[2] * delta2;
+  cc.outer = (cc.status != 0 || cc.iter >= ctl->maxit || tot[2] <= ctl->tol2 * cc.bnorm2) ? 1 : 0;
+  cc.done = cc.outer;
+}
+
+// x64 += promote(x32) for every column
+__global__ void k_accum(int64_t n, double2* __restrict__ X, const float2* __restrict__ X32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 v = X32[i];
+  X[i] = make_double2(X[i].x + (double)v.x, X[i].y + (double)v.y);
+}
+
 __global__ void k_diag(int64_t n, const int32_t* __restrict__ rowptr,
                        const int32_t* __restrict__ cols, const double* __restrict__ K,
                        const double* __restrict__ M, double* __restrict__ dK,
@@ -683,6 +792,12 @@ struct Batch {
   int32_t *esrc = nullptr, *cols_s = nullptr;
   double *Ks = nullptr, *Ms = nullptr;
   int dmax = 0, egrid = 1;
+  // mixed precision (optional): complex64 recursion vectors, float K, M in
+  // schedule order, residual drop per refinement
+  bool mixed = false;
+  double delta2 = 1e-8;
+  float2 *R32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *Z32 = nullptr, *X32 = nullptr;
+  float *Ks32 = nullptr, *Ms32 = nullptr;
   std::vector<void*> owned;
   ~Batch() {
     for (void* v : owned) hipFree(v);
@@ -700,32 +815,54 @@ struct Batch {
   }
 };
 
-template <int CT, int TM>
-static void launch_iter(Batch* B, hipStream_t st, double2* X) {
+template <int CT, int TM, typename VT, typename MT>
+static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q, VT* Z,
+                          const MT* Ks, const MT* Ms) {
   const dim3 g(B->grid), b(kBS), gr(B->R);
-  hipLaunchKernelGGL((k_pupdate<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->Z, B->P);
+  hipLaunchKernelGGL((k_pupdate<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->n, B->R, Z, P);
   int np = B->grid;
   if (B->em) {
     const dim3 ge(B->egrid);
     np = B->egrid;
     if (B->dmax <= 1)
-      hipLaunchKernelGGL((k_spmm_er<CT, TM, 1>), ge, b, 0, st, B->ctl, B->nent, B->R, B->em,
-                         B->cols_s, B->Ks, B->Ms, B->omu, B->P, B->Q, B->part);
+      hipLaunchKernelGGL((k_spmm_er<CT, TM, 1, VT, MT>), ge, b, 0, st, B->ctl, B->nent, B->R,
+                         B->em, B->cols_s, Ks, Ms, B->omu, P, Q, B->part);
     else if (B->dmax <= 2)
-      hipLaunchKernelGGL((k_spmm_er<CT, TM, 2>), ge, b, 0, st, B->ctl, B->nent, B->R, B->em,
-                         B->cols_s, B->Ks, B->Ms, B->omu, B->P, B->Q, B->part);
+      hipLaunchKernelGGL((k_spmm_er<CT, TM, 2, VT, MT>), ge, b, 0, st, B->ctl, B->nent, B->R,
+                         B->em, B->cols_s, Ks, Ms, B->omu, P, Q, B->part);
     else
-      hipLaunchKernelGGL((k_spmm_er<CT, TM, 6>), ge, b, 0, st, B->ctl, B->nent, B->R, B->em,
-                         B->cols_s, B->Ks, B->Ms, B->omu, B->P, B->Q, B->part);
+      hipLaunchKernelGGL((k_spmm_er<CT, TM, 6, VT, MT>), ge, b, 0, st, B->ctl, B->nent, B->R,
+                         B->em, B->cols_s, Ks, Ms, B->omu, P, Q, B->part);
   } else {
     hipLaunchKernelGGL((k_spmm<CT, TM>), g, b, 0, st, B->ctl, B->n, B->R, B->rowptr, B->cols,
-                       B->K, B->M, B->omu, B->P, B->Q, B->part);
+                       B->K, B->M, B->omu, (const double2*)P, (double2*)Q, B->part);
   }
   hipLaunchKernelGGL(k_fin_mu, gr, b, 0, st, B->ctl, B->col, B->R, B->part, np);
-  hipLaunchKernelGGL((k_update<CT, TM>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK, B->dM,
-                     B->omu, X, B->Rr, B->P, B->Q, B->Z, B->part);
+  hipLaunchKernelGGL((k_update<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK,
+                     B->dM, B->omu, X, Rr, P, Q, Z, B->part);
   hipLaunchKernelGGL(k_fin_update, gr, b, 0, st, B->ctl, B->col, B->R, B->part, B->grid);
   hipLaunchKernelGGL(k_fin_all, dim3(1), b, 0, st, B->ctl, B->col, B->R);
+}
+
+template <int CT, int TM>
+static void launch_iter(Batch* B, hipStream_t st, double2* X) {
+  if (B->mixed)
+    launch_iter_t<CT, TM, float2, float>(B, st, B->X32, B->R32, B->P32, B->Q32, B->Z32, B->Ks32,
+                                         B->Ms32);
+  else
+    launch_iter_t<CT, TM, double2, double>(B, st, X, B->Rr, B->P, B->Q, B->Z, B->Ks, B->Ms);
+}
+
+template <int CT, int TM>
+static void launch_refine(Batch* B, hipStream_t st, const double2* Bv, const double2* X,
+                          int first) {
+  hipLaunchKernelGGL((k_refine<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->rowptr,
+                     B->cols, B->K, B->M, B->dK, B->dM, B->omu, X, Bv, B->R32, B->Z32, B->P32,
+                     B->X32, B->part);
+  hipLaunchKernelGGL(k_fin_refine, dim3(B->R), dim3(kBS), 0, st, B->ctl, B->col, B->R, B->part,
+                     B->grid, first, B->delta2);
+  hipLaunchKernelGGL(k_ctl_resume, dim3(1), dim3(64), 0, st, B->ctl);
+  hipLaunchKernelGGL(k_fin_all, dim3(1), dim3(kBS), 0, st, B->ctl, B->col, B->R);
 }
 
 template <int CT, int TM>
@@ -745,9 +882,13 @@ static void launch_true(Batch* B, hipStream_t st, const double2* Bv, const doubl
 // Jacobi diagonals K_ii, M_ii from the caller's K and M, and the schedule-order
 // copies of K and M (at creation and after the caller has reassembled them in place)
 static int refresh_diag(Batch* B, hipStream_t st) {
-  if (B->em)
-    hipLaunchKernelGGL(k_sched_km, dim3((32 * B->nent + kBS - 1) / kBS), dim3(kBS), 0, st, B->nent,
-                       B->em, B->esrc, B->K, B->M, B->Ks, B->Ms);
+  if (B->em) {
+    hipLaunchKernelGGL((k_sched_km<double>), dim3((32 * B->nent + kBS - 1) / kBS), dim3(kBS), 0,
+                       st, B->nent, B->em, B->esrc, B->K, B->M, B->Ks, B->Ms);
+    if (B->mixed)
+      hipLaunchKernelGGL((k_sched_km<float>), dim3((32 * B->nent + kBS - 1) / kBS), dim3(kBS), 0,
+                         st, B->nent, B->em, B->esrc, B->K, B->M, B->Ks32, B->Ms32);
+  }
   PG_HIP(hipMemsetAsync(B->bad, 0, sizeof(int32_t), st));
   hipLaunchKernelGGL(k_diag, dim3((B->n + kBS - 1) / kBS), dim3(kBS), 0, st, B->n, B->rowptr,
                      B->cols, B->K, B->M, B->dK, B->dM, B->bad);
@@ -868,31 +1009,57 @@ int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations,
   double2* X = reinterpret_cast<double2*>(x);
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, B->ctl, B->maxit, B->rtol * B->rtol,
                      B->R);
-  PG_BATCH_DISPATCH(launch_init, B, st, Bv, X);
-  PG_CHECK_LAUNCH();
-  // iterate until every column's recursion has converged, then check the true
-  // residuals; columns above the tolerance restart from their current iterate
-  for (int restart = 0;; ++restart) {
-    while (true) {
-      for (int k = 0; k < B->check; ++k) PG_BATCH_DISPATCH(launch_iter, B, st, X);
-      PG_CHECK_LAUNCH();
+  if (B->mixed) {
+    // iterative refinement: complex64 COCG rounds between FP64 residuals
+    PG_HIP(hipMemsetAsync(X, 0, (size_t)B->n * B->R * sizeof(double2), st));
+    PG_BATCH_DISPATCH(launch_refine, B, st, Bv, X, 1);
+    PG_CHECK_LAUNCH();
+    for (int round = 0;; ++round) {
       PG_HIP(hipMemcpyAsync(B->hctl, B->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
       PG_HIP(hipStreamSynchronize(st));
-      if (B->hctl->all_done) break;
+      if (B->hctl->all_done || round >= 200) break;
+      while (true) {
+        for (int k = 0; k < B->check; ++k) PG_BATCH_DISPATCH(launch_iter, B, st, X);
+        PG_CHECK_LAUNCH();
+        PG_HIP(hipMemcpyAsync(B->hctl, B->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        if (B->hctl->all_done) break;
+      }
+      hipLaunchKernelGGL(k_accum, dim3((unsigned)(((int64_t)B->n * B->R + kBS - 1) / kBS)),
+                         dim3(kBS), 0, st, (int64_t)B->n * B->R, X, B->X32);
+      PG_BATCH_DISPATCH(launch_refine, B, st, Bv, X, 0);
+      PG_CHECK_LAUNCH();
     }
-    PG_BATCH_DISPATCH(launch_true, B, st, Bv, X);
-    PG_CHECK_LAUNCH();
     PG_HIP(hipMemcpyAsync(B->hcol, B->col, B->R * sizeof(Col), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
-    bool again = false;
-    for (int r = 0; r < B->R; ++r) {
-      const Col& c = B->hcol[r];
-      if (c.status == 0 && c.iter < B->maxit && c.res2 > B->rtol * B->rtol * c.bnorm2)
-        again = true;
-    }
-    if (!again || restart >= 20) break;
-    PG_BATCH_DISPATCH(launch_reinit, B, st, Bv, X);
+    for (int r = 0; r < B->R; ++r) B->hcol[r].res2 = B->hcol[r].res64;
+  } else {
+    PG_BATCH_DISPATCH(launch_init, B, st, Bv, X);
     PG_CHECK_LAUNCH();
+    // iterate until every column's recursion has converged, then check the true
+    // residuals; columns above the tolerance restart from their current iterate
+    for (int restart = 0;; ++restart) {
+      while (true) {
+        for (int k = 0; k < B->check; ++k) PG_BATCH_DISPATCH(launch_iter, B, st, X);
+        PG_CHECK_LAUNCH();
+        PG_HIP(hipMemcpyAsync(B->hctl, B->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+        PG_HIP(hipStreamSynchronize(st));
+        if (B->hctl->all_done) break;
+      }
+      PG_BATCH_DISPATCH(launch_true, B, st, Bv, X);
+      PG_CHECK_LAUNCH();
+      PG_HIP(hipMemcpyAsync(B->hcol, B->col, B->R * sizeof(Col), hipMemcpyDeviceToHost, st));
+      PG_HIP(hipStreamSynchronize(st));
+      bool again = false;
+      for (int r = 0; r < B->R; ++r) {
+        const Col& c = B->hcol[r];
+        if (c.status == 0 && c.iter < B->maxit && c.res2 > B->rtol * B->rtol * c.bnorm2)
+          again = true;
+      }
+      if (!again || restart >= 20) break;
+      PG_BATCH_DISPATCH(launch_reinit, B, st, Bv, X);
+      PG_CHECK_LAUNCH();
+    }
   }
   for (int r = 0; r < B->R; ++r) {
     const Col& c = B->hcol[r];
@@ -975,6 +1142,24 @@ int pg_batch_set_entities(void* batch, const int32_t* block_ptr, int64_t nblocks
   B->dmax = dmax;
   B->em = dem;
   return refresh_diag(B, st);
+}
+
+int pg_batch_set_mixed(void* batch, double inner_rtol, void* stream) {
+  if (!batch) PG_FAIL(PG_ERR_ARG, "batch is NULL");
+  Batch* B = static_cast<Batch*>(batch);
+  if (!B->em) PG_FAIL(PG_ERR_ARG, "mixed precision needs the entities (pg_batch_set_entities)");
+  if (B->mixed) return PG_OK;
+  const size_t nr = (size_t)B->n * B->R;
+  int rc;
+  if ((rc = B->alloc(&B->R32, nr)) != PG_OK || (rc = B->alloc(&B->P32, nr)) != PG_OK ||
+      (rc = B->alloc(&B->Q32, nr)) != PG_OK || (rc = B->alloc(&B->Z32, nr)) != PG_OK ||
+      (rc = B->alloc(&B->X32, nr)) != PG_OK || (rc = B->alloc(&B->Ks32, B->nnz)) != PG_OK ||
+      (rc = B->alloc(&B->Ms32, B->nnz)) != PG_OK)
+    return rc;
+  const double d = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 1e-4;
+  B->delta2 = d * d;
+  B->mixed = true;
+  return refresh_diag(B, as_stream(stream));
 }
 
 int pg_batch_refresh(void* batch, void* stream) {

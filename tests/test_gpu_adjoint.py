@@ -45,8 +45,8 @@ def test_device_gradient_vs_oracle_and_fd(p):
         assert abs(fd - g[e]) <= 1e-5 * np.abs(g).max() + 1e-4 * abs(fd), (e, fd, g[e])
 
 
-@pytest.mark.parametrize("max_batch", [64, 4])
-def test_batched_gradient_matches_per_rhs(max_batch):
+@pytest.mark.parametrize("max_batch,precision", [(64, "fp64"), (4, "fp64"), (64, "mixed")])
+def test_batched_gradient_matches_per_rhs(max_batch, precision):
     """Batched forward/adjoint solves (zero-padded widths, several chunks) give the
     per-right-hand-side gradient."""
     w = Workload(4, 2, block=False)
@@ -58,7 +58,8 @@ def test_batched_gradient_matches_per_rhs(max_batch):
     m0 = np.log(w.sigma[:, 0])
     seq = InversionProblem(T, src, freqs, rx, rtol=1e-12)
     seq.observed = seq.forward(true)
-    bat = InversionProblem(T, src, freqs, rx, rtol=1e-12, batched=True, max_batch=max_batch)
+    bat = InversionProblem(T, src, freqs, rx, rtol=1e-12, batched=True, max_batch=max_batch,
+                           precision=precision)
     obs = bat.forward(true)
     assert np.abs(obs - seq.observed).max() <= 1e-9 * np.abs(seq.observed).max()
     bat.observed = seq.observed
