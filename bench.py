@@ -291,6 +291,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of host work for the CPU baseline sample (0 = skip)")
     ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--frequency", type=float, default=None,
+                    help="source frequency of the solve leg (default: the workload's, 1 Hz)")
     ap.add_argument("--pc", default="bjacobi", choices=["jacobi", "bjacobi"])
     ap.add_argument("--legs", default="solve,batch,multimesh,adjoint",
                     help="comma list of: solve (headline), batch (RHS/s), multimesh (RHS/s), "
@@ -310,7 +312,9 @@ def main():
     from petgem_amd import core
     from petgem_amd.synthetic import CONFIGS, Workload
 
-    cfg = CONFIGS[args.config]
+    cfg = dict(CONFIGS[args.config])
+    if args.frequency:
+        cfg["frequency"] = args.frequency
     t0 = time.perf_counter()
     w = Workload(**cfg, seed=2026)
     T = w.topo

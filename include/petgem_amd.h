@@ -219,7 +219,9 @@ typedef struct pg_solve_opts {
   const int32_t *block_ptr; /* PG_PC_BJACOBI: host array (nblocks+1), DOF ranges, sizes 1..8 */
   int64_t nblocks;
   int32_t spmv;       /* PG_SPMV_*: SpMV kernel family (AUTO = fastest by timing) */
-  int32_t pad;
+  int32_t stall_iters;/* mixed: iterations without a 10 % drop of a complex64 round's best
+                         residual after which the round counts as stagnated and the solve
+                         continues in FP64 (0 = 400) */
   const int32_t *entity_order; /* optional host permutation of the nblocks entities: the
                                   order in which the entity-row SpMV visits them (a
                                   spatially local order keeps the gathers of x in cache);
@@ -230,7 +232,7 @@ typedef struct pg_solve_info {
   int32_t iterations;       /* Krylov iterations (inner iterations for mixed) */
   int32_t refinements;      /* FP64 residual replacements (mixed) */
   int32_t status;           /* PG_OK, PG_ERR_BREAKDOWN, or -1 = not converged */
-  int32_t pad;
+  int32_t fp64_fallback;    /* mixed: 1 if a stagnating complex64 round handed over to FP64 */
   double relres;            /* true FP64 ||b - A x|| / ||b|| at exit */
   double spmv_ms;           /* COCG: mean duration of the iterations' SpMV kernel in this
                                solve (HIP events around one launch per host check) */

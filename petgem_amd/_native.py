@@ -31,12 +31,13 @@ class PlanDev(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("ksp", _I32), ("pc", _I32), ("precision", _I32), ("restart", _I32),
                 ("maxit", _I32), ("check_every", _I32), ("rtol", _D), ("inner_rtol", _D),
-                ("block_ptr", _P), ("nblocks", _I64), ("spmv", _I32), ("pad", _I32),
+                ("block_ptr", _P), ("nblocks", _I64), ("spmv", _I32), ("stall_iters", _I32),
                 ("entity_order", _P)]
 
 
 class SolveInfo(ctypes.Structure):
-    _fields_ = [("iterations", _I32), ("refinements", _I32), ("status", _I32), ("pad", _I32),
+    _fields_ = [("iterations", _I32), ("refinements", _I32), ("status", _I32),
+                ("fp64_fallback", _I32),
                 ("relres", _D), ("spmv_ms", _D), ("spmv_samples", _I32), ("pad2", _I32)]
 
 

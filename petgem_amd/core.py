@@ -425,7 +425,7 @@ class KrylovSolver:
 
     def __init__(self, plan, values, ksp="cocg", pc="jacobi", precision="fp64", rtol=1e-8,
                  maxit=100000, check_every=25, inner_rtol=1e-4, restart=30, block_ptr=None,
-                 spmv="auto", entity_order=None, stream=None):
+                 spmv="auto", entity_order=None, stall_iters=0, stream=None):
         self.plan = plan
         self.values = values
         if pc == "bjacobi" and block_ptr is None:
@@ -441,7 +441,7 @@ class KrylovSolver:
         self.opts = nat.SolveOpts(_KSP[ksp], _PC[pc], _PREC[precision], int(restart), int(maxit),
                                   int(check_every), float(rtol), float(inner_rtol),
                                   None if self._bptr is None else self._bptr.ctypes.data, nb,
-                                  _SPMV[spmv], 0,
+                                  _SPMV[spmv], int(stall_iters),
                                   None if self._order is None else self._order.ctypes.data)
         h = ctypes.c_void_p()
         nat.check(nat.lib().pg_solver_create(plan.ndof, plan.nnz, nat.ptr(plan.rowptr),
@@ -471,7 +471,7 @@ class KrylovSolver:
                                             ctypes.byref(info), nat.stream_handle(stream)))
         return x, dict(iterations=info.iterations, refinements=info.refinements,
                        status=info.status, relres=info.relres, spmv_ms=info.spmv_ms,
-                       spmv_samples=info.spmv_samples)
+                       spmv_samples=info.spmv_samples, fp64_fallback=info.fp64_fallback)
 
     def __del__(self):
         try:

@@ -57,7 +57,15 @@ struct Scal {
   double res2, bnorm2, tol2, ref2;
   double res64;
   int32_t iter, maxit, done, status;
+  // stagnation watch of the complex64 recursion (mixed precision): best |r|^2
+  // of the current round and the iteration of its last 10 % improvement
+  double best2;
+  int32_t last_imp, stall_limit, stalled, pad;
 };
+
+// a complex64 round whose best residual norm has not improved by 10 % for
+// this many iterations (default) has stagnated; the solve then continues in FP64
+constexpr int kStallIters = 400;
 
 // ---------------------------------------------------------------------------
 // complex helpers
@@ -459,6 +467,14 @@ __global__ void __launch_bounds__(kBS) k_fin_update(Scal* __restrict__ s, const 
     s->done = 1;
   } else if (tot[2] <= s->tol2 * s->bnorm2 || tot[2] <= s->ref2 || s->iter >= s->maxit) {
     s->done = 1;  // converged recursion, refinement point, or iteration limit
+  } else if (s->stall_limit > 0) {
+    if (tot[2] < 0.81 * s->best2) {
+      s->best2 = tot[2];
+      s->last_imp = s->iter;
+    } else if (s->iter - s->last_imp >= s->stall_limit) {
+      s->stalled = 1;  // stagnated round
+      s->done = 1;
+    }
   }
 }
 
@@ -554,6 +570,9 @@ __global__ void __launch_bounds__(kBS) k_fin_reset(Scal* __restrict__ s, const d
   s->beta = make_double2(0.0, 0.0);
   s->res2 = tot[2];
   if (has_b) s->bnorm2 = tot[3];
+  s->best2 = tot[2];
+  s->last_imp = s->iter;
+  s->stalled = 0;
   s->done = 0;
 }
 
@@ -735,6 +754,9 @@ __global__ void k_set_done(Scal* s, int v) {
 }
 __global__ void k_set_ref2(Scal* s, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) s->ref2 = v;
+}
+__global__ void k_set_stall(Scal* s, int limit) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->stall_limit = limit;
 }
 
 // ---------------------------------------------------------------------------
@@ -1072,12 +1094,13 @@ static void launch_iteration(Solver* S, hipStream_t st, const SpmvCfg& c, const 
 static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* info,
                       hipStream_t st) {
   const bool mixed = S->o.precision == PG_PREC_MIXED;
+  bool low = mixed;  // current recursion in complex64
   const int64_t n = S->n;
   const double tol2 = S->o.rtol * S->o.rtol;
   hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, S->o.maxit, tol2);
   if (mixed) PG_HIP(hipMemsetAsync(S->x32, 0, n * sizeof(float2), st));
   PG_TRY(true_residual(S, st, b, x));
-  if (mixed)
+  if (low)
     launch_reset<float2>(S, st, S->r32, S->dinv32, S->binv32, S->z32, S->p32, b);
   else
     launch_reset<double2>(S, st, S->r, S->dinv, S->binv, S->z, S->p, b);
@@ -1089,6 +1112,7 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
   int spmv_n = 0;
   info->status = -1;
   info->refinements = 0;
+  info->fp64_fallback = 0;
   info->spmv_ms = 0.0;
   info->spmv_samples = 0;
   if (bnorm2 == 0.0) {
@@ -1106,13 +1130,17 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
     return PG_OK;
   }
   const double delta2 = S->o.inner_rtol * S->o.inner_rtol;
-  if (mixed) hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
+  if (low) {
+    hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
+    hipLaunchKernelGGL(k_set_stall, dim3(1), dim3(64), 0, st, S->s,
+                       S->o.stall_iters > 0 ? S->o.stall_iters : kStallIters);
+  }
   const int check = std::max(1, S->o.check_every);
   while (true) {
     // the first iteration of every batch runs in full (the flags were clear
     // when the batch was enqueued): time its SpMV with events on this stream
     for (int k = 0; k < check; ++k) {
-      if (mixed)
+      if (low)
         launch_iteration<float2, float2>(S, st, S->c32, S->vals32, S->x32, S->r32, S->p32, S->q32,
                                          S->dinv32, S->binv32, S->z32, k == 0);
       else
@@ -1134,8 +1162,8 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
       break;
     }
     if (!hs.done) continue;
-    // paused: converged recursion, refinement point, or iteration limit
-    if (mixed) {
+    // paused: converged recursion, refinement point, stagnation or iteration limit
+    if (low) {
       hipLaunchKernelGGL(k_accumulate, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, x, S->x32);
       refinements++;
     }
@@ -1148,7 +1176,14 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
       break;
     }
     if (hs.iter >= S->o.maxit) break;
-    if (!mixed) {
+    if (low && hs.stalled) {
+      // the complex64 recursion stagnated: continue in FP64 from x
+      low = false;
+      info->fp64_fallback = 1;
+      hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, 0.0);
+      hipLaunchKernelGGL(k_set_stall, dim3(1), dim3(64), 0, st, S->s, 0);
+      launch_reset<double2>(S, st, S->r, S->dinv, S->binv, S->z, S->p, nullptr);
+    } else if (!low) {
       // recursive residual converged but the true one did not: restart from x
       if (++restarts > 20) break;
       launch_reset<double2>(S, st, S->r, S->dinv, S->binv, S->z, S->p, nullptr);
@@ -1326,17 +1361,14 @@ static int refresh_values(Solver* S, hipStream_t st) {
                          reinterpret_cast<float*>(S->binv32));
     }
   } else if (S->o.pc == PG_PC_JACOBI) {
+    hipLaunchKernelGGL(k_jacobi_inv<double2>, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->rowptr,
+                       S->cols, S->vals, S->dinv, S->bad);
     if (mixed)
       hipLaunchKernelGGL(k_jacobi_inv<float2>, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->rowptr,
                          S->cols, S->vals, S->dinv32, S->bad);
-    else
-      hipLaunchKernelGGL(k_jacobi_inv<double2>, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n,
-                         S->rowptr, S->cols, S->vals, S->dinv, S->bad);
   } else {
-    if (mixed)
-      hipLaunchKernelGGL(k_ones32, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->dinv32);
-    else
-      hipLaunchKernelGGL(k_ones, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->dinv);
+    hipLaunchKernelGGL(k_ones, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->dinv);
+    if (mixed) hipLaunchKernelGGL(k_ones32, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, S->dinv32);
   }
   PG_CHECK_LAUNCH();
   PG_HIP(hipMemcpyAsync(&S->h->iter, S->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1544,6 +1576,12 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   PG_A(S->q, n);
   PG_A(S->bad, 1);
   PG_A(S->dnorm, 1);
+  // FP64 recursion vectors (also in mixed precision: the fallback when a
+  // complex64 round stagnates)
+  PG_A(S->r, n);
+  PG_A(S->p, n);
+  PG_A(S->z, n);
+  PG_A(S->dinv, n);
   if (mixed) {
     PG_A(S->vals32, nnz);
     PG_A(S->r32, n);
@@ -1552,11 +1590,6 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
     PG_A(S->z32, n);
     PG_A(S->x32, n);
     PG_A(S->dinv32, n);
-  } else {
-    PG_A(S->r, n);
-    PG_A(S->p, n);
-    PG_A(S->z, n);
-    PG_A(S->dinv, n);
   }
   if (opts->block_ptr && opts->nblocks > 0) {
     S->nblocks = opts->nblocks;

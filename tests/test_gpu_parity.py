@@ -299,3 +299,28 @@ def test_c1_full_size_fp64_and_mixed():
     print(f"C1 iters fp64={g64['info']['iterations']} mixed={g32['info']['iterations']} "
           f"(refinements {g32['info']['refinements']}) field diff {d:.2e}", flush=True)
     assert d <= 1e-6
+
+
+def test_mixed_stagnation_hands_over_to_fp64():
+    """A complex64 round declared stagnant (here after a single iteration without a
+    10 % residual drop) hands the solve over to FP64, which still converges."""
+    w = Workload(5, 2, noise=0.1, seed=3)
+    T = w.topo
+    mesh = core.DeviceMesh.from_topology(T, DEV)
+    asm = core.Assembler(mesh)
+    asm.geometry(w.sigma)
+    omega = core.omega_of(w.frequency)
+    ent_bc, dof_bc = mesh.entity_mask(T.boundary_dofs)
+    vals = asm.assemble(omega, ent_bc=ent_bc)
+    el = int(core.locate_points(mesh, np.array(w.source["position"])[None]).cpu()[0])
+    b = core.source_vector(mesh, el, w.source["position"], core.dipole_moment(w.source), omega)
+    core.apply_dirichlet(asm.plan, None, dof_bc, b)
+    ref, _ = core.KrylovSolver(asm.plan, vals, rtol=1e-10).solve(b)
+    S = core.KrylovSolver(asm.plan, vals, precision="mixed", rtol=1e-8, stall_iters=1,
+                          pc="bjacobi", block_ptr=mesh.block_ptr(),
+                          entity_order=mesh.entity_order())
+    x, info = S.solve(b)
+    print(f"stagnation hand-over: {info}", flush=True)
+    assert info["fp64_fallback"] == 1 and info["status"] == 0 and info["relres"] <= 1e-8
+    xr, xh = core.as_complex(ref), core.as_complex(x)
+    assert np.abs(xh - xr).max() <= 1e-6 * np.abs(xr).max()
