@@ -117,7 +117,7 @@ def batched_leg(args, log):
     src_el = core.locate_points(mesh, srcs).cpu().numpy()
     freqs = [0.1, 0.25, 0.5, 1.0]
     cols = [(si, f) for f in freqs for si in range(len(xs))]
-    width = args.batch_width
+    width = args.batch_width or min(64, len(xs))
     chunks = [cols[i:i + width] for i in range(0, len(cols), width)]
     asm.geometry(sigma)
     core.assemble_km(asm, ent_bc, out=(K, M))
@@ -297,8 +297,9 @@ def main():
     ap.add_argument("--legs", default="solve,batch,multimesh,adjoint",
                     help="comma list of: solve (headline), batch (RHS/s), multimesh (RHS/s), "
                          "adjoint (grad/s)")
-    ap.add_argument("--batch-width", type=int, default=64)
-    ap.add_argument("--batch-precision", default="fp64", choices=["fp64", "mixed"],
+    ap.add_argument("--batch-width", type=int, default=0,
+                    help="columns per batched solve (0: one frequency per batch, at most 64)")
+    ap.add_argument("--batch-precision", default="mixed", choices=["fp64", "mixed"],
                     help="batched and adjoint legs: FP64, or complex64 under FP64 refinement")
     ap.add_argument("--batch-positions", type=int, default=16,
                     help="HED positions of the batched leg (of 64; x 4 frequencies)")

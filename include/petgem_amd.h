@@ -283,8 +283,10 @@ int pg_batch_set_entities(void *batch, const int32_t *block_ptr, int64_t nblocks
 /* Optional, after pg_batch_set_entities: complex64 COCG per column under FP64
  * iterative refinement (the FP64 residual b - A x is formed whenever a column's
  * complex64 residual has dropped by inner_rtol, and the column's complex64
- * recursion restarts from it).  Keeps complex64 vectors and float K, M. */
-int pg_batch_set_mixed(void *batch, double inner_rtol, void *stream);
+ * recursion restarts from it).  A column whose complex64 round goes
+ * stall_iters iterations (0 = 400) without a 10 % drop of its best residual
+ * finishes in FP64 from its iterate.  Keeps complex64 vectors and float K, M. */
+int pg_batch_set_mixed(void *batch, double inner_rtol, int stall_iters, void *stream);
 /* Re-derive the Jacobi diagonals (and the schedule-order copies of K and M)
  * after the caller has reassembled K and M in place. */
 int pg_batch_refresh(void *batch, void *stream);

@@ -258,7 +258,8 @@ class BatchSolver:
     """Many right-hand sides (each with its own frequency) in one batched COCG."""
 
     def __init__(self, plan, K, M, frequencies, rtol=1e-8, maxit=100000, block_ptr=None,
-                 entity_order=None, precision="fp64", inner_rtol=1e-4, stream=None):
+                 entity_order=None, precision="fp64", inner_rtol=1e-4, stall_iters=0,
+                 stream=None):
         self.plan, self.K, self.M = plan, K, M
         self.omega = np.array([omega_of(f) for f in frequencies], dtype=np.float64)
         self.nrhs = self.omega.size
@@ -280,7 +281,7 @@ class BatchSolver:
             if block_ptr is None:
                 raise ValueError("the mixed-precision batch needs block_ptr (entity rows)")
             nat.check(nat.lib().pg_batch_set_mixed(self.handle, float(inner_rtol),
-                                                   nat.stream_handle(stream)))
+                                                   int(stall_iters), nat.stream_handle(stream)))
         elif precision != "fp64":
             raise ValueError(f"precision must be fp64 or mixed, not {precision}")
 

@@ -31,7 +31,9 @@ struct Col {  // per-column scalars
   double res2, bnorm2;
   double ref2;    // mixed precision: inner target |r|^2 (0 in FP64)
   double res64;   // mixed precision: FP64 |b - A x|^2 at the last refinement
-  int32_t iter, done, status, outer;  // outer: converged (mixed precision)
+  double best2;   // mixed precision: best |r|^2 of the current complex64 round
+  int32_t iter, done, status, outer;  // outer: finished with complex64 (mixed precision)
+  int32_t last_imp, stalled, pad0, pad1;  // stagnation watch of the complex64 round
 };
 
 // complex helpers for the two vector precisions
@@ -46,7 +48,7 @@ __device__ inline float2 from_d<float2>(double2 v) { return make_float2((float)v
 
 struct Ctl {
   double tol2;
-  int32_t maxit, ncols, all_done, pad;
+  int32_t maxit, ncols, all_done, stall_limit;  // stall_limit: 0 = no watch (FP64)
 };
 
 // chunk of rows for this workgroup (XCD-contiguous renumbering)
@@ -409,6 +411,14 @@ __global__ void __launch_bounds__(kBS) k_fin_update(const Ctl* ctl, Col* col, in
     cc.done = 1;
   } else if (tot[2] <= ctl->tol2 * cc.bnorm2 || tot[2] <= cc.ref2 || cc.iter >= ctl->maxit) {
     cc.done = 1;  // converged, refinement point (mixed), or iteration limit
+  } else if (ctl->stall_limit > 0) {
+    if (tot[2] < 0.81 * cc.best2) {
+      cc.best2 = tot[2];
+      cc.last_imp = cc.iter;
+    } else if (cc.iter - cc.last_imp >= ctl->stall_limit) {
+      cc.stalled = 1;  // complex64 round stagnated: the column finishes in FP64
+      cc.done = 1;
+    }
   }
 }
 
@@ -629,11 +639,15 @@ __global__ void __launch_bounds__(kBS) k_fin_reinit(const Ctl* ctl, Col* col, in
   cc.res2 = tot[2];
   cc.beta = make_double2(0.0, 0.0);
   cc.alpha = make_double2(0.0, 0.0);
+  cc.ref2 = 0.0;
   cc.done = (cc.status != 0 || cc.iter >= ctl->maxit || tot[2] <= ctl->tol2 * cc.bnorm2) ? 1 : 0;
 }
 
 __global__ void k_ctl_resume(Ctl* ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctl->all_done = 0;
+}
+__global__ void k_ctl_stall(Ctl* ctl, int limit) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl->stall_limit = limit;
 }
 
 // Mixed precision, refinement step: r64 = b - A x (FP64, the caller's K and M),
@@ -722,7 +736,10 @@ __global__ void __launch_bounds__(kBS) k_fin_refine(const Ctl* ctl, Col* col, in
   cc.beta = make_double2(0.0, 0.0);
   cc.alpha = make_double2(0.0, 0.0);
   cc.ref2 = tot[2] * delta2;
-  cc.outer = (cc.status != 0 || cc.iter >= ctl->maxit || tot[2] <= ctl->tol2 * cc.bnorm2) ? 1 : 0;
+  cc.best2 = tot[2];
+  cc.last_imp = cc.iter;
+  cc.outer = (cc.status != 0 || cc.stalled || cc.iter >= ctl->maxit ||
+              tot[2] <= ctl->tol2 * cc.bnorm2) ? 1 : 0;
   cc.done = cc.outer;
 }
 
@@ -796,6 +813,7 @@ struct Batch {
   // schedule order, residual drop per refinement
   bool mixed = false;
   double delta2 = 1e-8;
+  int stall = 400;
   float2 *R32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *Z32 = nullptr, *X32 = nullptr;
   float *Ks32 = nullptr, *Ms32 = nullptr;
   std::vector<void*> owned;
@@ -974,7 +992,7 @@ int pg_batch_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t
   PG_BA(B->omu, R);
   PG_BA(B->dK, n);
   PG_BA(B->dM, n);
-  PG_BA(B->part, (size_t)3 * R * B->grid);
+  PG_BA(B->part, (size_t)3 * R * kMaxGrid);  // partials of any launch shape
   PG_BA(B->Rr, (size_t)n * R);
   PG_BA(B->P, (size_t)n * R);
   PG_BA(B->Q, (size_t)n * R);
@@ -1009,8 +1027,10 @@ int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations,
   double2* X = reinterpret_cast<double2*>(x);
   hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(64), 0, st, B->ctl, B->maxit, B->rtol * B->rtol,
                      B->R);
+  bool fp64_loop = !B->mixed;
   if (B->mixed) {
     // iterative refinement: complex64 COCG rounds between FP64 residuals
+    hipLaunchKernelGGL(k_ctl_stall, dim3(1), dim3(64), 0, st, B->ctl, B->stall);
     PG_HIP(hipMemsetAsync(X, 0, (size_t)B->n * B->R * sizeof(double2), st));
     PG_BATCH_DISPATCH(launch_refine, B, st, Bv, X, 1);
     PG_CHECK_LAUNCH();
@@ -1032,10 +1052,24 @@ int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations,
     }
     PG_HIP(hipMemcpyAsync(B->hcol, B->col, B->R * sizeof(Col), hipMemcpyDeviceToHost, st));
     PG_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < B->R; ++r) B->hcol[r].res2 = B->hcol[r].res64;
+    // columns whose complex64 rounds stagnated continue in FP64 from their iterate
+    for (int r = 0; r < B->R; ++r) {
+      const Col& c = B->hcol[r];
+      B->hcol[r].res2 = c.res64;
+      if (c.status == 0 && c.iter < B->maxit && c.res64 > B->rtol * B->rtol * c.bnorm2)
+        fp64_loop = true;
+    }
+    if (fp64_loop) {
+      hipLaunchKernelGGL(k_ctl_stall, dim3(1), dim3(64), 0, st, B->ctl, 0);
+      B->mixed = false;  // the iteration kernels below run in FP64
+      PG_BATCH_DISPATCH(launch_reinit, B, st, Bv, X);
+      PG_CHECK_LAUNCH();
+    }
   } else {
     PG_BATCH_DISPATCH(launch_init, B, st, Bv, X);
     PG_CHECK_LAUNCH();
+  }
+  if (fp64_loop) {
     // iterate until every column's recursion has converged, then check the true
     // residuals; columns above the tolerance restart from their current iterate
     for (int restart = 0;; ++restart) {
@@ -1060,6 +1094,7 @@ int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations,
       PG_BATCH_DISPATCH(launch_reinit, B, st, Bv, X);
       PG_CHECK_LAUNCH();
     }
+    B->mixed = B->R32 != nullptr;  // restore the batch's precision for the next solve
   }
   for (int r = 0; r < B->R; ++r) {
     const Col& c = B->hcol[r];
@@ -1144,7 +1179,7 @@ int pg_batch_set_entities(void* batch, const int32_t* block_ptr, int64_t nblocks
   return refresh_diag(B, st);
 }
 
-int pg_batch_set_mixed(void* batch, double inner_rtol, void* stream) {
+int pg_batch_set_mixed(void* batch, double inner_rtol, int stall_iters, void* stream) {
   if (!batch) PG_FAIL(PG_ERR_ARG, "batch is NULL");
   Batch* B = static_cast<Batch*>(batch);
   if (!B->em) PG_FAIL(PG_ERR_ARG, "mixed precision needs the entities (pg_batch_set_entities)");
@@ -1158,6 +1193,7 @@ int pg_batch_set_mixed(void* batch, double inner_rtol, void* stream) {
     return rc;
   const double d = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 1e-4;
   B->delta2 = d * d;
+  B->stall = stall_iters > 0 ? stall_iters : 400;
   B->mixed = true;
   return refresh_diag(B, as_stream(stream));
 }

@@ -34,6 +34,7 @@ def main(argv=None):
                     help="also run the batched solver on the 30^3 p=2 mesh with this many "
                          "columns for --batch-iters iterations (0 = skip)")
     ap.add_argument("--batch-iters", type=int, default=50)
+    ap.add_argument("--batch-precision", default="fp64", choices=["fp64", "mixed"])
     args = ap.parse_args(argv)
 
     import torch
@@ -90,12 +91,12 @@ def main(argv=None):
     out["copy_ms"] = ms
     out["copy_GBps_rw"] = 2 * nbytes / (ms * 1e-3) / 1e9
     if args.batch_width:
-        out.update(batch_run(args.batch_width, args.batch_iters))
+        out.update(batch_run(args.batch_width, args.batch_iters, args.batch_precision))
     print(json.dumps(out), flush=True)
     return 0
 
 
-def batch_run(width, iters):
+def batch_run(width, iters, precision="fp64"):
     """Fixed number of batched COCG iterations (the iteration limit stops them)."""
     import numpy as np
     import torch
@@ -120,10 +121,13 @@ def batch_run(width, iters):
         B[:, c, :] = b
     out = {}
     bptr = mesh.block_ptr()
-    for name, bp, od in (("row", None, None), ("entity_natural", bptr, None),
-                         ("entity_morton", bptr, mesh.entity_order())):
+    runs = (("row", None, None), ("entity_natural", bptr, None),
+            ("entity_morton", bptr, mesh.entity_order()))
+    if precision == "mixed":
+        runs = runs[1:]
+    for name, bp, od in runs:
         bs = core.BatchSolver(asm.plan, K, M, [1.0] * width, rtol=1e-30, maxit=iters, block_ptr=bp,
-                              entity_order=od)
+                              entity_order=od, precision=precision)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()

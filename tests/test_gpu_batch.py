@@ -34,7 +34,9 @@ def test_km_assembly_consistent_with_complex():
                                                   (True, "mixed")])
 @pytest.mark.parametrize("n,p,nsrc,freqs", [(6, 2, 4, [0.25, 1.0]), (4, 2, 16, [0.1, 0.25, 0.5, 1.0]),
                                             (4, 1, 32, [0.5, 1.0, 2.0, 4.0]),
-                                            (3, 3, 2, [0.5, 1.0])])
+                                            (3, 3, 2, [0.5, 1.0]), (4, 3, 8, [0.5, 1.0]),
+                                            (5, 1, 8, [0.25, 0.5, 1.0, 2.0]),
+                                            (5, 2, 16, [1.0])])
 def test_batch_matches_single_rhs(n, p, nsrc, freqs, entity_rows, precision):
     w, mesh, asm, ent_bc, dof_bc = _setup(n, p)
     K, M = core.assemble_km(asm, ent_bc)
@@ -63,3 +65,31 @@ def test_batch_matches_single_rhs(n, p, nsrc, freqs, entity_rows, precision):
         x1, inf1 = core.KrylovSolver(asm.plan, vals, rtol=1e-10).solve(cols[r])
         x1 = core.as_complex(x1)
         assert np.abs(Xh[:, r] - x1).max() <= 1e-7 * np.abs(x1).max()
+
+
+def test_mixed_batch_stagnating_columns_finish_in_fp64():
+    """Columns whose complex64 rounds are declared stagnant (after one iteration
+    without a 10 % drop) continue in FP64 from their iterate and converge."""
+    w, mesh, asm, ent_bc, dof_bc = _setup(5, 2)
+    K, M = core.assemble_km(asm, ent_bc)
+    xs = np.linspace(-600.0, 600.0, 4)
+    srcs = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
+    el = core.locate_points(mesh, srcs).cpu().numpy()
+    cols, fr = [], []
+    for f in (0.25, 1.0):
+        for s in range(4):
+            b = core.source_vector(mesh, int(el[s]), srcs[s], (1.0, 0.0, 0.0), core.omega_of(f))
+            core.apply_dirichlet(asm.plan, None, dof_bc, b)
+            cols.append(b)
+            fr.append(f)
+    B = torch.stack(cols, dim=1).contiguous()
+    ref, _ = core.BatchSolver(asm.plan, K, M, fr, rtol=1e-10).solve(B)
+    bs = core.BatchSolver(asm.plan, K, M, fr, rtol=1e-10, block_ptr=mesh.block_ptr(),
+                          entity_order=mesh.entity_order(), precision="mixed", stall_iters=1)
+    for rep in range(2):  # the batch returns to mixed precision for the next solve
+        X, info = bs.solve(B)
+        print(f"stalled mixed batch: iterations {info['iterations']}, relres "
+              f"{info['relres'].max():.2e}", flush=True)
+        assert (info["status"] == 0).all() and info["relres"].max() <= 1e-10
+        Xh, Rh = core.as_complex(X), core.as_complex(ref)
+        assert np.abs(Xh - Rh).max() <= 1e-7 * np.abs(Rh).max()
