@@ -39,7 +39,9 @@ def spmv_key(code, dmax, vt):
     """Kernel name (as the profile summaries print it) of the solve's SpMV from the
     solver's configuration code kind*1000 + G*10 + U."""
     kind, g, u = code // 1000, (code % 1000) // 10, code % 10
-    if kind:
+    if kind == 2:
+        return f"k_spmv_er2<{g},{u},1,{vt},{vt}>"
+    if kind == 1:
         d = 1 if dmax <= 1 else (2 if dmax <= 2 else 6)
         return f"k_spmv_er<{g},{u},{d},1,{vt},{vt}>"
     return f"k_spmv<{g},{u},1,{vt},{vt}>"

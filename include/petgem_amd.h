@@ -206,6 +206,7 @@ int pg_demote_z2c(int64_t n, const double *src, float *dst, void *stream);
 #define PG_SPMV_AUTO 0
 #define PG_SPMV_ROW 1
 #define PG_SPMV_ENTITY 2
+#define PG_SPMV_PAIRS 3  /* entity rows with column pairs: every entity two DOFs (p = 2) */
 
 typedef struct pg_solve_opts {
   int32_t ksp;        /* PG_KSP_* */

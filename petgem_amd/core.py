@@ -418,7 +418,7 @@ _PC = {"none": nat.PG_PC_NONE, "jacobi": nat.PG_PC_JACOBI, "bjacobi": nat.PG_PC_
 _PREC = {"fp64": nat.PG_PREC_FP64, "mixed": nat.PG_PREC_MIXED}
 
 
-_SPMV = {"auto": 0, "row": 1, "entity": 2}
+_SPMV = {"auto": 0, "row": 1, "entity": 2, "entity_pairs": 3}
 
 
 class KrylovSolver:

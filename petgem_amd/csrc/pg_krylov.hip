@@ -320,6 +320,106 @@ __global__ void __launch_bounds__(kBS) k_spmv_er(const Scal* __restrict__ s, int
   if (MODE != 0) write_partials<2>(red, part);
 }
 
+// Entity-row SpMV for meshes whose entities all carry two DOFs (p = 2): every
+// entity has d = 2 rows and its columns come in aligned pairs (c, c+1) of one
+// column entity, so each lane loads one column index, one 2-value slice of x
+// and one 2-value slice of each row per column pair (16-byte loads for
+// complex64).  cols2 holds the first column of each pair (cbase / 2 onwards).
+template <typename T>
+struct PairOf;
+template <>
+struct PairOf<float2> {
+  using type = float4;
+};
+template <>
+struct PairOf<double2> {
+  using type = double4;
+};
+
+template <int G, int U, int MODE, typename VT, typename XT>
+__global__ void __launch_bounds__(kBS) k_spmv_er2(const Scal* __restrict__ s, int64_t nent,
+                                                  const int4* __restrict__ emeta,
+                                                  const int32_t* __restrict__ cols2,
+                                                  const VT* __restrict__ vals,
+                                                  const XT* __restrict__ x, XT* __restrict__ y,
+                                                  const double2* __restrict__ b, double* part) {
+  using VP = typename PairOf<VT>::type;
+  using XP = typename PairOf<XT>::type;
+  if (MODE != 0 && s->done) return;
+  constexpr int EPB = kBS / G;
+  const Sweep sw = xcd_sweep(nent, EPB);
+  const int lane = threadIdx.x % G;
+  double red[2] = {0.0, 0.0};
+  int64_t ent = sw.first + threadIdx.x / G;
+  int4 mn = make_int4(0, 0, 0, 0);
+  if (ent < sw.end) mn = emeta[ent];
+  for (; ent < sw.end; ent += sw.stride) {
+    const int r0 = mn.x, L = mn.y >> 8, c2 = mn.z >> 1, vbase = mn.w;
+    const int L2 = L >> 1;
+    const int64_t nx = ent + sw.stride;
+    if (nx < sw.end) mn = emeta[nx];
+    const VP* v0 = reinterpret_cast<const VP*>(vals + vbase);
+    const VP* v1 = reinterpret_cast<const VP*>(vals + vbase + L);
+    XT acc0 = czero<XT>(), acc1 = czero<XT>();
+    for (int tb = lane; tb < L2; tb += G * U) {
+      int c[U];
+      XP xv[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int t = tb + j * G;
+        c[j] = t < L2 ? cols2[c2 + t] : -1;
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        xv[j] = c[j] >= 0 ? *reinterpret_cast<const XP*>(x + c[j]) : XP{0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const int t = tb + j * G;
+        const VP a = t < L2 ? v0[t] : VP{0, 0, 0, 0};
+        const VP e = t < L2 ? v1[t] : VP{0, 0, 0, 0};
+        acc0 = cfma(XT{a.x, a.y}, XT{xv[j].x, xv[j].y}, acc0);
+        acc0 = cfma(XT{a.z, a.w}, XT{xv[j].z, xv[j].w}, acc0);
+        acc1 = cfma(XT{e.x, e.y}, XT{xv[j].x, xv[j].y}, acc1);
+        acc1 = cfma(XT{e.z, e.w}, XT{xv[j].z, xv[j].w}, acc1);
+      }
+    }
+    acc0.x = group_sum<G>(acc0.x);
+    acc0.y = group_sum<G>(acc0.y);
+    acc1.x = group_sum<G>(acc1.x);
+    acc1.y = group_sum<G>(acc1.y);
+    if (lane == 0) {
+      spmv_row_out<MODE, XT>(r0, acc0, x, y, b, red);
+      spmv_row_out<MODE, XT>(r0 + 1, acc1, x, y, b, red);
+    }
+  }
+  if (MODE != 0) write_partials<2>(red, part);
+}
+
+// column pairs: all entities two rows, every column list made of aligned pairs
+__global__ void k_check_pairs(int64_t nent, const int4* __restrict__ em,
+                              const int32_t* __restrict__ cols_s, int32_t* bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nent) return;
+  const int4 m = em[i];
+  const int d = m.y & 0xff, L = m.y >> 8;
+  if (d != 2 || (L & 1) || (m.z & 1) || (m.w & 1)) {
+    bad[0] = 1;
+    return;
+  }
+  for (int j = 0; j < L; j += 2) {
+    const int c = cols_s[m.z + j];
+    if ((c & 1) || cols_s[m.z + j + 1] != c + 1) {
+      bad[0] = 1;
+      return;
+    }
+  }
+}
+__global__ void k_make_pairs(int64_t n2, const int32_t* __restrict__ cols_s,
+                             int32_t* __restrict__ cols2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2) cols2[i] = cols_s[2 * i];
+}
+
 // scheduled copies: entity i's column list (L entries from CSR slot src[i]) and
 // its d x L value block (contiguous in the CSR) into the schedule-order arrays
 __global__ void __launch_bounds__(kBS) k_sched_cols(int64_t nent, const int4* __restrict__ em,
@@ -849,7 +949,7 @@ static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) 
 // SpMV configurations and dispatch
 // ---------------------------------------------------------------------------
 struct SpmvCfg {
-  int kind = 0;  // 0 = row, 1 = entity-row
+  int kind = 0;  // 0 = row, 1 = entity-row, 2 = entity-row with column pairs
   int G = 16, U = 2, D = 1;
   int grid = 1;
 };
@@ -900,11 +1000,30 @@ static void spmv_er_d(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t n
 }
 
 template <int MODE, typename VT, typename XT>
+static void spmv_er2(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t nent, const int4* em,
+                     const int32_t* ci, const VT* v, const XT* x, XT* y, const double2* b,
+                     double* part) {
+  const dim3 g(c.grid), bl(kBS);
+#define PG_ER2(GG, UU)                                                                        \
+  if (c.G == GG && c.U == UU) {                                                               \
+    hipLaunchKernelGGL((k_spmv_er2<GG, UU, MODE, VT, XT>), g, bl, 0, st, s, nent, em, ci, v, x, \
+                       y, b, part);                                                           \
+    return;                                                                                   \
+  }
+  PG_GU_LIST(PG_ER2)
+#undef PG_ER2
+  hipLaunchKernelGGL((k_spmv_er2<16, 2, MODE, VT, XT>), g, bl, 0, st, s, nent, em, ci, v, x, y, b,
+                     part);
+}
+
+template <int MODE, typename VT, typename XT>
 static void spmv_launch(const SpmvCfg& c, hipStream_t st, const Scal* s, int64_t n, int64_t nent,
                         const int4* em, const int32_t* rp, const int32_t* ci, const VT* v,
                         const XT* x, XT* y, const double2* b, double* part) {
   if (c.kind == 0) {
     spmv_row<MODE, VT, XT>(c, st, s, n, rp, ci, v, x, y, b, part);
+  } else if (c.kind == 2) {
+    spmv_er2<MODE, VT, XT>(c, st, s, nent, em, ci, v, x, y, b, part);
   } else if (c.D == 1) {
     spmv_er_d<MODE, VT, XT, 1>(c, st, s, nent, em, ci, v, x, y, b, part);
   } else if (c.D == 2) {
@@ -943,6 +1062,8 @@ struct Solver {
   int4* emeta = nullptr;
   int32_t* esrc = nullptr;
   int32_t* cols_s = nullptr;
+  int32_t* cols2 = nullptr;  // first column of each pair (every entity two DOFs)
+  bool pairs_ok = false;
   double2* vals_s64 = nullptr;
   double2* binv = nullptr;
   float2* binv32 = nullptr;
@@ -995,9 +1116,10 @@ static inline const float2* sched_values(Solver* S, const float2*) { return S->v
 template <int MODE, typename VT, typename XT>
 static void spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x, XT* y,
                  const double2* b) {
-  if (c.kind == 1)
-    spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->emeta, S->rowptr, S->cols_s,
-                              sched_values(S, v), x, y, b, S->part);
+  if (c.kind >= 1)
+    spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->emeta, S->rowptr,
+                              c.kind == 2 ? S->cols2 : S->cols_s, sched_values(S, v), x, y, b,
+                              S->part);
   else
     spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, nullptr, S->rowptr, S->cols, v, x, y,
                               b, S->part);
@@ -1429,14 +1551,30 @@ static int setup_entities(Solver* S, const int32_t* bp, const int32_t* order, hi
   hipLaunchKernelGGL(k_sched_cols, dim3(nblk(32 * nb, kBS)), dim3(kBS), 0, st, nb, S->emeta,
                      S->esrc, S->cols, S->cols_s);
   PG_CHECK_LAUNCH();
+  // column pairs (every entity two DOFs, p = 2)
+  PG_HIP(hipMemsetAsync(S->bad, 0, sizeof(int32_t), st));
+  hipLaunchKernelGGL(k_check_pairs, dim3(nblk(nb, kBS)), dim3(kBS), 0, st, nb, S->emeta, S->cols_s,
+                     S->bad);
+  PG_CHECK_LAUNCH();
+  PG_HIP(hipMemcpyAsync(&S->h->iter, S->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  if (S->h->iter == 0) {
+    if ((rc = S->alloc(&S->cols2, cb / 2)) != PG_OK) return rc;
+    hipLaunchKernelGGL(k_make_pairs, dim3(nblk(cb / 2, kBS)), dim3(kBS), 0, st, cb / 2, S->cols_s,
+                       S->cols2);
+    PG_CHECK_LAUNCH();
+    S->pairs_ok = true;
+  }
   return PG_OK;
 }
 
 // pick the fastest SpMV configuration for this matrix (row / entity-row, G, U)
 static int autotune(Solver* S, hipStream_t st) {
   const bool mixed = S->o.precision == PG_PREC_MIXED;
-  if (S->o.spmv == PG_SPMV_ENTITY && !S->er_ok)
+  if ((S->o.spmv == PG_SPMV_ENTITY || S->o.spmv == PG_SPMV_PAIRS) && !S->er_ok)
     PG_FAIL(PG_ERR_ARG, "entity-row SpMV needs block_ptr ranges whose rows share one column list");
+  if (S->o.spmv == PG_SPMV_PAIRS && !S->pairs_ok)
+    PG_FAIL(PG_ERR_ARG, "the column-pair SpMV needs every entity to carry two DOFs (p = 2)");
   std::vector<SpmvCfg> cands;
   const int gu[6][2] = {{8, 4}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {64, 1}};
   for (const auto& g : gu) {
@@ -1445,13 +1583,17 @@ static int autotune(Solver* S, hipStream_t st) {
     c.G = g[0];
     c.U = g[1];
     c.grid = grid_for(S->n, c.G);
-    if (S->o.spmv != PG_SPMV_ENTITY) cands.push_back(c);
+    if (S->o.spmv == PG_SPMV_AUTO || S->o.spmv == PG_SPMV_ROW) cands.push_back(c);
     if (S->er_ok && S->o.spmv != PG_SPMV_ROW) {
       SpmvCfg e = c;
       e.kind = 1;
       e.D = S->maxblk <= 1 ? 1 : (S->maxblk <= 2 ? 2 : 6);
       e.grid = grid_for(S->nblocks, e.G);
-      cands.push_back(e);
+      if (S->o.spmv != PG_SPMV_PAIRS) cands.push_back(e);
+      if (S->pairs_ok && S->o.spmv != PG_SPMV_ENTITY) {
+        e.kind = 2;
+        cands.push_back(e);
+      }
     }
   }
   // timing inputs: x = 1 (finite), y = FP64 scratch q (allocated in every mode);
@@ -1463,7 +1605,7 @@ static int autotune(Solver* S, hipStream_t st) {
     // the complex64 copy has one layout: schedule order when entity rows exist
     std::vector<SpmvCfg> c32;
     for (const SpmvCfg& c : cands)
-      if ((c.kind == 1) == S->er_ok) c32.push_back(c);
+      if ((c.kind >= 1) == S->er_ok) c32.push_back(c);
     if (c32.empty()) PG_FAIL(PG_ERR_ARG, "no SpMV kernel for the complex64 layout");
     hipLaunchKernelGGL(k_ones32, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->p32);
     S->c32 = pick_spmv<float2, float2>(S, c32, st, S->vals32, S->p32, S->q32);
@@ -1529,7 +1671,7 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   if (opts->pc != PG_PC_NONE && opts->pc != PG_PC_JACOBI && opts->pc != PG_PC_BJACOBI)
     PG_FAIL(PG_ERR_UNSUPPORTED, "pc type %d not supported", opts->pc);
   if (!(opts->rtol > 0.0)) PG_FAIL(PG_ERR_ARG, "rtol must be positive");
-  if (opts->spmv < PG_SPMV_AUTO || opts->spmv > PG_SPMV_ENTITY)
+  if (opts->spmv < PG_SPMV_AUTO || opts->spmv > PG_SPMV_PAIRS)
     PG_FAIL(PG_ERR_ARG, "spmv kind %d not supported", opts->spmv);
   std::vector<int32_t> hboff;
   int maxblk = 0;

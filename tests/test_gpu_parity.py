@@ -324,3 +324,18 @@ def test_mixed_stagnation_hands_over_to_fp64():
     assert info["fp64_fallback"] == 1 and info["status"] == 0 and info["relres"] <= 1e-8
     xr, xh = core.as_complex(ref), core.as_complex(x)
     assert np.abs(xh - xr).max() <= 1e-6 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_column_pair_spmv_matches_entity_rows(precision):
+    """p = 2 (two DOFs per entity): the column-pair entity-row SpMV gives the
+    solution of the scalar entity-row SpMV and of SuperLU."""
+    w = Workload(6, 2, noise=0.1, seed=11, n_rx=21)
+    g2 = _forward_gpu(w, precision=precision, pc="bjacobi", spmv="entity_pairs")
+    g1 = _forward_gpu(w, precision=precision, pc="bjacobi", spmv="entity")
+    r = _oracle_forward(w)
+    for g in (g1, g2):
+        assert g["info"]["status"] == 0 and g["info"]["relres"] <= 1e-8
+    ferr = np.abs(g2["fields"] - r["fields"]).max() / np.abs(r["fields"]).max()
+    assert ferr <= 1e-6
+    assert np.abs(g2["x"] - g1["x"]).max() <= 1e-6 * np.abs(g1["x"]).max()
