@@ -301,7 +301,7 @@ def main():
                          "adjoint (grad/s)")
     ap.add_argument("--batch-width", type=int, default=0,
                     help="columns per batched solve (0: one frequency per batch, at most 64)")
-    ap.add_argument("--batch-precision", default="mixed", choices=["fp64", "mixed"],
+    ap.add_argument("--batch-precision", default="fp64", choices=["fp64", "mixed"],
                     help="batched and adjoint legs: FP64, or complex64 under FP64 refinement")
     ap.add_argument("--batch-positions", type=int, default=16,
                     help="HED positions of the batched leg (of 64; x 4 frequencies)")
