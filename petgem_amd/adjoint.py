@@ -110,8 +110,12 @@ class InversionProblem:
         for fi, f in enumerate(self.freqs):
             self.asm.assemble(core.omega_of(f), ent_bc=self.ent_bc, out=self.values[fi])
             if self.solvers[fi] is None:
+                # COCG + entity-block Jacobi with the entity-row SpMV in Morton order
                 self.solvers[fi] = core.KrylovSolver(self.asm.plan, self.values[fi],
-                                                     precision=self.precision, rtol=self.rtol)
+                                                     precision=self.precision, rtol=self.rtol,
+                                                     pc="bjacobi",
+                                                     block_ptr=self.mesh.block_ptr(),
+                                                     entity_order=self.mesh.entity_order())
             else:
                 self.solvers[fi].refresh()
 
