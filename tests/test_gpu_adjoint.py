@@ -71,3 +71,45 @@ def test_batched_gradient_matches_per_rhs(max_batch, precision):
           flush=True)
     assert abs(phi_b - phi_s) <= 1e-9 * phi_s
     assert err <= 1e-8
+
+
+def test_adjoint_fd_twenty_elements_c5_setup():
+    """The C5 sources and frequencies (8 HED x 2) on an 8^3 mesh (too coarse to
+    resolve the 200 m resistive block, so the observed model is the layered one
+    with seeded log-normal perturbations): the device gradient against the FP64
+    CPU oracle gradient and central differences (h = 1e-4) on 20 seeded random
+    elements."""
+    w0 = Workload(8, 2, block=False)
+    T = w0.topo
+    xs = np.linspace(-1750.0, 1750.0, 8)
+    src = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
+    freqs = [0.25, 1.0]
+    rx = receiver_line(21, z=-1000.0)
+    m0 = np.log(w0.sigma[:, 0])
+    true = m0 + 0.3 * np.random.default_rng(5).standard_normal(T.n_elems)
+    prob = InversionProblem(T, src, freqs, rx, rtol=1e-12, batched=True)
+    prob.observed = prob.forward(true)
+    phi, g = prob.misfit_and_gradient(m0)
+    g = g.cpu().numpy()
+    ora = OracleInversion(T, src, freqs, rx)
+    ora.observed = prob.observed
+    phi_ref, g_ref = ora.misfit_and_gradient(m0)
+    err = np.abs(g - g_ref).max() / np.abs(g_ref).max()
+    print(f"C5 set-up on 8^3: phi {phi:.9e} (oracle {phi_ref:.9e}), grad rel err {err:.2e}",
+          flush=True)
+    # the 1/|E_obs|^2 weights amplify the solver tolerance at weak-field receivers
+    assert abs(phi - phi_ref) <= 1e-6 * phi_ref and err <= 1e-6
+    # central differences resolve the misfit to ~1e-8 relative (the solver
+    # tolerance, amplified by the weights), i.e. gradients down to ~1e-4 of the
+    # largest: the 20 seeded elements are drawn from those above 1 % of it
+    h = 1e-4
+    worst = 0.0
+    cand = np.flatnonzero(np.abs(g) >= 1e-2 * np.abs(g).max())
+    for e in np.random.default_rng(20).choice(cand, 20, replace=False):
+        mp, mm = m0.copy(), m0.copy()
+        mp[e] += h
+        mm[e] -= h
+        fd = (prob.misfit(prob.forward(mp)) - prob.misfit(prob.forward(mm))) / (2 * h)
+        worst = max(worst, abs(fd - g[e]) / abs(g[e]))
+        assert abs(fd - g[e]) <= 1e-3 * abs(g[e]) + 2e-4 * np.abs(g).max(), (e, fd, g[e])
+    print(f"central differences on 20 elements: worst |fd - g| / |g| = {worst:.2e}", flush=True)
