@@ -57,7 +57,10 @@ size_t pg_element_workspace_bytes(int64_t n_elems);
  * edges_nodes (E,6,2) [edgesNodes.dat], faces_edges (E,4,3)
  * [facesEdges.dat], sigma (E,2) [conductivityModel.dat].
  * Replaces computeJacobian (petgem/hvfem.py:101-119) and
- * computeElementOrientation (petgem/hvfem.py:122-220). */
+ * computeElementOrientation (petgem/hvfem.py:122-220).  The Jacobian
+ * determinant is signed, as in the reference; a degenerate element (zero or
+ * non-finite determinant) or a negative / non-finite conductivity returns
+ * PG_ERR_ARG (the call waits for the stream to check). */
 int pg_element_geometry(int p, int64_t n_elems, const double *coords,
                         const int32_t *elems_nodes, const int32_t *elems_edges,
                         const int32_t *edges_nodes, const int32_t *faces_edges,

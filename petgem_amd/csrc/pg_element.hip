@@ -84,11 +84,13 @@ struct Tables {
 // -------------------------------------------------------------------------
 // geometry + orientation (one thread per element)
 // -------------------------------------------------------------------------
-// workspace layout: coef[E][12] (det*G' then H'/det), then orient[E] (int32)
+// workspace layout: coef[E][12] (det*G' then H'/det), then orient[E] (int32),
+// then one int32 flag (invalid element or conductivity)
 __host__ __device__ inline double* ws_coef(void* ws) { return reinterpret_cast<double*>(ws); }
 __host__ __device__ inline int32_t* ws_orient(void* ws, int64_t E) {
   return reinterpret_cast<int32_t*>(reinterpret_cast<double*>(ws) + 12 * E);
 }
+__host__ __device__ inline int32_t* ws_flag(void* ws, int64_t E) { return ws_orient(ws, E) + E; }
 
 __device__ inline int element_orientation(const int32_t* en, const int32_t* ee, const int32_t* edn,
                                           const int32_t* fe, int* eo, int* fo) {
@@ -144,12 +146,17 @@ __global__ void k_geometry(int64_t E, const double* __restrict__ coords,
                            const int32_t* __restrict__ en, const int32_t* __restrict__ ee,
                            const int32_t* __restrict__ edn, const int32_t* __restrict__ fe,
                            const double* __restrict__ sigma, double* __restrict__ coef,
-                           int32_t* __restrict__ orient) {
+                           int32_t* __restrict__ orient, int32_t* __restrict__ flag) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   double J[3][3], iJ[3][3], det;
   jacobian(coords + 12 * e, J, iJ, &det);
   const double s[3] = {sigma[2 * e], sigma[2 * e], sigma[2 * e + 1]};
+  // the determinant is signed, as in the reference (hvfem.py:265); a degenerate
+  // element or a negative / non-finite conductivity is an input error
+  if (!(det != 0.0) || !isfinite(det) || !(s[0] >= 0.0) || !(s[2] >= 0.0) || !isfinite(s[0]) ||
+      !isfinite(s[2]))
+    flag[0] = 1;  // every writer stores the same value
   const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
   double* out = coef + 12 * e;
   const double idet = 1.0 / det;
@@ -634,7 +641,7 @@ int pg_tables_destroy(void* tables) {
 }
 
 size_t pg_element_workspace_bytes(int64_t n_elems) {
-  return (size_t)n_elems * (12 * sizeof(double) + sizeof(int32_t));
+  return (size_t)n_elems * (12 * sizeof(double) + sizeof(int32_t)) + sizeof(int32_t);
 }
 
 int pg_element_geometry(int p, int64_t E, const double* coords, const int32_t* en,
@@ -643,9 +650,18 @@ int pg_element_geometry(int p, int64_t E, const double* coords, const int32_t* e
   if (p < 1 || p > kMaxP) PG_FAIL(PG_ERR_UNSUPPORTED, "order p=%d not supported", p);
   if (E <= 0) PG_FAIL(PG_ERR_ARG, "n_elems must be positive");
   if (!coords || !en || !ee || !edn || !fe || !sigma || !ws) PG_FAIL(PG_ERR_ARG, "NULL input");
-  hipLaunchKernelGGL(k_geometry, dim3(nblocks(E, 256)), dim3(256), 0, as_stream(stream), E,
-                     coords, en, ee, edn, fe, sigma, ws_coef(ws), ws_orient(ws, E));
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, 1, ws_flag(ws, E), 0);
+  hipLaunchKernelGGL(k_geometry, dim3(nblocks(E, 256)), dim3(256), 0, s, E, coords, en, ee, edn,
+                     fe, sigma, ws_coef(ws), ws_orient(ws, E), ws_flag(ws, E));
   PG_CHECK_LAUNCH();
+  int32_t flag = 0;
+  PG_HIP(hipMemcpyAsync(&flag, ws_flag(ws, E), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  PG_HIP(hipStreamSynchronize(s));
+  if (flag)
+    PG_FAIL(PG_ERR_ARG,
+            "invalid element input: a degenerate element (zero or non-finite Jacobian "
+            "determinant) or a negative / non-finite conductivity");
   return PG_OK;
 }
 
