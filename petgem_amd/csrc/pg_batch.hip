@@ -24,7 +24,6 @@ namespace batch {
 
 constexpr int kBS = 256;
 constexpr int kMaxGrid = 1024;
-constexpr int kMaxR = 1024;
 
 struct Col {  // per-column scalars
   double2 rho, mu, alpha, beta;
@@ -817,10 +816,10 @@ struct Batch {
   float2 *R32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *Z32 = nullptr, *X32 = nullptr;
   float *Ks32 = nullptr, *Ms32 = nullptr;
   std::vector<void*> owned;
-  ~Batch() {
-    for (void* v : owned) hipFree(v);
-    if (hctl) hipHostFree(hctl);
-    if (hcol) hipHostFree(hcol);
+  ~Batch() {  // release errors cannot be reported from here
+    for (void* v : owned) (void)hipFree(v);
+    if (hctl) (void)hipHostFree(hctl);
+    if (hcol) (void)hipHostFree(hcol);
   }
   template <typename T>
   int alloc(T** p, size_t count) {
@@ -1008,7 +1007,10 @@ int pg_batch_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t
   }
   std::vector<double> om(R);
   for (int r = 0; r < R; ++r) om[r] = omega[r] * 4.0 * 3.14159265358979323846 * 1e-7;
-  hipMemcpy(B->omu, om.data(), R * sizeof(double), hipMemcpyHostToDevice);
+  if (hipMemcpy(B->omu, om.data(), R * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    delete B;
+    PG_FAIL(PG_ERR_HIP, "frequency upload failed");
+  }
   const int rc_diag = refresh_diag(B, st);
   if (rc_diag != PG_OK) {
     delete B;

@@ -616,15 +616,15 @@ int pg_tables_create(int p, void** tables) {
   Tables* t = new Tables();
   t->p = p;
   t->V = n_variants(p);
-  hipGetDevice(&t->device);
-  hipError_t err = hipMalloc(&t->d_T, T.size() * sizeof(double));
+  hipError_t err = hipGetDevice(&t->device);
+  if (err == hipSuccess) err = hipMalloc(&t->d_T, T.size() * sizeof(double));
   if (err != hipSuccess) {
     delete t;
     PG_FAIL(PG_ERR_HIP, "hipMalloc(tables): %s", hipGetErrorString(err));
   }
   err = hipMemcpy(t->d_T, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice);
   if (err != hipSuccess) {
-    hipFree(t->d_T);
+    (void)hipFree(t->d_T);
     delete t;
     PG_FAIL(PG_ERR_HIP, "hipMemcpy(tables): %s", hipGetErrorString(err));
   }
@@ -635,7 +635,7 @@ int pg_tables_create(int p, void** tables) {
 int pg_tables_destroy(void* tables) {
   if (!tables) return PG_OK;
   Tables* t = static_cast<Tables*>(tables);
-  hipFree(t->d_T);
+  (void)hipFree(t->d_T);
   delete t;
   return PG_OK;
 }

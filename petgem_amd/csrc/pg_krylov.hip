@@ -1087,12 +1087,12 @@ struct Solver {
     *ptr = static_cast<T*>(v);
     return PG_OK;
   }
-  ~Solver() {
+  ~Solver() {  // release errors cannot be reported from here
     for (hipEvent_t e : ev)
-      if (e) hipEventDestroy(e);
-    for (void* v : owned) hipFree(v);
-    if (h) hipHostFree(h);
-    if (hhost) hipHostFree(hhost);
+      if (e) (void)hipEventDestroy(e);
+    for (void* v : owned) (void)hipFree(v);
+    if (h) (void)hipHostFree(h);
+    if (hhost) (void)hipHostFree(hhost);
   }
 };
 
@@ -1137,18 +1137,19 @@ static int true_residual(Solver* S, hipStream_t st, const double2* b, const doub
 template <typename VT, typename XT>
 static float time_spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x,
                        XT* y, int reps) {
-  hipEvent_t a, b;
-  hipEventCreate(&a);
-  hipEventCreate(&b);
-  spmv<1, VT, XT>(S, c, st, v, x, y, nullptr);
-  hipEventRecord(a, st);
-  for (int i = 0; i < reps; ++i) spmv<1, VT, XT>(S, c, st, v, x, y, nullptr);
-  hipEventRecord(b, st);
-  hipEventSynchronize(b);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, a, b);
-  hipEventDestroy(a);
-  hipEventDestroy(b);
+  hipEvent_t a = nullptr, b = nullptr;
+  float ms = 1e30f;  // an untimed candidate is never picked
+  if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+    spmv<1, VT, XT>(S, c, st, v, x, y, nullptr);
+    if (hipEventRecord(a, st) == hipSuccess) {
+      for (int i = 0; i < reps; ++i) spmv<1, VT, XT>(S, c, st, v, x, y, nullptr);
+      if (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+          hipEventElapsedTime(&ms, a, b) != hipSuccess)
+        ms = 1e30f * reps;
+    }
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
   return ms / reps;
 }
 
@@ -1196,9 +1197,10 @@ static void launch_iteration(Solver* S, hipStream_t st, const SpmvCfg& c, const 
                              XT* r, XT* p, XT* q, const XT* dinv, const XT* binv, XT* z,
                              bool sample) {
   hipLaunchKernelGGL(k_pupdate<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, z, p);
-  if (sample) hipEventRecord(S->ev[0], st);
+  // a failed record only loses this timing sample (the elapsed-time query fails)
+  if (sample) (void)hipEventRecord(S->ev[0], st);
   spmv<1, VT, XT>(S, c, st, vals, p, q, nullptr);
-  if (sample) hipEventRecord(S->ev[1], st);
+  if (sample) (void)hipEventRecord(S->ev[1], st);
   hipLaunchKernelGGL(k_fin_mu, dim3(1), dim3(kBS), 0, st, S->s, S->part, c.grid);
   int np;
   if (S->o.pc == PG_PC_BJACOBI) {
@@ -1819,27 +1821,27 @@ int pg_solver_profile(void* solver, int reps, double* ms64, double* ms32, int32_
   *ms32 = 0.0;
   if (mixed) {
     hipLaunchKernelGGL(k_ones32, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->p32);
-    hipEventRecord(a, st);
+    PG_HIP(hipEventRecord(a, st));
     for (int i = 0; i < reps; ++i)
       spmv<1, float2, float2>(S, S->c32, st, S->vals32, S->p32, S->q32, nullptr);
-    hipEventRecord(b, st);
-    hipEventSynchronize(b);
-    hipEventElapsedTime(&t, a, b);
+    PG_HIP(hipEventRecord(b, st));
+    PG_HIP(hipEventSynchronize(b));
+    PG_HIP(hipEventElapsedTime(&t, a, b));
     *ms32 = t / reps;
   }
   hipLaunchKernelGGL(k_ones, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->r64);
-  hipEventRecord(a, st);
+  PG_HIP(hipEventRecord(a, st));
   for (int i = 0; i < reps; ++i)
     spmv<1, double2, double2>(S, S->c64, st, S->vals, S->r64, S->q, nullptr);
-  hipEventRecord(b, st);
-  hipEventSynchronize(b);
-  hipEventElapsedTime(&t, a, b);
+  PG_HIP(hipEventRecord(b, st));
+  PG_HIP(hipEventSynchronize(b));
+  PG_HIP(hipEventElapsedTime(&t, a, b));
   *ms64 = t / reps;
   // configuration code: kind*1000 + G*10 + U
   *g64 = S->c64.kind * 1000 + S->c64.G * 10 + S->c64.U;
   *g32 = S->c32.kind * 1000 + S->c32.G * 10 + S->c32.U;
-  hipEventDestroy(a);
-  hipEventDestroy(b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
   PG_CHECK_LAUNCH();
   return PG_OK;
 }

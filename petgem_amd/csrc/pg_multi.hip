@@ -399,13 +399,13 @@ struct Multi {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<void*> owned;
-  ~Multi() {
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
-    if (st) hipStreamDestroy(st);
-    for (void* v : owned) hipFree(v);
-    if (hctl) hipHostFree(hctl);
-    if (hseg) hipHostFree(hseg);
+  ~Multi() {  // release errors cannot be reported from here
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (st) (void)hipStreamDestroy(st);
+    for (void* v : owned) (void)hipFree(v);
+    if (hctl) (void)hipHostFree(hctl);
+    if (hseg) (void)hipHostFree(hseg);
   }
   template <typename T>
   int alloc(T** ptr, size_t count) {
