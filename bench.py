@@ -125,7 +125,7 @@ def batched_leg(args, log):
     core.assemble_km(asm, ent_bc, out=(K, M))
     bptr, order = mesh.block_ptr(), mesh.entity_order()
     solvers = [core.BatchSolver(asm.plan, K, M, [f for _, f in ch], rtol=args.rtol,
-                                block_ptr=bptr, entity_order=order,
+                                block_ptr=bptr, entity_order=order, pc="bjacobi",
                                 precision=args.batch_precision) for ch in chunks]
     B = torch.zeros((T.ndof, width, 2), dtype=torch.float64, device=dev)
     times, iters = [], []
@@ -180,8 +180,9 @@ def batched_leg(args, log):
             "sequential_mixed_bjacobi_rhs_per_s": round(len(cols) / seq_t, 3),
             "workload": (f"c4: p=2, 30^3 Kuhn x 6 = {T.n_elems} tets, {T.ndof} DOFs, "
                          f"{asm.plan.nnz} nnz; {len(xs)} of the 64 HED x-positions (100 m apart "
-                         f"at z=-950 m) x {freqs} Hz, rtol {args.rtol}, COCG + Jacobi per "
-                         f"column ({args.batch_precision}), K/M assembled once per batch")}
+                         f"at z=-950 m) x {freqs} Hz, rtol {args.rtol}, COCG + entity-block "
+                         f"Jacobi per column ({args.batch_precision}), K/M assembled once per "
+                         "batch")}
 
 
 def multimesh_leg(args, log):
@@ -291,7 +292,8 @@ def adjoint_leg(args, log):
                          "2 frequencies (0.25, 1 Hz), 101 receivers, rtol "
                          f"{args.rtol}; forward + adjoint solves + element contraction; "
                          f"batched = two 16-column batched solves ({args.batch_precision}, "
-                         "Jacobi), per_rhs = 32 single solves (mixed, entity-block Jacobi)")}
+                         "entity-block Jacobi), per_rhs = 32 single solves (mixed, entity-block "
+                         "Jacobi)")}
 
 
 def main():

@@ -291,8 +291,12 @@ int pg_batch_set_entities(void *batch, const int32_t *block_ptr, int64_t nblocks
  * stall_iters iterations (0 = 400) without a 10 % drop of its best residual
  * finishes in FP64 from its iterate.  Keeps complex64 vectors and float K, M. */
 int pg_batch_set_mixed(void *batch, double inner_rtol, int stall_iters, void *stream);
-/* Re-derive the Jacobi diagonals (and the schedule-order copies of K and M)
- * after the caller has reassembled K and M in place. */
+/* Optional, after pg_batch_set_entities: entity-block Jacobi instead of point
+ * Jacobi.  For every distinct frequency of the batch and every entity, the
+ * inverse of the entity's block of K - i omega mu0 M (at most 6 x 6). */
+int pg_batch_set_block_jacobi(void *batch, void *stream);
+/* Re-derive the Jacobi diagonals (block inverses, schedule-order copies of K
+ * and M) after the caller has reassembled K and M in place. */
 int pg_batch_refresh(void *batch, void *stream);
 int pg_batch_destroy(void *batch);
 

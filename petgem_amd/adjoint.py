@@ -103,7 +103,7 @@ class InversionProblem:
                                                         rtol=self.rtol,
                                                         block_ptr=self.mesh.block_ptr(),
                                                         entity_order=self.mesh.entity_order(),
-                                                        precision=self.precision)
+                                                        precision=self.precision, pc="bjacobi")
                 else:
                     self.batches[ci].refresh()
             return

@@ -259,7 +259,7 @@ class BatchSolver:
 
     def __init__(self, plan, K, M, frequencies, rtol=1e-8, maxit=100000, block_ptr=None,
                  entity_order=None, precision="fp64", inner_rtol=1e-4, stall_iters=0,
-                 stream=None):
+                 pc="jacobi", stream=None):
         self.plan, self.K, self.M = plan, K, M
         self.omega = np.array([omega_of(f) for f in frequencies], dtype=np.float64)
         self.nrhs = self.omega.size
@@ -277,6 +277,12 @@ class BatchSolver:
             nat.check(nat.lib().pg_batch_set_entities(self.handle, bp.ctypes.data, bp.size - 1,
                                                       None if od is None else od.ctypes.data,
                                                       nat.stream_handle(stream)))
+        if pc == "bjacobi":
+            if block_ptr is None:
+                raise ValueError("block Jacobi needs block_ptr (entity rows)")
+            nat.check(nat.lib().pg_batch_set_block_jacobi(self.handle, nat.stream_handle(stream)))
+        elif pc != "jacobi":
+            raise ValueError(f"pc must be jacobi or bjacobi, not {pc}")
         if precision == "mixed":
             if block_ptr is None:
                 raise ValueError("the mixed-precision batch needs block_ptr (entity rows)")

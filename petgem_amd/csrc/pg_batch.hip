@@ -750,6 +750,210 @@ __global__ void k_accum(int64_t n, double2* __restrict__ X, const float2* __rest
   X[i] = make_double2(X[i].x + (double)v.x, X[i].y + (double)v.y);
 }
 
+// ---------------------------------------------------------------------------
+// Entity-block Jacobi for the batch: per mesh entity (block_ptr range of d <= 6
+// DOFs) and per distinct frequency f, the inverse of A_f restricted to the
+// entity's DOFs, A_f = K - i omega_f mu0 M; column r uses frequency fidx[r].
+// ---------------------------------------------------------------------------
+constexpr int kBlk = 6;
+
+// one thread per (entity, frequency): gather the d x d block from the CSR K, M
+// and invert it (Gauss-Jordan, partial pivoting); row-major at
+// binv[f * nb2 + boff[e]] (boff: prefix sums of d^2, nb2 = their total)
+__global__ void k_batch_binv(int64_t nent, int nf, int64_t nb2, const int32_t* __restrict__ bptr,
+                             const int32_t* __restrict__ boff,
+                             const int32_t* __restrict__ rowptr,
+                             const int32_t* __restrict__ cols, const double* __restrict__ K,
+                             const double* __restrict__ M, const double* __restrict__ omf,
+                             double2* __restrict__ binv, int32_t* __restrict__ bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nent * nf) return;
+  const int64_t e = i % nent;
+  const int f = (int)(i / nent);
+  const int i0 = bptr[e], d = bptr[e + 1] - i0;
+  double2 a[kBlk][kBlk], inv[kBlk][kBlk];
+  for (int t = 0; t < d; ++t) {
+    const int row = i0 + t;
+    int lo = rowptr[row], hi = rowptr[row + 1] - 1, pos = -1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int c = cols[mid];
+      if (c == i0) {
+        pos = mid;
+        break;
+      }
+      if (c < i0) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    if (pos < 0 || pos + d > rowptr[row + 1] || cols[pos + d - 1] != i0 + d - 1) {
+      bad[0] = 1;
+      return;
+    }
+    for (int u = 0; u < d; ++u) {
+      a[t][u] = make_double2(K[pos + u], -omf[f] * M[pos + u]);
+      inv[t][u] = make_double2(t == u ? 1.0 : 0.0, 0.0);
+    }
+  }
+  for (int c = 0; c < d; ++c) {
+    int piv = c;
+    double best = hypot(a[c][c].x, a[c][c].y);
+    for (int t = c + 1; t < d; ++t) {
+      const double v = hypot(a[t][c].x, a[t][c].y);
+      if (v > best) {
+        best = v;
+        piv = t;
+      }
+    }
+    if (!(best > 0.0)) {
+      bad[0] = 1;
+      return;
+    }
+    if (piv != c)
+      for (int u = 0; u < d; ++u) {
+        const double2 t1 = a[c][u], t2 = inv[c][u];
+        a[c][u] = a[piv][u];
+        inv[c][u] = inv[piv][u];
+        a[piv][u] = t1;
+        inv[piv][u] = t2;
+      }
+    const double2 ip = zdiv(make_double2(1.0, 0.0), a[c][c]);
+    for (int u = 0; u < d; ++u) {
+      a[c][u] = zmul(a[c][u], ip);
+      inv[c][u] = zmul(inv[c][u], ip);
+    }
+    for (int t = 0; t < d; ++t) {
+      if (t == c) continue;
+      const double2 fct = a[t][c];
+      for (int u = 0; u < d; ++u) {
+        a[t][u] = zsub(a[t][u], zmul(fct, a[c][u]));
+        inv[t][u] = zsub(inv[t][u], zmul(fct, inv[c][u]));
+      }
+    }
+  }
+  double2* out = binv + (int64_t)f * nb2 + boff[e];
+  for (int t = 0; t < d; ++t)
+    for (int u = 0; u < d; ++u) out[t * d + u] = inv[t][u];
+}
+
+// entities of this workgroup (the same chunking as the row kernels, over entities)
+__device__ inline void ents_of_block(int64_t nent, int64_t* lo, int64_t* hi) {
+  rows_of_block(nent, lo, hi);
+}
+
+// x += alpha p; r -= alpha q; z = B_e r (entity blocks); partials of r^T z, |r|^2
+template <int CT, int TMAX, typename VT>
+__global__ void __launch_bounds__(kBS) k_update_blk(const Ctl* ctl, const Col* col, int64_t nent,
+                                                    int R, const int32_t* __restrict__ bptr,
+                                                    const int32_t* __restrict__ boff, int64_t nb2,
+                                                    const double2* __restrict__ binv,
+                                                    const int32_t* __restrict__ fidx,
+                                                    VT* __restrict__ X, VT* __restrict__ Rr,
+                                                    const VT* __restrict__ P,
+                                                    const VT* __restrict__ Q,
+                                                    VT* __restrict__ Z, double* part) {
+  if (ctl->all_done) return;
+  constexpr int EPB = kBS / CT;
+  const int c = threadIdx.x % CT, es = threadIdx.x / CT;
+  const int T = R / CT;
+  int64_t lo, hi;
+  ents_of_block(nent, &lo, &hi);
+  double v[TMAX][3];
+  VT al[TMAX];
+  int fi[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    v[t][0] = v[t][1] = v[t][2] = 0.0;
+    al[t] = from_d<VT>(t < T ? col[c + t * CT].alpha : make_double2(0.0, 0.0));
+    fi[t] = t < T ? fidx[c + t * CT] : 0;
+  }
+  for (int64_t e = lo + es; e < hi; e += EPB) {
+    const int i0 = bptr[e], d = bptr[e + 1] - i0;
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int cc = c + t * CT;
+        double2 rl[kBlk];
+#pragma unroll
+        for (int k = 0; k < kBlk; ++k)
+          if (k < d) {
+            const int64_t idx = (int64_t)(i0 + k) * R + cc;
+            const VT pv = P[idx], qv = Q[idx];
+            VT xv = X[idx], rv = Rr[idx];
+            xv.x += al[t].x * pv.x - al[t].y * pv.y;
+            xv.y += al[t].x * pv.y + al[t].y * pv.x;
+            rv.x -= al[t].x * qv.x - al[t].y * qv.y;
+            rv.y -= al[t].x * qv.y + al[t].y * qv.x;
+            X[idx] = xv;
+            Rr[idx] = rv;
+            rl[k] = to_d(rv);
+          }
+        const double2* Bm = binv + (int64_t)fi[t] * nb2 + boff[e];
+#pragma unroll
+        for (int k = 0; k < kBlk; ++k)
+          if (k < d) {
+            double2 zk = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < kBlk; ++u)
+              if (u < d) zk = zadd(zk, zmul(Bm[k * d + u], rl[u]));
+            const VT zv = from_d<VT>(zk);
+            Z[(int64_t)(i0 + k) * R + cc] = zv;
+            const double2 zd = to_d(zv);
+            v[t][0] += rl[k].x * zd.x - rl[k].y * zd.y;
+            v[t][1] += rl[k].x * zd.y + rl[k].y * zd.x;
+            v[t][2] += rl[k].x * rl[k].x + rl[k].y * rl[k].y;
+          }
+      }
+  }
+  column_partials<CT, 3, TMAX>(R, v, part);
+}
+
+// z = B_e r for every entity and column (after a (re)start that formed r);
+// overwrites the r^T z partials (slots 0, 1) of the kernel before it
+template <int CT, int TMAX, typename VT>
+__global__ void __launch_bounds__(kBS) k_bapply(int64_t nent, int R,
+                                                const int32_t* __restrict__ bptr,
+                                                const int32_t* __restrict__ boff, int64_t nb2,
+                                                const double2* __restrict__ binv,
+                                                const int32_t* __restrict__ fidx,
+                                                const VT* __restrict__ Rr, VT* __restrict__ Z,
+                                                double* part) {
+  constexpr int EPB = kBS / CT;
+  const int c = threadIdx.x % CT, es = threadIdx.x / CT;
+  const int T = R / CT;
+  int64_t lo, hi;
+  ents_of_block(nent, &lo, &hi);
+  double v[TMAX][2];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) v[t][0] = v[t][1] = 0.0;
+  for (int64_t e = lo + es; e < hi; e += EPB) {
+    const int i0 = bptr[e], d = bptr[e + 1] - i0;
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int cc = c + t * CT;
+        double2 rl[kBlk];
+#pragma unroll
+        for (int k = 0; k < kBlk; ++k)
+          if (k < d) rl[k] = to_d(Rr[(int64_t)(i0 + k) * R + cc]);
+        const double2* Bm = binv + (int64_t)fidx[cc] * nb2 + boff[e];
+#pragma unroll
+        for (int k = 0; k < kBlk; ++k)
+          if (k < d) {
+            double2 zk = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < kBlk; ++u)
+              if (u < d) zk = zadd(zk, zmul(Bm[k * d + u], rl[u]));
+            const VT zv = from_d<VT>(zk);
+            Z[(int64_t)(i0 + k) * R + cc] = zv;
+            const double2 zd = to_d(zv);
+            v[t][0] += rl[k].x * zd.x - rl[k].y * zd.y;
+            v[t][1] += rl[k].x * zd.y + rl[k].y * zd.x;
+          }
+      }
+  }
+  column_partials<CT, 2, TMAX>(R, v, part);
+}
+
 __global__ void k_diag(int64_t n, const int32_t* __restrict__ rowptr,
                        const int32_t* __restrict__ cols, const double* __restrict__ K,
                        const double* __restrict__ M, double* __restrict__ dK,
@@ -813,6 +1017,13 @@ struct Batch {
   bool mixed = false;
   double delta2 = 1e-8;
   int stall = 400;
+  // entity-block Jacobi (optional): entity DOF ranges, d^2 prefix sums, block
+  // inverses per distinct frequency, frequency index per column
+  int32_t *bptr = nullptr, *boff = nullptr, *fidx = nullptr;
+  int64_t nb2 = 0;
+  int nfreq = 0;
+  double2* binv = nullptr;
+  std::vector<double> hom;  // omega * mu0 per column (host)
   float2 *R32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *Z32 = nullptr, *X32 = nullptr;
   float *Ks32 = nullptr, *Ms32 = nullptr;
   std::vector<void*> owned;
@@ -855,8 +1066,12 @@ static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q,
                        B->K, B->M, B->omu, (const double2*)P, (double2*)Q, B->part);
   }
   hipLaunchKernelGGL(k_fin_mu, gr, b, 0, st, B->ctl, B->col, B->R, B->part, np);
-  hipLaunchKernelGGL((k_update<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK,
-                     B->dM, B->omu, X, Rr, P, Q, Z, B->part);
+  if (B->binv)
+    hipLaunchKernelGGL((k_update_blk<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->nent, B->R,
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, X, Rr, P, Q, Z, B->part);
+  else
+    hipLaunchKernelGGL((k_update<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK,
+                       B->dM, B->omu, X, Rr, P, Q, Z, B->part);
   hipLaunchKernelGGL(k_fin_update, gr, b, 0, st, B->ctl, B->col, B->R, B->part, B->grid);
   hipLaunchKernelGGL(k_fin_all, dim3(1), b, 0, st, B->ctl, B->col, B->R);
 }
@@ -876,6 +1091,9 @@ static void launch_refine(Batch* B, hipStream_t st, const double2* Bv, const dou
   hipLaunchKernelGGL((k_refine<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->rowptr,
                      B->cols, B->K, B->M, B->dK, B->dM, B->omu, X, Bv, B->R32, B->Z32, B->P32,
                      B->X32, B->part);
+  if (B->binv)
+    hipLaunchKernelGGL((k_bapply<CT, TM, float2>), dim3(B->grid), dim3(kBS), 0, st, B->nent, B->R,
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->R32, B->Z32, B->part);
   hipLaunchKernelGGL(k_fin_refine, dim3(B->R), dim3(kBS), 0, st, B->ctl, B->col, B->R, B->part,
                      B->grid, first, B->delta2);
   hipLaunchKernelGGL(k_ctl_resume, dim3(1), dim3(64), 0, st, B->ctl);
@@ -886,6 +1104,9 @@ template <int CT, int TM>
 static void launch_init(Batch* B, hipStream_t st, const double2* Bv, double2* X) {
   hipLaunchKernelGGL((k_init<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->dK, B->dM,
                      B->omu, Bv, X, B->Rr, B->Z, B->P, B->part);
+  if (B->binv)
+    hipLaunchKernelGGL((k_bapply<CT, TM, double2>), dim3(B->grid), dim3(kBS), 0, st, B->nent, B->R,
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->Rr, B->Z, B->part);
   hipLaunchKernelGGL(k_fin_init, dim3(B->R), dim3(kBS), 0, st, B->col, B->R, B->part, B->grid);
 }
 
@@ -909,11 +1130,22 @@ static int refresh_diag(Batch* B, hipStream_t st) {
   PG_HIP(hipMemsetAsync(B->bad, 0, sizeof(int32_t), st));
   hipLaunchKernelGGL(k_diag, dim3((B->n + kBS - 1) / kBS), dim3(kBS), 0, st, B->n, B->rowptr,
                      B->cols, B->K, B->M, B->dK, B->dM, B->bad);
+  if (B->binv) {
+    std::vector<double> fo;  // distinct frequencies, in order of first appearance
+    for (double v : B->hom)
+      if (std::find(fo.begin(), fo.end(), v) == fo.end()) fo.push_back(v);
+    double* dfo = reinterpret_cast<double*>(B->part);  // staging: partials are free here
+    PG_HIP(hipMemcpyAsync(dfo, fo.data(), fo.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    const int64_t work = B->nent * (int64_t)fo.size();
+    hipLaunchKernelGGL(k_batch_binv, dim3((unsigned)((work + kBS - 1) / kBS)), dim3(kBS), 0, st,
+                       B->nent, (int)fo.size(), B->nb2, B->bptr, B->boff, B->rowptr, B->cols, B->K,
+                       B->M, dfo, B->binv, B->bad);
+  }
   PG_CHECK_LAUNCH();
   PG_HIP(hipMemcpyAsync(&B->hctl->maxit, B->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
   PG_HIP(hipStreamSynchronize(st));
   if (B->hctl->maxit != 0)
-    PG_FAIL(PG_ERR_ARG, "zero diagonal entry: Jacobi preconditioner undefined");
+    PG_FAIL(PG_ERR_ARG, "zero diagonal entry or singular block: preconditioner undefined");
   return PG_OK;
 }
 
@@ -922,6 +1154,9 @@ static void launch_reinit(Batch* B, hipStream_t st, const double2* Bv, const dou
   hipLaunchKernelGGL((k_reinit<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->rowptr,
                      B->cols, B->K, B->M, B->dK, B->dM, B->omu, X, Bv, B->Rr, B->Z, B->P,
                      B->part);
+  if (B->binv)
+    hipLaunchKernelGGL((k_bapply<CT, TM, double2>), dim3(B->grid), dim3(kBS), 0, st, B->nent, B->R,
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->Rr, B->Z, B->part);
   hipLaunchKernelGGL(k_fin_reinit, dim3(B->R), dim3(kBS), 0, st, B->ctl, B->col, B->R, B->part,
                      B->grid);
   hipLaunchKernelGGL(k_ctl_resume, dim3(1), dim3(64), 0, st, B->ctl);
@@ -1007,6 +1242,7 @@ int pg_batch_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t
   }
   std::vector<double> om(R);
   for (int r = 0; r < R; ++r) om[r] = omega[r] * 4.0 * 3.14159265358979323846 * 1e-7;
+  B->hom = om;
   if (hipMemcpy(B->omu, om.data(), R * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
     delete B;
     PG_FAIL(PG_ERR_HIP, "frequency upload failed");
@@ -1178,6 +1414,7 @@ int pg_batch_set_entities(void* batch, const int32_t* block_ptr, int64_t nblocks
   B->nent = nblocks;
   B->dmax = dmax;
   B->em = dem;
+  B->bptr = bp;
   return refresh_diag(B, st);
 }
 
@@ -1198,6 +1435,39 @@ int pg_batch_set_mixed(void* batch, double inner_rtol, int stall_iters, void* st
   B->stall = stall_iters > 0 ? stall_iters : 400;
   B->mixed = true;
   return refresh_diag(B, as_stream(stream));
+}
+
+int pg_batch_set_block_jacobi(void* batch, void* stream) {
+  if (!batch) PG_FAIL(PG_ERR_ARG, "batch is NULL");
+  Batch* B = static_cast<Batch*>(batch);
+  if (!B->em) PG_FAIL(PG_ERR_ARG, "block Jacobi needs the entities (pg_batch_set_entities)");
+  if (B->binv) return PG_OK;
+  std::vector<int32_t> bp(B->nent + 1), off(B->nent + 1);
+  PG_HIP(hipMemcpy(bp.data(), B->bptr, (B->nent + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  off[0] = 0;
+  for (int64_t e = 0; e < B->nent; ++e) {
+    const int d = bp[e + 1] - bp[e];
+    off[e + 1] = off[e] + d * d;
+  }
+  std::vector<double> fo;
+  std::vector<int32_t> fi(B->R);
+  for (int r = 0; r < B->R; ++r) {
+    auto it = std::find(fo.begin(), fo.end(), B->hom[r]);
+    fi[r] = (int32_t)(it - fo.begin());
+    if (it == fo.end()) fo.push_back(B->hom[r]);
+  }
+  if ((int64_t)fo.size() > kMaxGrid) PG_FAIL(PG_ERR_ARG, "too many distinct frequencies");
+  int rc;
+  if ((rc = B->alloc(&B->boff, B->nent + 1)) != PG_OK || (rc = B->alloc(&B->fidx, B->R)) != PG_OK ||
+      (rc = B->alloc(&B->binv, (size_t)fo.size() * off[B->nent])) != PG_OK)
+    return rc;
+  PG_HIP(hipMemcpy(B->boff, off.data(), (B->nent + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  PG_HIP(hipMemcpy(B->fidx, fi.data(), B->R * sizeof(int32_t), hipMemcpyHostToDevice));
+  B->nb2 = off[B->nent];
+  B->nfreq = (int)fo.size();
+  const int rc2 = refresh_diag(B, as_stream(stream));
+  if (rc2 != PG_OK) B->binv = nullptr;  // stays allocated; the batch keeps point Jacobi
+  return rc2;
 }
 
 int pg_batch_refresh(void* batch, void* stream) {

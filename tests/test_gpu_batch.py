@@ -30,14 +30,17 @@ def test_km_assembly_consistent_with_complex():
     assert np.abs(A.imag - (-(om * core.MU0) * M)).max() <= 1e-15 * np.abs(A).max()
 
 
-@pytest.mark.parametrize("entity_rows,precision", [(False, "fp64"), (True, "fp64"),
-                                                  (True, "mixed")])
+@pytest.mark.parametrize("entity_rows,precision,pc", [(False, "fp64", "jacobi"),
+                                                     (True, "fp64", "jacobi"),
+                                                     (True, "mixed", "jacobi"),
+                                                     (True, "fp64", "bjacobi"),
+                                                     (True, "mixed", "bjacobi")])
 @pytest.mark.parametrize("n,p,nsrc,freqs", [(6, 2, 4, [0.25, 1.0]), (4, 2, 16, [0.1, 0.25, 0.5, 1.0]),
                                             (4, 1, 32, [0.5, 1.0, 2.0, 4.0]),
                                             (3, 3, 2, [0.5, 1.0]), (4, 3, 8, [0.5, 1.0]),
                                             (5, 1, 8, [0.25, 0.5, 1.0, 2.0]),
                                             (5, 2, 16, [1.0])])
-def test_batch_matches_single_rhs(n, p, nsrc, freqs, entity_rows, precision):
+def test_batch_matches_single_rhs(n, p, nsrc, freqs, entity_rows, precision, pc):
     w, mesh, asm, ent_bc, dof_bc = _setup(n, p)
     K, M = core.assemble_km(asm, ent_bc)
     xs = np.linspace(-600.0, 600.0, nsrc)
@@ -54,9 +57,9 @@ def test_batch_matches_single_rhs(n, p, nsrc, freqs, entity_rows, precision):
     bs = core.BatchSolver(asm.plan, K, M, fr, rtol=1e-10,
                           block_ptr=mesh.block_ptr() if entity_rows else None,
                           entity_order=mesh.entity_order() if entity_rows else None,
-                          precision=precision)
+                          precision=precision, pc=pc)
     X, info = bs.solve(B)
-    print(f"{precision} R={len(fr)} iterations min/max {info['iterations'].min()}/{info['iterations'].max()} "
+    print(f"{precision} {pc} R={len(fr)} iterations min/max {info['iterations'].min()}/{info['iterations'].max()} "
           f"relres max {info['relres'].max():.2e}", flush=True)
     assert (info["status"] == 0).all() and info["relres"].max() <= 1e-10
     Xh = core.as_complex(X)
