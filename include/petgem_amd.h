@@ -257,7 +257,9 @@ int pg_solver_solve(void *solver, const double *b, double *x, pg_solve_info *inf
 int pg_solver_destroy(void *solver);
 /* Average duration (ms) of the solver's own SpMV kernel over `reps` launches
  * on `stream`, timed with HIP events: complex128 (ms64) and, for mixed
- * precision, complex64 (ms32); g64/g32 = lanes per row chosen at creation. */
+ * precision, complex64 (ms32); g64/g32 = the configurations the autotuner
+ * chose at creation, coded kind * 1000 + G * 10 + U (kind 0 CSR rows,
+ * 1 entity rows, 2 entity rows with column pairs; G lanes, U loads in flight). */
 int pg_solver_profile(void *solver, int reps, double *ms64, double *ms32, int32_t *g64,
                       int32_t *g32, void *stream);
 
