@@ -210,6 +210,7 @@ int pg_demote_z2c(int64_t n, const double *src, float *dst, void *stream);
 #define PG_SPMV_ROW 1
 #define PG_SPMV_ENTITY 2
 #define PG_SPMV_PAIRS 3  /* entity rows with column pairs: every entity two DOFs (p = 2) */
+#define PG_SPMV_ELEMENT 4 /* element-by-element operator (pg_solve_opts.eop) */
 
 typedef struct pg_solve_opts {
   int32_t ksp;        /* PG_KSP_* */
@@ -230,6 +231,12 @@ typedef struct pg_solve_opts {
                                   order in which the entity-row SpMV visits them (a
                                   spatially local order keeps the gathers of x in cache);
                                   NULL = natural order */
+  const void *eop;    /* optional element-by-element operator of the same system
+                         (pg_eop_create): with PG_SPMV_AUTO the Krylov iterations use it
+                         when it is faster than the CSR kernels, with PG_SPMV_ELEMENT
+                         always; the true residual always uses the CSR matrix.  The
+                         operator must outlive the solver, and after a reassembly
+                         pg_eop_refresh must accompany pg_solver_refresh. */
 } pg_solve_opts;
 
 typedef struct pg_solve_info {
@@ -259,9 +266,36 @@ int pg_solver_destroy(void *solver);
  * on `stream`, timed with HIP events: complex128 (ms64) and, for mixed
  * precision, complex64 (ms32); g64/g32 = the configurations the autotuner
  * chose at creation, coded kind * 1000 + G * 10 + U (kind 0 CSR rows,
- * 1 entity rows, 2 entity rows with column pairs; G lanes, U loads in flight). */
+ * 1 entity rows, 2 entity rows with column pairs, 3 element-by-element
+ * operator; G lanes, U loads in flight). */
 int pg_solver_profile(void *solver, int reps, double *ms64, double *ms32, int32_t *g64,
                       int32_t *g32, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Element-by-element operator: y = A x for A = sum_e (K_e - i omega mu0 M_e)
+ * with the PEC rows and columns replaced by the identity, without the
+ * assembled matrix (the element matrices are recomputed from the geometry
+ * workspace and the reference tables at every product).  The same product as
+ * MatMult on the assembled system (petgem/solver.py:584-590, inside KSPSolve),
+ * to rounding.  Orders p = 1, 2.
+ *   ws        pg_element_geometry workspace (read at creation and refresh)
+ *   dofs      device (E, n) element DOFs, as for the assembly plan
+ *   block_ptr host (nblocks+1) DOF ranges of the mesh entities
+ *   order     host permutation of the entities (NULL = natural): the order in
+ *             which the entities' rows are summed (a spatial order keeps the
+ *             element results in cache)
+ *   ent_bc    device uint8 per entity, nonzero = PEC (nullable)
+ * ---------------------------------------------------------------------- */
+int pg_eop_create(int p, int64_t E, const void *ws, const int32_t *dofs, int64_t ndof,
+                  const int32_t *block_ptr, int64_t nblocks, const int32_t *order,
+                  const uint8_t *ent_bc, double omega, void **op, void *stream);
+/* Re-read the element coefficients after pg_element_geometry (new conductivity)
+ * or for a new frequency. */
+int pg_eop_refresh(void *op, const void *ws, double omega, void *stream);
+/* y = A x in complex128 / complex64 (interleaved, length 2 ndof). */
+int pg_eop_apply_z(const void *op, const double *x, double *y, void *stream);
+int pg_eop_apply_c(const void *op, const float *x, float *y, void *stream);
+int pg_eop_destroy(void *op);
 
 /* ------------------------------------------------------------------------
  * Batched multi-RHS solve on one mesh (many sources and frequencies in one
@@ -273,9 +307,11 @@ int pg_solver_profile(void *solver, int reps, double *ms64, double *ms32, int32_
 int pg_batch_create(int64_t n, int64_t nnz, const int32_t *rowptr, const int32_t *cols,
                     const double *K, const double *M, int nrhs, const double *omega,
                     double rtol, int maxit, void **batch, void *stream);
-/* Independent COCG + Jacobi per column, true-residual check at the end;
- * per-column iterations, relres and status (0 ok, -1 not converged,
- * PG_ERR_BREAKDOWN) into host arrays (each nullable). */
+/* Independent COCG per column, each column converging on its own; point Jacobi
+ * unless pg_batch_set_block_jacobi was called, complex128 unless
+ * pg_batch_set_mixed was called.  True-residual check at the end; per-column
+ * iterations, relres and status (0 ok, -1 not converged, PG_ERR_BREAKDOWN)
+ * into host arrays (each nullable). */
 int pg_batch_solve(void *batch, const double *b, double *x, int32_t *iterations, double *relres,
                    int32_t *status, void *stream);
 /* Optional: the mesh entities' DOF ranges (host array, nblocks+1 entries, sizes

@@ -32,7 +32,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("ksp", _I32), ("pc", _I32), ("precision", _I32), ("restart", _I32),
                 ("maxit", _I32), ("check_every", _I32), ("rtol", _D), ("inner_rtol", _D),
                 ("block_ptr", _P), ("nblocks", _I64), ("spmv", _I32), ("stall_iters", _I32),
-                ("entity_order", _P)]
+                ("entity_order", _P), ("eop", _P)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -69,6 +69,12 @@ SIGNATURES = {
                                            _P]),
     "pg_adjoint_gradient_rows": (ctypes.c_int, [_P, _I64, _P, _P, ctypes.c_int, _I64, _P, _P, _P,
                                                 _P, _P]),
+    "pg_eop_create": (ctypes.c_int, [ctypes.c_int, _I64, _P, _P, _I64, _P, _I64, _P, _P, _D,
+                                     ctypes.POINTER(_P), _P]),
+    "pg_eop_refresh": (ctypes.c_int, [_P, _P, _D, _P]),
+    "pg_eop_apply_z": (ctypes.c_int, [_P, _P, _P, _P]),
+    "pg_eop_apply_c": (ctypes.c_int, [_P, _P, _P, _P]),
+    "pg_eop_destroy": (ctypes.c_int, [_P]),
     "pg_spmv_z": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     "pg_spmv_c": (ctypes.c_int, [_I64, _P, _P, _P, _P, _P, _P]),
     "pg_demote_z2c": (ctypes.c_int, [_I64, _P, _P, _P]),

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpetgem_amd.so")
 SOURCES = ["pg_api.cpp", "pg_plan.cpp", "pg_element.hip", "pg_krylov.hip", "pg_batch.hip",
-           "pg_multi.hip"]
+           "pg_multi.hip", "pg_elemop.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -424,7 +424,54 @@ _PC = {"none": nat.PG_PC_NONE, "jacobi": nat.PG_PC_JACOBI, "bjacobi": nat.PG_PC_
 _PREC = {"fp64": nat.PG_PREC_FP64, "mixed": nat.PG_PREC_MIXED}
 
 
-_SPMV = {"auto": 0, "row": 1, "entity": 2, "entity_pairs": 3}
+_SPMV = {"auto": 0, "row": 1, "entity": 2, "entity_pairs": 3, "element": 4}
+
+
+class ElementOperator:
+    """Element-by-element product y = A x of the system Assembler.assemble builds
+    (A = sum_e K_e - i omega mu0 M_e, PEC entities as identity rows), without
+    the assembled matrix: the element matrices are recomputed from the geometry
+    workspace and the tables at every product.  Orders p = 1, 2.  Passed to
+    KrylovSolver(eop=...), whose iterations then use it when it is the faster
+    kernel (or always, with spmv="element")."""
+
+    def __init__(self, assembler, omega, ent_bc=None, entity_order=None, stream=None):
+        m = assembler.mesh
+        self.mesh = m
+        self.ndof = m.ndof
+        self._bptr = np.ascontiguousarray(m.block_ptr(), np.int32)
+        self._order = None if entity_order is None else np.ascontiguousarray(entity_order,
+                                                                            np.int32)
+        self._ent_bc = ent_bc
+        self._ws = assembler.workspace
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pg_eop_create(
+            m.p, m.n_elems, nat.ptr(self._ws), nat.ptr(m.dofs), m.ndof,
+            self._bptr.ctypes.data, self._bptr.size - 1,
+            None if self._order is None else self._order.ctypes.data, nat.ptr(ent_bc),
+            float(omega), ctypes.byref(h), nat.stream_handle(stream)))
+        self.handle = h
+
+    def refresh(self, omega, stream=None):
+        """Re-read the element coefficients (after Assembler.geometry with a new
+        conductivity, or for another frequency)."""
+        nat.check(nat.lib().pg_eop_refresh(self.handle, nat.ptr(self._ws), float(omega),
+                                           nat.stream_handle(stream)))
+
+    def apply(self, x, stream=None):
+        """y = A x; x (ndof, 2) float64 (complex128) or float32 (complex64)."""
+        y = torch.empty_like(x)
+        fn = nat.lib().pg_eop_apply_z if x.dtype == torch.float64 else nat.lib().pg_eop_apply_c
+        nat.check(fn(self.handle, nat.ptr(x), nat.ptr(y), nat.stream_handle(stream)))
+        return y
+
+    def __del__(self):
+        try:
+            if self.handle:
+                nat.lib().pg_eop_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
 
 
 class KrylovSolver:
@@ -432,9 +479,10 @@ class KrylovSolver:
 
     def __init__(self, plan, values, ksp="cocg", pc="jacobi", precision="fp64", rtol=1e-8,
                  maxit=100000, check_every=25, inner_rtol=1e-4, restart=30, block_ptr=None,
-                 spmv="auto", entity_order=None, stall_iters=0, stream=None):
+                 spmv="auto", entity_order=None, stall_iters=0, eop=None, stream=None):
         self.plan = plan
         self.values = values
+        self.eop = eop  # kept alive as long as the solver
         if pc == "bjacobi" and block_ptr is None:
             raise ValueError("pc='bjacobi' needs block_ptr (DeviceMesh.block_ptr())")
         # block_ptr (the mesh entities' DOF ranges) also enables the entity-row SpMV
@@ -449,7 +497,8 @@ class KrylovSolver:
                                   int(check_every), float(rtol), float(inner_rtol),
                                   None if self._bptr is None else self._bptr.ctypes.data, nb,
                                   _SPMV[spmv], int(stall_iters),
-                                  None if self._order is None else self._order.ctypes.data)
+                                  None if self._order is None else self._order.ctypes.data,
+                                  None if eop is None else eop.handle)
         h = ctypes.c_void_p()
         nat.check(nat.lib().pg_solver_create(plan.ndof, plan.nnz, nat.ptr(plan.rowptr),
                                              nat.ptr(plan.cols), nat.ptr(values),

@@ -1,0 +1,568 @@
+// Element-by-element ("matrix-free") application of the assembled operator
+//     A = sum_e P_e^T (K_e - i omega mu0 M_e) P_e,   PEC rows and columns -> identity,
+// for gfx950, as an alternative to the CSR SpMV inside the Krylov solver.
+//
+// The element matrices are never stored.  With the reference-tensor form used
+// by the assembly (K_e, M_e linear in 6 + 6 geometric coefficients, tables T
+// per orientation variant), one element's product costs about 4100 FMAs
+// (symmetric, upper triangle) on 288 bytes of input and output per element
+// in complex64, against about 2800 bytes per element for streaming the CSR
+// rows the element contributes to.  The product is split in two kernels:
+//
+//  k_eop_local  one lane per element, 64 elements of one orientation
+//               combination per wavefront, so every table entry is a
+//               wave-uniform (scalar) load; writes the element's n local
+//               results y_e = A_e x_e (slot layout [wave][i][lane]).
+//  k_eop_sum    one thread per mesh entity (edge, face), in the entity
+//               schedule: sums the entity's local results over its elements
+//               in a fixed order (atomic-free, bitwise reproducible), and
+//               fuses the COCG dot product or the residual.
+//
+// Orders p = 1 and 2 (n = 6, 20).  The summation order differs from the
+// assembled CSR's, so results agree with the CSR SpMV to rounding.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+#include <vector>
+
+#include "pg_basis.h"
+#include "pg_common.h"
+#include "pg_eop.h"
+
+namespace pg {
+
+namespace eop {
+
+constexpr int kBS = 256;
+constexpr int kMaxGrid = 2048;
+constexpr double kMu0 = 4.0 * 3.14159265358979323846 * 1e-7;  // petgem/solver.py:175
+
+template <typename R>
+struct Cx;
+template <>
+struct Cx<double> {
+  using T = double2;
+};
+template <>
+struct Cx<float> {
+  using T = float2;
+};
+
+__device__ inline double2 to_d(double2 v) { return v; }
+__device__ inline double2 to_d(float2 v) { return make_double2(v.x, v.y); }
+__device__ inline void store(double2* p, double2 v) { *p = v; }
+__device__ inline void store(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
+template <int NV>
+__device__ inline void write_partials(double (&v)[NV], double* part) {
+  __shared__ double sm[NV][kBS / kWave];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sm[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < kBS / kWave; ++i) s += sm[k][i];
+      part[(int64_t)k * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// local products.  coef[w][c][lane]: c < 6 the mass coefficients times
+// -omega mu0, c >= 6 the curl-curl coefficients; tab[combo][k][c] the table
+// entries of pair k = (i, j >= i) (row-major upper triangle), c < 6 mass,
+// c >= 6 curl-curl.  edofs[w][i][lane] = global DOF, -1 for a PEC DOF or an
+// empty lane (its x is taken as 0).
+// ---------------------------------------------------------------------------
+template <int P, typename R>
+__global__ void __launch_bounds__(kBS) k_eop_local(int64_t nw, const int32_t* __restrict__ done,
+                                                   const int32_t* __restrict__ wcombo,
+                                                   const int32_t* __restrict__ edofs,
+                                                   const R* __restrict__ coef,
+                                                   const R* __restrict__ tab,
+                                                   const typename Cx<R>::T* __restrict__ x,
+                                                   typename Cx<R>::T* __restrict__ ye) {
+  using C = typename Cx<R>::T;
+  constexpr int N = P * (P + 2) * (P + 3) / 2;
+  constexpr int NP = N * (N + 1) / 2;
+  if (done && *done) return;
+  // XCD-contiguous ranges of waves (workgroup b runs on XCD b % 8)
+  const int64_t nb = gridDim.x;
+  int64_t b = blockIdx.x;
+  if (nb % 8 == 0) b = (b % 8) * (nb / 8) + b / 8;
+  const int64_t w = b * (kBS / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  if (w >= nw) return;
+  const int lane = threadIdx.x % kWave;
+  const int combo = __builtin_amdgcn_readfirstlane(wcombo[w]);
+  const R* __restrict__ t = tab + (int64_t)combo * NP * 12;
+  C xv[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int d = edofs[(w * N + i) * kWave + lane];
+    xv[i] = d >= 0 ? x[d] : C{0, 0};
+  }
+  R cf[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) cf[c] = coef[(w * 12 + c) * kWave + lane];
+  C y[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) y[i] = C{0, 0};
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = i; j < N; ++j) {
+      const R* tk = t + (i * N - (i * (i - 1)) / 2 + (j - i)) * 12;
+      R im = cf[0] * tk[0];
+      R re = cf[6] * tk[6];
+#pragma unroll
+      for (int c = 1; c < 6; ++c) {
+        im = fma(cf[c], tk[c], im);
+        re = fma(cf[6 + c], tk[6 + c], re);
+      }
+      y[i].x = fma(re, xv[j].x, fma(-im, xv[j].y, y[i].x));
+      y[i].y = fma(re, xv[j].y, fma(im, xv[j].x, y[i].y));
+      if (j != i) {
+        y[j].x = fma(re, xv[i].x, fma(-im, xv[i].y, y[j].x));
+        y[j].y = fma(re, xv[i].y, fma(im, xv[i].x, y[j].y));
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) ye[(w * N + i) * kWave + lane] = y[i];
+}
+
+// ---------------------------------------------------------------------------
+// per-entity sums.  gm[i] = {r0, d, s0, s1} for the i-th scheduled entity:
+// rows r0 .. r0+d, local results at gslot[s0 .. s1) (+ t * 64 for DOF t);
+// s0 == s1 marks a PEC entity (identity rows).  MODE as in the CSR SpMV.
+// XCD x (workgroup b runs on XCD b % 8) sweeps the x-th eighth of the schedule.
+// ---------------------------------------------------------------------------
+template <int D, int MODE, typename C>
+__global__ void __launch_bounds__(kBS) k_eop_sum(int64_t nent, const int32_t* __restrict__ done,
+                                                 const int4* __restrict__ gm,
+                                                 const int32_t* __restrict__ gslot,
+                                                 const C* __restrict__ ye, const C* __restrict__ x,
+                                                 C* __restrict__ y, const double2* __restrict__ b,
+                                                 double* part) {
+  if (done && *done) return;
+  const int64_t blk = blockIdx.x, nb = gridDim.x;
+  int64_t first = blk * kBS, end = nent, stride = nb * kBS;
+  if (nb % 8 == 0) {
+    const int64_t xc = blk % 8, nbx = nb / 8;
+    first = nent * xc / 8 + (blk / 8) * kBS;
+    end = nent * (xc + 1) / 8;
+    stride = nbx * kBS;
+  }
+  double red[2] = {0.0, 0.0};
+  for (int64_t ent = first + threadIdx.x; ent < end; ent += stride) {
+    const int4 m = gm[ent];
+    const int r0 = m.x, d = m.y, s0 = m.z, s1 = m.w;
+    C acc[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) acc[t] = C{0, 0};
+    if (s0 == s1) {
+#pragma unroll
+      for (int t = 0; t < D; ++t)
+        if (t < d) acc[t] = x[r0 + t];
+    } else {
+      for (int k = s0; k < s1; ++k) {
+        const int base = gslot[k];
+#pragma unroll
+        for (int t = 0; t < D; ++t)
+          if (t < d) {
+            const C v = ye[base + t * kWave];
+            acc[t].x += v.x;
+            acc[t].y += v.y;
+          }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      if (t < d) {
+        const int row = r0 + t;
+        if (MODE == 2) {
+          const double2 bb = b[row];
+          const double2 rr = make_double2(bb.x - (double)acc[t].x, bb.y - (double)acc[t].y);
+          store(y + row, rr);
+          red[0] += rr.x * rr.x + rr.y * rr.y;
+        } else {
+          y[row] = acc[t];
+          if (MODE == 1) {
+            const double2 pv = to_d(x[row]), qv = to_d(acc[t]);
+            red[0] += pv.x * qv.x - pv.y * qv.y;
+            red[1] += pv.x * qv.y + pv.y * qv.x;
+          }
+        }
+      }
+  }
+  if (MODE == 1) write_partials<2>(red, part);
+  if (MODE == 2) {
+    double r1[1] = {red[0]};
+    write_partials<1>(r1, part);
+  }
+}
+
+// coefficients in slot order: mass part times -omega mu0, then curl-curl part
+template <typename R>
+__global__ void k_eop_coef(int64_t nslot, const int32_t* __restrict__ perm,
+                           const double* __restrict__ wscoef, double omega_mu,
+                           R* __restrict__ coef) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslot) return;
+  const int64_t w = s / kWave, lane = s % kWave;
+  const int64_t e = perm[s];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) {
+    double v = e >= 0 ? wscoef[12 * e + c] : 0.0;
+    if (c < 6) v = -(omega_mu * v);
+    coef[(w * 12 + c) * kWave + lane] = (R)v;
+  }
+}
+
+}  // namespace eop
+
+struct ElemOp {
+  int p = 0, N = 0, D = 1;
+  int64_t E = 0, ndof = 0, nent = 0, nw = 0;
+  int ncombo = 0, grid = 1;
+  int32_t* wcombo = nullptr;
+  int32_t* edofs = nullptr;
+  int32_t* perm = nullptr;
+  int4* gm = nullptr;
+  int32_t* gslot = nullptr;
+  double* coef64 = nullptr;
+  float* coef32 = nullptr;
+  double* tab64 = nullptr;
+  float* tab32 = nullptr;
+  double2* ye64 = nullptr;
+  float2* ye32 = nullptr;
+  std::vector<void*> owned;
+
+  template <typename T>
+  int alloc(T** ptr, size_t count) {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess)
+      PG_FAIL(PG_ERR_HIP, "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+    owned.push_back(v);
+    *ptr = static_cast<T*>(v);
+    return PG_OK;
+  }
+  ~ElemOp() {  // release errors cannot be reported from here
+    for (void* v : owned) (void)hipFree(v);
+  }
+};
+
+int eop_grid(const ElemOp* op) { return op->grid; }
+int64_t eop_ndof(const ElemOp* op) { return op->ndof; }
+
+template <typename R>
+static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
+                        const typename eop::Cx<R>::T* x, typename eop::Cx<R>::T* y,
+                        const double2* b, double* part, hipStream_t st,
+                        typename eop::Cx<R>::T* ye, const R* coef, const R* tab) {
+  using C = typename eop::Cx<R>::T;
+  const int wpb = eop::kBS / kWave;
+  int64_t nbl = (op->nw + wpb - 1) / wpb;
+  if (nbl >= 64) nbl = (nbl + 7) / 8 * 8;
+  if (op->p == 1)
+    hipLaunchKernelGGL((eop::k_eop_local<1, R>), dim3((unsigned)nbl), dim3(eop::kBS), 0, st,
+                       op->nw, done, op->wcombo, op->edofs, coef, tab, x, ye);
+  else
+    hipLaunchKernelGGL((eop::k_eop_local<2, R>), dim3((unsigned)nbl), dim3(eop::kBS), 0, st,
+                       op->nw, done, op->wcombo, op->edofs, coef, tab, x, ye);
+  const dim3 g(op->grid), bl(eop::kBS);
+#define PG_SUM(DD, MM)                                                                        \
+  hipLaunchKernelGGL((eop::k_eop_sum<DD, MM, C>), g, bl, 0, st, op->nent, done, op->gm,       \
+                     op->gslot, (const C*)ye, x, y, b, part)
+  if (op->D == 1) {
+    if (mode == 0) PG_SUM(1, 0);
+    else if (mode == 1) PG_SUM(1, 1);
+    else PG_SUM(1, 2);
+  } else {
+    if (mode == 0) PG_SUM(2, 0);
+    else if (mode == 1) PG_SUM(2, 1);
+    else PG_SUM(2, 2);
+  }
+#undef PG_SUM
+}
+
+void eop_launch(const ElemOp* op, int mode, const int32_t* done, const double2* x, double2* y,
+                const double2* b, double* part, hipStream_t st) {
+  launch_impl<double>(op, mode, done, x, y, b, part, st, op->ye64, op->coef64, op->tab64);
+}
+void eop_launch(const ElemOp* op, int mode, const int32_t* done, const float2* x, float2* y,
+                const double2* b, double* part, hipStream_t st) {
+  launch_impl<float>(op, mode, done, x, y, b, part, st, op->ye32, op->coef32, op->tab32);
+}
+
+static int eop_refresh(ElemOp* op, const void* ws, double omega, hipStream_t st) {
+  const double* wscoef = reinterpret_cast<const double*>(ws);
+  const int64_t nslot = op->nw * kWave;
+  const double om = omega * eop::kMu0;
+  const dim3 g((unsigned)((nslot + 255) / 256)), b(256);
+  hipLaunchKernelGGL(eop::k_eop_coef<double>, g, b, 0, st, nslot, op->perm, wscoef, om,
+                     op->coef64);
+  hipLaunchKernelGGL(eop::k_eop_coef<float>, g, b, 0, st, nslot, op->perm, wscoef, om,
+                     op->coef32);
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+
+int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t ndof,
+                  const int32_t* block_ptr, int64_t nblocks, const int32_t* order,
+                  const uint8_t* ent_bc, double omega, void** out, void* stream) {
+  if (!out || !ws || !dofs || !block_ptr || E <= 0 || ndof <= 0 || nblocks <= 0)
+    PG_FAIL(PG_ERR_ARG, "bad arguments");
+  if (p < 1 || p > 2)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator: order p=%d not supported (1, 2)", p);
+  const int N = ndof_element(p);
+  const int nle = p == 1 ? 6 : 10;  // local entities carrying DOFs
+  const int maxd = p == 1 ? 1 : 2;
+  hipStream_t st = as_stream(stream);
+  const int32_t* bp = block_ptr;
+  if (bp[0] != 0 || bp[nblocks] != ndof)
+    PG_FAIL(PG_ERR_ARG, "block_ptr must start at 0 and end at ndof");
+  for (int64_t g = 0; g < nblocks; ++g)
+    if (bp[g + 1] - bp[g] < 1 || bp[g + 1] - bp[g] > maxd)
+      PG_FAIL(PG_ERR_ARG, "entity sizes must be in 1..%d at p=%d", maxd, p);
+  // host copies of the element DOFs, orientations and PEC entities
+  std::vector<int32_t> hd((size_t)E * N), hor(E);
+  std::vector<uint8_t> hbc(nblocks, 0);
+  const int32_t* dor = reinterpret_cast<const int32_t*>(reinterpret_cast<const double*>(ws) + 12 * E);
+  PG_HIP(hipMemcpyAsync(hd.data(), dofs, hd.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipMemcpyAsync(hor.data(), dor, E * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (ent_bc)
+    PG_HIP(hipMemcpyAsync(hbc.data(), ent_bc, nblocks, hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  std::vector<int32_t> ent_of(ndof);
+  for (int64_t g = 0; g < nblocks; ++g)
+    for (int d = bp[g]; d < bp[g + 1]; ++d) ent_of[d] = (int32_t)g;
+  // each local entity's global entity; its DOFs must be that entity's range in order
+  std::vector<int32_t> eg((size_t)E * nle);
+  std::vector<int64_t> cnt(nblocks, 0);
+  for (int64_t e = 0; e < E; ++e)
+    for (int la = 0; la < nle; ++la) {
+      const int off = entity_local_offset(la, p), nd = entity_ndof(la, p);
+      const int d0 = hd[e * N + off];
+      if (d0 < 0 || d0 >= ndof) PG_FAIL(PG_ERR_ARG, "element %lld: DOF out of range", (long long)e);
+      const int g = ent_of[d0];
+      if (bp[g + 1] - bp[g] != nd || bp[g] != d0)
+        PG_FAIL(PG_ERR_ARG, "element %lld: local entity %d does not match a block_ptr range",
+                (long long)e, la);
+      for (int k = 1; k < nd; ++k)
+        if (hd[e * N + off + k] != d0 + k)
+          PG_FAIL(PG_ERR_ARG, "element %lld: entity DOFs out of order", (long long)e);
+      eg[e * nle + la] = g;
+      if (!hbc[g]) cnt[g]++;
+    }
+  for (int64_t g = 0; g < nblocks; ++g)
+    if (!hbc[g] && cnt[g] == 0)
+      PG_FAIL(PG_ERR_ARG, "entity %lld belongs to no element", (long long)g);
+  std::vector<int32_t> pos(nblocks);
+  if (order) {
+    std::vector<char> seen(nblocks, 0);
+    for (int64_t i = 0; i < nblocks; ++i) {
+      if (order[i] < 0 || order[i] >= nblocks || seen[order[i]])
+        PG_FAIL(PG_ERR_ARG, "entity_order must be a permutation of 0..nblocks-1");
+      seen[order[i]] = 1;
+      pos[order[i]] = (int32_t)i;
+    }
+  } else {
+    std::iota(pos.begin(), pos.end(), 0);
+  }
+  // orientation combinations
+  std::map<int, int> cid;
+  std::vector<int> combos;
+  std::vector<int32_t> ec(E);
+  for (int64_t e = 0; e < E; ++e) {
+    auto it = cid.find(hor[e]);
+    if (it == cid.end()) {
+      it = cid.emplace(hor[e], (int)combos.size()).first;
+      combos.push_back(hor[e]);
+    }
+    ec[e] = it->second;
+  }
+  // element order: by combination, then by the earliest schedule position of
+  // the element's entities (spatial order when the schedule is)
+  std::vector<int64_t> key(E);
+  for (int64_t e = 0; e < E; ++e) {
+    int32_t m = pos[eg[e * nle]];
+    for (int la = 1; la < nle; ++la) m = std::min(m, pos[eg[e * nle + la]]);
+    key[e] = ((int64_t)ec[e] << 32) | (uint32_t)m;
+  }
+  std::vector<int64_t> sorted(E);
+  std::iota(sorted.begin(), sorted.end(), 0);
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  // waves of 64 elements of one combination
+  std::vector<int32_t> wcombo, perm;
+  for (int64_t k = 0; k < E;) {
+    const int c = ec[sorted[k]];
+    int64_t k1 = k;
+    while (k1 < E && ec[sorted[k1]] == c) ++k1;
+    for (int64_t a = k; a < k1; a += kWave) {
+      wcombo.push_back(c);
+      for (int l = 0; l < kWave; ++l) perm.push_back(a + l < k1 ? (int32_t)sorted[a + l] : -1);
+    }
+    k = k1;
+  }
+  const int64_t nw = (int64_t)wcombo.size(), nslot = nw * kWave;
+  if (nslot * N >= (1LL << 31)) PG_FAIL(PG_ERR_UNSUPPORTED, "too many elements");
+  std::vector<int32_t> hed((size_t)nslot * N, -1);
+  for (int64_t s = 0; s < nslot; ++s) {
+    const int64_t e = perm[s];
+    if (e < 0) continue;
+    const int64_t w = s / kWave, l = s % kWave;
+    for (int i = 0; i < N; ++i) {
+      const int d = hd[e * N + i];
+      hed[(w * N + i) * kWave + l] = hbc[ent_of[d]] ? -1 : d;
+    }
+  }
+  // per-entity lists of local-result slots, in schedule order; within an
+  // entity in ascending slot order (a fixed summation order)
+  std::vector<int64_t> start(nblocks + 1, 0);
+  for (int64_t i = 0; i < nblocks; ++i) {
+    const int64_t g = order ? order[i] : i;
+    start[i + 1] = start[i] + (hbc[g] ? 0 : cnt[g]);
+  }
+  if (start[nblocks] >= (1LL << 31)) PG_FAIL(PG_ERR_UNSUPPORTED, "too many incidences");
+  std::vector<int32_t> gslot(start[nblocks]);
+  std::vector<int64_t> cur(nblocks);
+  for (int64_t g = 0; g < nblocks; ++g) cur[g] = start[pos[g]];
+  for (int64_t s = 0; s < nslot; ++s) {
+    const int64_t e = perm[s];
+    if (e < 0) continue;
+    const int64_t w = s / kWave, l = s % kWave;
+    for (int la = 0; la < nle; ++la) {
+      const int g = eg[e * nle + la];
+      if (hbc[g]) continue;
+      gslot[cur[g]++] = (int32_t)((w * N + entity_local_offset(la, p)) * kWave + l);
+    }
+  }
+  std::vector<int4> gm(nblocks);
+  for (int64_t i = 0; i < nblocks; ++i) {
+    const int64_t g = order ? order[i] : i;
+    gm[i] = make_int4(bp[g], bp[g + 1] - bp[g], (int)start[i], (int)start[i + 1]);
+  }
+  // per-combination tables of the upper triangle
+  const int V = n_variants(p), NP = N * (N + 1) / 2, nc = (int)combos.size();
+  std::vector<double> T((size_t)V * V * 12);
+  int rc = pg_tables_host(p, T.data(), (int64_t)T.size());
+  if (rc != PG_OK) return rc;
+  std::vector<double> tab((size_t)nc * NP * 12);
+  std::vector<int> var(N);
+  for (int c = 0; c < nc; ++c) {
+    for (int la = 0; la < nle; ++la)
+      for (int k = 0; k < entity_ndof(la, p); ++k)
+        var[entity_local_offset(la, p) + k] =
+            entity_variant_base(la, orient_of(combos[c], la), p) + k;
+    int k = 0;
+    for (int i = 0; i < N; ++i)
+      for (int j = i; j < N; ++j, ++k)
+        for (int q = 0; q < 12; ++q)
+          tab[((size_t)c * NP + k) * 12 + q] = T[((size_t)var[i] * V + var[j]) * 12 + q];
+  }
+  std::vector<float> tab32(tab.begin(), tab.end());
+
+  ElemOp* op = new ElemOp();
+  op->p = p;
+  op->N = N;
+  op->D = maxd;
+  op->E = E;
+  op->ndof = ndof;
+  op->nent = nblocks;
+  op->nw = nw;
+  op->ncombo = nc;
+  {
+    int64_t g = (nblocks + eop::kBS - 1) / eop::kBS;
+    g = std::min<int64_t>(g, eop::kMaxGrid);
+    if (g >= 64) g = (g / 8) * 8;
+    op->grid = (int)std::max<int64_t>(g, 1);
+  }
+#define PG_A(ptr, cnt)                            \
+  if ((rc = op->alloc(&(ptr), (cnt))) != PG_OK) { \
+    delete op;                                    \
+    return rc;                                    \
+  }
+  PG_A(op->wcombo, nw);
+  PG_A(op->edofs, hed.size());
+  PG_A(op->perm, nslot);
+  PG_A(op->gm, nblocks);
+  PG_A(op->gslot, gslot.size());
+  PG_A(op->coef64, nslot * 12);
+  PG_A(op->coef32, nslot * 12);
+  PG_A(op->tab64, tab.size());
+  PG_A(op->tab32, tab.size());
+  PG_A(op->ye64, nslot * N);
+  PG_A(op->ye32, nslot * N);
+#undef PG_A
+  const struct {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  } up[] = {{op->wcombo, wcombo.data(), wcombo.size() * sizeof(int32_t)},
+            {op->edofs, hed.data(), hed.size() * sizeof(int32_t)},
+            {op->perm, perm.data(), perm.size() * sizeof(int32_t)},
+            {op->gm, gm.data(), gm.size() * sizeof(int4)},
+            {op->gslot, gslot.data(), gslot.size() * sizeof(int32_t)},
+            {op->tab64, tab.data(), tab.size() * sizeof(double)},
+            {op->tab32, tab32.data(), tab32.size() * sizeof(float)}};
+  for (const auto& u : up)
+    if (u.bytes && hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      delete op;
+      PG_FAIL(PG_ERR_HIP, "element operator upload failed");
+    }
+  if ((rc = eop_refresh(op, ws, omega, st)) != PG_OK) {
+    delete op;
+    return rc;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    delete op;
+    PG_FAIL(PG_ERR_HIP, "element operator setup failed");
+  }
+  *out = op;
+  return PG_OK;
+}
+
+int pg_eop_refresh(void* op, const void* ws, double omega, void* stream) {
+  if (!op || !ws) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  return eop_refresh(static_cast<ElemOp*>(op), ws, omega, as_stream(stream));
+}
+
+int pg_eop_apply_z(const void* op, const double* x, double* y, void* stream) {
+  if (!op || !x || !y) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  eop_launch(static_cast<const ElemOp*>(op), 0, nullptr, reinterpret_cast<const double2*>(x),
+             reinterpret_cast<double2*>(y), nullptr, nullptr, as_stream(stream));
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_eop_apply_c(const void* op, const float* x, float* y, void* stream) {
+  if (!op || !x || !y) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  eop_launch(static_cast<const ElemOp*>(op), 0, nullptr, reinterpret_cast<const float2*>(x),
+             reinterpret_cast<float2*>(y), nullptr, nullptr, as_stream(stream));
+  PG_CHECK_LAUNCH();
+  return PG_OK;
+}
+
+int pg_eop_destroy(void* op) {
+  delete static_cast<ElemOp*>(op);
+  return PG_OK;
+}
+
+}  // extern "C"

@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "pg_common.h"
+#include "pg_eop.h"
 
 namespace pg {
 
@@ -949,7 +950,8 @@ static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) 
 // SpMV configurations and dispatch
 // ---------------------------------------------------------------------------
 struct SpmvCfg {
-  int kind = 0;  // 0 = row, 1 = entity-row, 2 = entity-row with column pairs
+  int kind = 0;  // 0 = row, 1 = entity-row, 2 = entity-row with column pairs,
+                 // 3 = element-by-element operator
   int G = 16, U = 2, D = 1;
   int grid = 1;
 };
@@ -1044,6 +1046,8 @@ struct Solver {
   const double2* vals = nullptr;  // caller's complex128 values
   float2* vals32 = nullptr;       // owned complex64 copy (mixed)
   SpmvCfg c64, c32;               // chosen SpMV kernels
+  SpmvCfg cres;                   // CSR kernel of the FP64 true residual
+  const ElemOp* eop = nullptr;    // caller's element-by-element operator (optional)
   int gv = 1;
   Scal* s = nullptr;
   Scal* h = nullptr;  // pinned host mirror
@@ -1116,7 +1120,9 @@ static inline const float2* sched_values(Solver* S, const float2*) { return S->v
 template <int MODE, typename VT, typename XT>
 static void spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x, XT* y,
                  const double2* b) {
-  if (c.kind >= 1)
+  if (c.kind == 3)
+    eop_launch(S->eop, MODE, MODE != 0 ? &S->s->done : nullptr, x, y, b, S->part, st);
+  else if (c.kind >= 1)
     spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->emeta, S->rowptr,
                               c.kind == 2 ? S->cols2 : S->cols_s, sched_values(S, v), x, y, b,
                               S->part);
@@ -1125,10 +1131,11 @@ static void spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const
                               b, S->part);
 }
 
-// r64 = b - A x (FP64) and |r64|^2 into s->res64
+// r64 = b - A x (FP64) and |r64|^2 into s->res64, always with the assembled
+// CSR matrix (the system whose residual the tolerance refers to)
 static int true_residual(Solver* S, hipStream_t st, const double2* b, const double2* x) {
-  spmv<2, double2, double2>(S, S->c64, st, S->vals, x, S->r64, b);
-  hipLaunchKernelGGL(k_fin_res64, dim3(1), dim3(kBS), 0, st, S->s, S->part, S->c64.grid);
+  spmv<2, double2, double2>(S, S->cres, st, S->vals, x, S->r64, b);
+  hipLaunchKernelGGL(k_fin_res64, dim3(1), dim3(kBS), 0, st, S->s, S->part, S->cres.grid);
   PG_CHECK_LAUNCH();
   return PG_OK;
 }
@@ -1507,7 +1514,7 @@ static int refresh_values(Solver* S, hipStream_t st) {
 // descriptors, column lists and value blocks in schedule order.
 static int setup_entities(Solver* S, const int32_t* bp, const int32_t* order, hipStream_t st) {
   const int64_t nb = S->nblocks;
-  if (S->o.spmv == PG_SPMV_ROW) {  // CSR-row kernels only: no schedule copies
+  if (S->o.spmv == PG_SPMV_ROW || S->o.spmv == PG_SPMV_ELEMENT) {  // no schedule copies
     S->er_ok = false;
     return PG_OK;
   }
@@ -1577,6 +1584,7 @@ static int autotune(Solver* S, hipStream_t st) {
     PG_FAIL(PG_ERR_ARG, "entity-row SpMV needs block_ptr ranges whose rows share one column list");
   if (S->o.spmv == PG_SPMV_PAIRS && !S->pairs_ok)
     PG_FAIL(PG_ERR_ARG, "the column-pair SpMV needs every entity to carry two DOFs (p = 2)");
+  const bool only_eop = S->o.spmv == PG_SPMV_ELEMENT;
   std::vector<SpmvCfg> cands;
   const int gu[6][2] = {{8, 4}, {16, 2}, {16, 4}, {32, 1}, {32, 2}, {64, 1}};
   for (const auto& g : gu) {
@@ -1585,8 +1593,8 @@ static int autotune(Solver* S, hipStream_t st) {
     c.G = g[0];
     c.U = g[1];
     c.grid = grid_for(S->n, c.G);
-    if (S->o.spmv == PG_SPMV_AUTO || S->o.spmv == PG_SPMV_ROW) cands.push_back(c);
-    if (S->er_ok && S->o.spmv != PG_SPMV_ROW) {
+    if (S->o.spmv == PG_SPMV_AUTO || S->o.spmv == PG_SPMV_ROW || only_eop) cands.push_back(c);
+    if (S->er_ok && S->o.spmv != PG_SPMV_ROW && !only_eop) {
       SpmvCfg e = c;
       e.kind = 1;
       e.D = S->maxblk <= 1 ? 1 : (S->maxblk <= 2 ? 2 : 6);
@@ -1602,15 +1610,39 @@ static int autotune(Solver* S, hipStream_t st) {
   // scalars cleared so that the MODE 1 kernels run to completion
   hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(64), 0, st, S->s, 1, 0.0);
   hipLaunchKernelGGL(k_ones, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->r64);
-  S->c64 = pick_spmv<double2, double2>(S, cands, st, S->vals, S->r64, S->q);
+  // the CSR kernels: the true residual's, and the candidates against the
+  // element-by-element operator (kind 3), which is tried when the caller gave one
+  S->cres = pick_spmv<double2, double2>(S, cands, st, S->vals, S->r64, S->q);
+  SpmvCfg ce;
+  ce.kind = 3;
+  ce.G = ce.U = 0;
+  if (S->eop) ce.grid = eop_grid(S->eop);
+  S->c64 = S->cres;
+  if (only_eop) {
+    S->c64 = ce;
+  } else if (S->eop && S->o.spmv == PG_SPMV_AUTO) {
+    const float t_csr = time_spmv<double2, double2>(S, S->cres, st, S->vals, S->r64, S->q, 20);
+    const float t_eop = time_spmv<double2, double2>(S, ce, st, S->vals, S->r64, S->q, 20);
+    if (t_eop < t_csr) S->c64 = ce;
+  }
   if (mixed) {
-    // the complex64 copy has one layout: schedule order when entity rows exist
-    std::vector<SpmvCfg> c32;
-    for (const SpmvCfg& c : cands)
-      if ((c.kind >= 1) == S->er_ok) c32.push_back(c);
-    if (c32.empty()) PG_FAIL(PG_ERR_ARG, "no SpMV kernel for the complex64 layout");
     hipLaunchKernelGGL(k_ones32, dim3(nblk(S->n, kBS)), dim3(kBS), 0, st, S->n, S->p32);
-    S->c32 = pick_spmv<float2, float2>(S, c32, st, S->vals32, S->p32, S->q32);
+    if (only_eop) {
+      S->c32 = ce;
+    } else {
+      // the complex64 copy has one layout: schedule order when entity rows exist
+      std::vector<SpmvCfg> c32;
+      for (const SpmvCfg& c : cands)
+        if ((c.kind >= 1) == S->er_ok) c32.push_back(c);
+      if (c32.empty()) PG_FAIL(PG_ERR_ARG, "no SpMV kernel for the complex64 layout");
+      S->c32 = pick_spmv<float2, float2>(S, c32, st, S->vals32, S->p32, S->q32);
+      if (S->eop && S->o.spmv == PG_SPMV_AUTO) {
+        const float t_csr =
+            time_spmv<float2, float2>(S, S->c32, st, S->vals32, S->p32, S->q32, 20);
+        const float t_eop = time_spmv<float2, float2>(S, ce, st, S->vals32, S->p32, S->q32, 20);
+        if (t_eop < t_csr) S->c32 = ce;
+      }
+    }
   }
   PG_CHECK_LAUNCH();
   return PG_OK;
@@ -1673,8 +1705,12 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   if (opts->pc != PG_PC_NONE && opts->pc != PG_PC_JACOBI && opts->pc != PG_PC_BJACOBI)
     PG_FAIL(PG_ERR_UNSUPPORTED, "pc type %d not supported", opts->pc);
   if (!(opts->rtol > 0.0)) PG_FAIL(PG_ERR_ARG, "rtol must be positive");
-  if (opts->spmv < PG_SPMV_AUTO || opts->spmv > PG_SPMV_PAIRS)
+  if (opts->spmv < PG_SPMV_AUTO || opts->spmv > PG_SPMV_ELEMENT)
     PG_FAIL(PG_ERR_ARG, "spmv kind %d not supported", opts->spmv);
+  if (opts->spmv == PG_SPMV_ELEMENT && !opts->eop)
+    PG_FAIL(PG_ERR_ARG, "PG_SPMV_ELEMENT needs an element operator (pg_eop_create)");
+  if (opts->eop && eop_ndof(static_cast<const ElemOp*>(opts->eop)) != n)
+    PG_FAIL(PG_ERR_ARG, "the element operator has a different number of DOFs");
   std::vector<int32_t> hboff;
   int maxblk = 0;
   if (opts->pc == PG_PC_BJACOBI && (!opts->block_ptr || opts->nblocks <= 0))
@@ -1700,6 +1736,7 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   S->o = *opts;
   S->o.block_ptr = nullptr;  // the host pointers are not kept
   S->o.entity_order = nullptr;
+  S->eop = static_cast<const ElemOp*>(opts->eop);
   if (S->o.maxit <= 0) S->o.maxit = 100000;
   if (S->o.check_every <= 0) S->o.check_every = 25;
   if (!(S->o.inner_rtol > 0.0) || S->o.inner_rtol >= 1.0) S->o.inner_rtol = 1e-4;

@@ -4,7 +4,8 @@ Runs each hot kernel a fixed number of times on one workload, so that a
 kernel-trace or hardware-counter profiler sees a short, regular program:
 
 * the solve's SpMV (``q = A p`` plus the ``p^T q`` partials) in complex128 and
-  complex64, with the configuration the solver's autotuner picked;
+  complex64, with the CSR configuration the solver's autotuner picked, and
+  (p = 1, 2) the element-by-element operator's two kernels doing the same;
 * the numeric assembly (element geometry + fused CSR assembly);
 * a plain device copy of ``--copy-mib`` MiB (torch's copy kernel), whose
   counter readings calibrate the HBM byte counters (bytes read and written
@@ -26,7 +27,10 @@ def main(argv=None):
     ap.add_argument("--config", default="c2")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--pc", default="jacobi", choices=["jacobi", "bjacobi", "none"])
-    ap.add_argument("--spmv", default="auto", choices=["auto", "row", "entity"])
+    ap.add_argument("--spmv", default="auto", choices=["auto", "row", "entity", "entity_pairs"],
+                    help="CSR kernel family")
+    ap.add_argument("--no-eop", action="store_true",
+                    help="skip the element-by-element operator")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--natural-order", action="store_true",
                     help="entity-row SpMV in natural entity order (default: Morton order)")
@@ -48,7 +52,7 @@ def main(argv=None):
     asm = core.Assembler(mesh)
     plan = asm.plan
     sigma = torch.from_numpy(w.sigma).to(dev)
-    _, dof_bc = mesh.entity_mask(T.boundary_dofs)
+    ent_bc, dof_bc = mesh.entity_mask(T.boundary_dofs)
     omega = core.omega_of(w.frequency)
     values = torch.empty((plan.nnz, 2), dtype=torch.float64, device=dev)
     asm.geometry(sigma)
@@ -57,15 +61,21 @@ def main(argv=None):
     out = dict(config=args.config, ndof=plan.ndof, nnz=plan.nnz, reps=args.reps)
     bptr = mesh.block_ptr()
     order = None if args.natural_order else mesh.entity_order()
-    for prec in ("fp64", "mixed"):
-        S = core.KrylovSolver(plan, values, precision=prec, pc=args.pc, spmv=args.spmv,
-                              block_ptr=bptr, entity_order=order)
-        prof = S.profile(reps=args.reps)
-        if prec == "fp64":
-            out["spmv_c128_ms"], out["spmv_c128_cfg"] = prof["ms64"], prof["g64"]
-        else:
-            out["spmv_c64_ms"], out["spmv_c64_cfg"] = prof["ms32"], prof["g32"]
-        del S
+    eop = None
+    if not args.no_eop and T.p <= 2:
+        eop = core.ElementOperator(asm, omega, ent_bc, order)
+    kinds = [("spmv", args.spmv)] + ([("eop", "element")] if eop is not None else [])
+    for key, kind in kinds:
+        for prec in ("fp64", "mixed"):
+            S = core.KrylovSolver(plan, values, precision=prec, pc=args.pc, spmv=kind,
+                                  block_ptr=bptr, entity_order=order,
+                                  eop=eop if kind == "element" else None)
+            prof = S.profile(reps=args.reps)
+            if prec == "fp64":
+                out[f"{key}_c128_ms"], out[f"{key}_c128_cfg"] = prof["ms64"], prof["g64"]
+            else:
+                out[f"{key}_c64_ms"], out[f"{key}_c64_cfg"] = prof["ms32"], prof["g32"]
+            del S
     torch.cuda.synchronize()
 
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

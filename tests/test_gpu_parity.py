@@ -246,9 +246,10 @@ def _forward_gpu(w, precision="fp64", rtol=1e-8, ksp="cocg", pc="jacobi", spmv="
     b = core.source_vector(mesh, el, w.source["position"], core.dipole_moment(w.source), omega)
     core.apply_dirichlet(asm.plan, vals, dof_bc, b)
     order = None if spmv == "entity_natural" else mesh.entity_order()
+    eop = core.ElementOperator(asm, omega, ent_bc, order) if spmv == "element" else None
     solver = core.KrylovSolver(asm.plan, vals, ksp=ksp, precision=precision, rtol=rtol, pc=pc,
                                spmv="entity" if spmv == "entity_natural" else spmv,
-                               block_ptr=mesh.block_ptr(), entity_order=order)
+                               block_ptr=mesh.block_ptr(), entity_order=order, eop=eop)
     x, info = solver.solve(b)
     rx_el = core.locate_points(mesh, w.receivers)
     f = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, x, omega))
@@ -270,8 +271,12 @@ def _oracle_forward(w):
                                                ("fp64", "bjacobi", "entity"),
                                                ("mixed", "jacobi", "entity"),
                                                ("mixed", "bjacobi", "row"),
-                                               ("mixed", "bjacobi", "entity_natural")])
+                                               ("mixed", "bjacobi", "entity_natural"),
+                                               ("fp64", "bjacobi", "element"),
+                                               ("mixed", "bjacobi", "element")])
 def test_forward_solve_vs_superlu(p, n, precision, pc, spmv):
+    if spmv == "element" and p == 3:
+        pytest.skip("the element-by-element operator covers p = 1, 2")
     w = Workload(n, p, noise=0.1, seed=7, n_rx=21)
     g = _forward_gpu(w, precision=precision, pc=pc, spmv=spmv)
     r = _oracle_forward(w)
