@@ -249,6 +249,8 @@ typedef struct pg_solve_info {
                                solve (HIP events around one launch per host check) */
   int32_t spmv_samples;     /* launches sampled for spmv_ms */
   int32_t pad2;
+  double spmv_first_ms;     /* element-by-element operator: mean duration of its first
+                               kernel (the element products) within spmv_ms; else 0 */
 } pg_solve_info;
 
 /* Create a solver for an n x n complex symmetric CSR matrix on the device

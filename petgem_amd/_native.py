@@ -38,7 +38,8 @@ class SolveOpts(ctypes.Structure):
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", _I32), ("refinements", _I32), ("status", _I32),
                 ("fp64_fallback", _I32),
-                ("relres", _D), ("spmv_ms", _D), ("spmv_samples", _I32), ("pad2", _I32)]
+                ("relres", _D), ("spmv_ms", _D), ("spmv_samples", _I32), ("pad2", _I32),
+                ("spmv_first_ms", _D)]
 
 
 SIGNATURES = {

@@ -527,7 +527,8 @@ class KrylovSolver:
                                             ctypes.byref(info), nat.stream_handle(stream)))
         return x, dict(iterations=info.iterations, refinements=info.refinements,
                        status=info.status, relres=info.relres, spmv_ms=info.spmv_ms,
-                       spmv_samples=info.spmv_samples, fp64_fallback=info.fp64_fallback)
+                       spmv_samples=info.spmv_samples, fp64_fallback=info.fp64_fallback,
+                       spmv_first_ms=info.spmv_first_ms)
 
     def __del__(self):
         try:

@@ -103,15 +103,21 @@ __global__ void __launch_bounds__(kBS) k_eop_local(int64_t nw, const int32_t* __
   const int lane = threadIdx.x % kWave;
   const int combo = __builtin_amdgcn_readfirstlane(wcombo[w]);
   const R* __restrict__ t = tab + (int64_t)combo * NP * 12;
-  C xv[N];
+  // all index and coefficient loads first, then all gathers, unconditionally
+  // (a masked DOF reads x[0] and is zeroed after), so that the wave waits for
+  // memory twice rather than once per DOF
+  int dv[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const int d = edofs[(w * N + i) * kWave + lane];
-    xv[i] = d >= 0 ? x[d] : C{0, 0};
-  }
+  for (int i = 0; i < N; ++i) dv[i] = edofs[(w * N + i) * kWave + lane];
   R cf[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) cf[c] = coef[(w * 12 + c) * kWave + lane];
+  C xv[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) xv[i] = x[dv[i] >= 0 ? dv[i] : 0];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (dv[i] < 0) xv[i] = C{0, 0};
   C y[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) y[i] = C{0, 0};
@@ -173,12 +179,33 @@ __global__ void __launch_bounds__(kBS) k_eop_sum(int64_t nent, const int32_t* __
       for (int t = 0; t < D; ++t)
         if (t < d) acc[t] = x[r0 + t];
     } else {
-      for (int k = s0; k < s1; ++k) {
-        const int base = gslot[k];
+      // two incidences per step (their loads in flight together), summed in
+      // ascending slot order
+      int k = s0;
+      for (; k + 2 <= s1; k += 2) {
+        const int b0 = gslot[k], b1 = gslot[k + 1];
+        C v0[D], v1[D];
 #pragma unroll
         for (int t = 0; t < D; ++t)
           if (t < d) {
-            const C v = ye[base + t * kWave];
+            v0[t] = ye[b0 + t * kWave];
+            v1[t] = ye[b1 + t * kWave];
+          }
+#pragma unroll
+        for (int t = 0; t < D; ++t)
+          if (t < d) {
+            acc[t].x += v0[t].x;
+            acc[t].y += v0[t].y;
+            acc[t].x += v1[t].x;
+            acc[t].y += v1[t].y;
+          }
+      }
+      if (k < s1) {
+        const int b0 = gslot[k];
+#pragma unroll
+        for (int t = 0; t < D; ++t)
+          if (t < d) {
+            const C v = ye[b0 + t * kWave];
             acc[t].x += v.x;
             acc[t].y += v.y;
           }
@@ -267,7 +294,7 @@ int64_t eop_ndof(const ElemOp* op) { return op->ndof; }
 template <typename R>
 static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
                         const typename eop::Cx<R>::T* x, typename eop::Cx<R>::T* y,
-                        const double2* b, double* part, hipStream_t st,
+                        const double2* b, double* part, hipStream_t st, hipEvent_t mid,
                         typename eop::Cx<R>::T* ye, const R* coef, const R* tab) {
   using C = typename eop::Cx<R>::T;
   const int wpb = eop::kBS / kWave;
@@ -279,6 +306,8 @@ static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
   else
     hipLaunchKernelGGL((eop::k_eop_local<2, R>), dim3((unsigned)nbl), dim3(eop::kBS), 0, st,
                        op->nw, done, op->wcombo, op->edofs, coef, tab, x, ye);
+  // a failed record only loses this timing sample (the elapsed-time query fails)
+  if (mid) (void)hipEventRecord(mid, st);
   const dim3 g(op->grid), bl(eop::kBS);
 #define PG_SUM(DD, MM)                                                                        \
   hipLaunchKernelGGL((eop::k_eop_sum<DD, MM, C>), g, bl, 0, st, op->nent, done, op->gm,       \
@@ -296,12 +325,12 @@ static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
 }
 
 void eop_launch(const ElemOp* op, int mode, const int32_t* done, const double2* x, double2* y,
-                const double2* b, double* part, hipStream_t st) {
-  launch_impl<double>(op, mode, done, x, y, b, part, st, op->ye64, op->coef64, op->tab64);
+                const double2* b, double* part, hipStream_t st, hipEvent_t mid) {
+  launch_impl<double>(op, mode, done, x, y, b, part, st, mid, op->ye64, op->coef64, op->tab64);
 }
 void eop_launch(const ElemOp* op, int mode, const int32_t* done, const float2* x, float2* y,
-                const double2* b, double* part, hipStream_t st) {
-  launch_impl<float>(op, mode, done, x, y, b, part, st, op->ye32, op->coef32, op->tab32);
+                const double2* b, double* part, hipStream_t st, hipEvent_t mid) {
+  launch_impl<float>(op, mode, done, x, y, b, part, st, mid, op->ye32, op->coef32, op->tab32);
 }
 
 static int eop_refresh(ElemOp* op, const void* ws, double omega, hipStream_t st) {

@@ -17,10 +17,11 @@ int64_t eop_ndof(const ElemOp* op);
 
 // MODE 0: y = A x.  MODE 1: y = A x and partials of x^T y (two per workgroup,
 // part[k * grid + b]).  MODE 2: y = b - A x and partials of |y|^2.  With `done`
-// non-null both kernels return at once while *done is set.
+// non-null both kernels return at once while *done is set.  A non-null `mid`
+// is recorded on `st` between the two kernels (timing of the element products).
 void eop_launch(const ElemOp* op, int mode, const int32_t* done, const double2* x, double2* y,
-                const double2* b, double* part, hipStream_t st);
+                const double2* b, double* part, hipStream_t st, hipEvent_t mid = nullptr);
 void eop_launch(const ElemOp* op, int mode, const int32_t* done, const float2* x, float2* y,
-                const double2* b, double* part, hipStream_t st);
+                const double2* b, double* part, hipStream_t st, hipEvent_t mid = nullptr);
 
 }  // namespace pg

@@ -1078,7 +1078,9 @@ struct Solver {
   double* partg = nullptr;
   double* dnorm = nullptr;
   int restart = 30;
-  hipEvent_t ev[2] = {nullptr, nullptr};  // SpMV sampling inside the solve
+  // SpMV sampling inside the solve: before, between the element operator's two
+  // kernels, after
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   std::vector<void*> owned;
 
   template <typename T>
@@ -1119,9 +1121,9 @@ static inline const float2* sched_values(Solver* S, const float2*) { return S->v
 
 template <int MODE, typename VT, typename XT>
 static void spmv(Solver* S, const SpmvCfg& c, hipStream_t st, const VT* v, const XT* x, XT* y,
-                 const double2* b) {
+                 const double2* b, hipEvent_t mid = nullptr) {
   if (c.kind == 3)
-    eop_launch(S->eop, MODE, MODE != 0 ? &S->s->done : nullptr, x, y, b, S->part, st);
+    eop_launch(S->eop, MODE, MODE != 0 ? &S->s->done : nullptr, x, y, b, S->part, st, mid);
   else if (c.kind >= 1)
     spmv_launch<MODE, VT, XT>(c, st, S->s, S->n, S->nblocks, S->emeta, S->rowptr,
                               c.kind == 2 ? S->cols2 : S->cols_s, sched_values(S, v), x, y, b,
@@ -1206,7 +1208,7 @@ static void launch_iteration(Solver* S, hipStream_t st, const SpmvCfg& c, const 
   hipLaunchKernelGGL(k_pupdate<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, z, p);
   // a failed record only loses this timing sample (the elapsed-time query fails)
   if (sample) (void)hipEventRecord(S->ev[0], st);
-  spmv<1, VT, XT>(S, c, st, vals, p, q, nullptr);
+  spmv<1, VT, XT>(S, c, st, vals, p, q, nullptr, sample && c.kind == 3 ? S->ev[2] : nullptr);
   if (sample) (void)hipEventRecord(S->ev[1], st);
   hipLaunchKernelGGL(k_fin_mu, dim3(1), dim3(kBS), 0, st, S->s, S->part, c.grid);
   int np;
@@ -1267,9 +1269,12 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
                        S->o.stall_iters > 0 ? S->o.stall_iters : kStallIters);
   }
   const int check = std::max(1, S->o.check_every);
+  double first_ms = 0.0;
+  int first_n = 0;
   while (true) {
     // the first iteration of every batch runs in full (the flags were clear
     // when the batch was enqueued): time its SpMV with events on this stream
+    const bool two_kernels = (low ? S->c32 : S->c64).kind == 3;
     for (int k = 0; k < check; ++k) {
       if (low)
         launch_iteration<float2, float2>(S, st, S->c32, S->vals32, S->x32, S->r32, S->p32, S->q32,
@@ -1285,6 +1290,10 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
       if (hipEventElapsedTime(&ms, S->ev[0], S->ev[1]) == hipSuccess) {
         spmv_ms += ms;
         ++spmv_n;
+      }
+      if (two_kernels && hipEventElapsedTime(&ms, S->ev[0], S->ev[2]) == hipSuccess) {
+        first_ms += ms;
+        ++first_n;
       }
     }
     const Scal hs = *S->h;
@@ -1331,6 +1340,7 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
   info->refinements = refinements;
   info->spmv_samples = spmv_n;
   info->spmv_ms = spmv_n ? spmv_ms / spmv_n : 0.0;
+  info->spmv_first_ms = first_n ? first_ms / first_n : 0.0;
   info->relres = std::sqrt(res64 / bnorm2);
   return PG_OK;
 }
@@ -1804,7 +1814,8 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   }
 #undef PG_A
   if (hipHostMalloc(reinterpret_cast<void**>(&S->h), sizeof(Scal), 0) != hipSuccess ||
-      hipEventCreate(&S->ev[0]) != hipSuccess || hipEventCreate(&S->ev[1]) != hipSuccess) {
+      hipEventCreate(&S->ev[0]) != hipSuccess || hipEventCreate(&S->ev[1]) != hipSuccess ||
+      hipEventCreate(&S->ev[2]) != hipSuccess) {
     delete S;
     PG_FAIL(PG_ERR_HIP, "hipHostMalloc / hipEventCreate failed");
   }

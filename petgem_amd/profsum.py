@@ -37,7 +37,7 @@ def short_name(name):
             break
     s = s[:cut]
     s = re.sub(r"HIP_vector_type<(float|double|int), (\d)u>", r"\g<1>\2", s)
-    s = re.sub(r"\bpg::(batch::)?", "", s)
+    s = re.sub(r"\bpg::((batch|eop)::)?", "", s)
     return s.replace(" ", "")
 
 

@@ -99,18 +99,25 @@ class Solver:
         out = inputSetup.output["directory_scratch"]
         Print.master("     Solving linear system")
         Timers()["SetBoundaries"].start()
-        _, dof_bc = self.mesh.entity_mask(self.boundaries)
+        ent_bc, dof_bc = self.mesh.entity_mask(self.boundaries)
         core.apply_dirichlet(self.assembler.plan, self.A, dof_bc, self.b)
         Timers()["SetBoundaries"].stop(device_sync)
         Timers()["Solver"].start()
         o = self.options
         for note in o.notes:
             Print.master(f"     solver option mapped: {note}")
+        order = self.mesh.entity_order()
+        # element-by-element products (p <= 2, PEC on whole entities, as the
+        # reference's boundary lists are): used when the autotuner finds them faster
+        bp = self.mesh.block_ptr()
+        whole = np.array_equal(np.repeat(ent_bc.cpu().numpy()[:bp.size - 1], np.diff(bp)),
+                               dof_bc.cpu().numpy())
+        eop = (core.ElementOperator(self.assembler, self.omega, ent_bc, order)
+               if self.mesh.p <= 2 and whole else None)
         solver = core.KrylovSolver(self.assembler.plan, self.A, ksp=o.ksp_type, pc=o.pc_type,
                                    precision=o.precision, rtol=o.rtol, maxit=o.max_it,
                                    restart=o.restart, inner_rtol=o.inner_rtol,
-                                   block_ptr=self.mesh.block_ptr(),
-                                   entity_order=self.mesh.entity_order())
+                                   block_ptr=bp, entity_order=order, eop=eop)
         self.x, self.info = solver.solve(self.b, self.x)
         device_sync()
         Timers()["Solver"].stop()
