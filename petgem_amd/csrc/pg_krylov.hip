@@ -550,6 +550,53 @@ __global__ void __launch_bounds__(kBS) k_update_blk(
   write_partials<3>(red, part);
 }
 
+// The same update when every block has two DOFs (p = 2: block k is DOFs 2k,
+// 2k+1 and its inverse the four entries at 4k): one 16-byte (complex64) or
+// 32-byte (complex128) access per vector and block, and no block index reads.
+template <typename XT>
+__global__ void __launch_bounds__(kBS) k_update_blk2(const Scal* __restrict__ s, int64_t nb,
+                                                     XT* __restrict__ x, XT* __restrict__ r,
+                                                     const XT* __restrict__ p,
+                                                     const XT* __restrict__ q,
+                                                     const XT* __restrict__ binv,
+                                                     XT* __restrict__ z, double* part) {
+  using V = typename PairOf<XT>::type;
+  if (s->done) return;
+  const double2 al = s->alpha;
+  const XT a = from_d<XT>(al);
+  const XT na = from_d<XT>(make_double2(-al.x, -al.y));
+  int64_t lo, hi;
+  chunk(nb, kBS, &lo, &hi);
+  double red[3] = {0.0, 0.0, 0.0};
+  V* x2 = reinterpret_cast<V*>(x);
+  V* r2 = reinterpret_cast<V*>(r);
+  V* z2 = reinterpret_cast<V*>(z);
+  const V* p2 = reinterpret_cast<const V*>(p);
+  const V* q2 = reinterpret_cast<const V*>(q);
+  const V* b2 = reinterpret_cast<const V*>(binv);
+  for (int64_t k = lo + threadIdx.x; k < hi; k += kBS) {
+    const V xv = x2[k], rv = r2[k], pv = p2[k], qv = q2[k], B0 = b2[2 * k], B1 = b2[2 * k + 1];
+    const XT xa = cfma(a, XT{pv.x, pv.y}, XT{xv.x, xv.y});
+    const XT xb = cfma(a, XT{pv.z, pv.w}, XT{xv.z, xv.w});
+    const XT ra = cfma(na, XT{qv.x, qv.y}, XT{rv.x, rv.y});
+    const XT rb = cfma(na, XT{qv.z, qv.w}, XT{rv.z, rv.w});
+    x2[k] = V{xa.x, xa.y, xb.x, xb.y};
+    r2[k] = V{ra.x, ra.y, rb.x, rb.y};
+    // z = B r, B row-major 2 x 2: B0 = (B00, B01), B1 = (B10, B11)
+    const XT za = cfma(XT{B0.z, B0.w}, rb, cfma(XT{B0.x, B0.y}, ra, czero<XT>()));
+    const XT zb = cfma(XT{B1.z, B1.w}, rb, cfma(XT{B1.x, B1.y}, ra, czero<XT>()));
+    z2[k] = V{za.x, za.y, zb.x, zb.y};
+    const double2 r0 = to_d(ra), r1 = to_d(rb), z0 = to_d(za), z1 = to_d(zb);
+    red[0] += r0.x * z0.x - r0.y * z0.y;
+    red[1] += r0.x * z0.y + r0.y * z0.x;
+    red[2] += r0.x * r0.x + r0.y * r0.y;
+    red[0] += r1.x * z1.x - r1.y * z1.y;
+    red[1] += r1.x * z1.y + r1.y * z1.x;
+    red[2] += r1.x * r1.x + r1.y * r1.y;
+  }
+  write_partials<3>(red, part);
+}
+
 // finish rho, beta, |r|^2 and the convergence / pause decision
 __global__ void __launch_bounds__(kBS) k_fin_update(Scal* __restrict__ s, const double* part,
                                                     int np) {
@@ -1059,6 +1106,7 @@ struct Solver {
   // mesh entities (DOF ranges): entity-row SpMV and entity-block Jacobi
   int64_t nblocks = 0;
   int gb = 1, maxblk = 0;
+  bool blk2 = false;  // every block two DOFs: the vectorised block update
   bool er_ok = false;
   int32_t *bptr = nullptr, *boff = nullptr;
   // entity-row SpMV in schedule order: descriptors {r0, d | L << 8, cbase, vbase},
@@ -1212,7 +1260,11 @@ static void launch_iteration(Solver* S, hipStream_t st, const SpmvCfg& c, const 
   if (sample) (void)hipEventRecord(S->ev[1], st);
   hipLaunchKernelGGL(k_fin_mu, dim3(1), dim3(kBS), 0, st, S->s, S->part, c.grid);
   int np;
-  if (S->o.pc == PG_PC_BJACOBI) {
+  if (S->o.pc == PG_PC_BJACOBI && S->blk2 && (reinterpret_cast<uintptr_t>(x) % 32) == 0) {
+    hipLaunchKernelGGL(k_update_blk2<XT>, dim3(S->gb), dim3(kBS), 0, st, S->s, S->nblocks, x, r, p,
+                       q, binv, z, S->part);
+    np = S->gb;
+  } else if (S->o.pc == PG_PC_BJACOBI) {
     hipLaunchKernelGGL(k_update_blk<XT>, dim3(S->gb), dim3(kBS), 0, st, S->s, S->nblocks, S->bptr,
                        S->boff, x, r, p, q, binv, z, S->part);
     np = S->gb;
@@ -1722,7 +1774,7 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   if (opts->eop && eop_ndof(static_cast<const ElemOp*>(opts->eop)) != n)
     PG_FAIL(PG_ERR_ARG, "the element operator has a different number of DOFs");
   std::vector<int32_t> hboff;
-  int maxblk = 0;
+  int maxblk = 0, minblk = 1 << 30;
   if (opts->pc == PG_PC_BJACOBI && (!opts->block_ptr || opts->nblocks <= 0))
     PG_FAIL(PG_ERR_ARG, "block Jacobi needs block_ptr and nblocks");
   if (opts->block_ptr && opts->nblocks > 0) {
@@ -1735,6 +1787,7 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
       const int d = bp[k + 1] - bp[k];
       if (d < 1 || d > kMaxBlock) PG_FAIL(PG_ERR_ARG, "block sizes must be in 1..%d", kMaxBlock);
       maxblk = std::max(maxblk, d);
+      minblk = std::min(minblk, d);
       if ((int64_t)hboff[k] + d * d >= (1LL << 31)) PG_FAIL(PG_ERR_ARG, "block storage too large");
       hboff[k + 1] = hboff[k] + d * d;
     }
@@ -1785,6 +1838,7 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   if (opts->block_ptr && opts->nblocks > 0) {
     S->nblocks = opts->nblocks;
     S->maxblk = maxblk;
+    S->blk2 = maxblk == 2 && minblk == 2;
     S->gb = vgrid_for(S->nblocks);
     PG_A(S->bptr, S->nblocks + 1);
     PG_A(S->boff, S->nblocks + 1);
