@@ -6,7 +6,7 @@ oracle for the device adjoint.  Built on the oracle's own forward pieces
 in ``postprocessing.py:566-614``, and
 
     g_e = sum_{s,f} -omega mu0 Im(lam_e^T M_e(sigma) x_e),
-    lam = A^-1 R^T (w conj(E - E_obs)),  w = 1/|E_obs|^2.
+    lam = A^-1 R^T (w conj(E - E_obs)),  w = 1/(|E_obs|^2 + floor^2).
 
 Pinned independently of that formula by central finite differences of the
 misfit itself (``tests/test_oracle_adjoint.py``).
@@ -21,8 +21,9 @@ MU0 = fem.MU0
 
 class OracleInversion:
     def __init__(self, topo, source_positions, frequencies, receivers, observed=None,
-                 moment=(1.0, 0.0, 0.0)):
+                 moment=(1.0, 0.0, 0.0), noise_floor=0.0):
         self.T = topo
+        self.noise_floor = float(noise_floor)
         T = topo
         self.eo, self.fo = otopo.orientations(T.elems_nodes, T.elems_edges, T.edges_nodes,
                                               T.elem_faces_edges)
@@ -70,14 +71,18 @@ class OracleInversion:
             out[si, fi] = self.fields(x)
         return out
 
+    def weights(self):
+        # relative data errors with an absolute floor: 1 / (|E_obs|^2 + floor^2)
+        return 1.0 / np.maximum(np.abs(self.observed) ** 2 + self.noise_floor ** 2, 1e-300)
+
     def misfit(self, fields):
-        w = 1.0 / np.maximum(np.abs(self.observed) ** 2, 1e-300)
+        w = self.weights()
         return 0.5 * float(np.sum(w * np.abs(fields - self.observed) ** 2))
 
     def misfit_and_gradient(self, log_sigma):
         T = self.T
         M, sols = self._solve_all(log_sigma)
-        w = 1.0 / np.maximum(np.abs(self.observed) ** 2, 1e-300)
+        w = self.weights()
         S, F = self.sources.shape[0], len(self.freqs)
         fields = np.zeros((S, F, self.rx.shape[0], 3), dtype=np.complex128)
         g = np.zeros(T.n_elems)

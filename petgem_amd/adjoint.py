@@ -5,7 +5,12 @@ sources s and frequencies f, with A_f x_sf = b_sf the PEC-constrained system
 of the reference's discretisation and E_sf = R x_sf the receiver fields
 (``postprocessing.py:566-614``), the misfit is
 
-    Phi = 1/2 sum_{s,f,r,c} w_{sfrc} |E_sfrc - E^obs_sfrc|^2,  w = 1/|E^obs|^2,
+    Phi = 1/2 sum_{s,f,r,c} w_{sfrc} |E_sfrc - E^obs_sfrc|^2,
+    w = 1/(|E^obs|^2 + floor^2)
+
+(relative data errors with an optional absolute noise floor; floor = 0 gives
+purely relative weights, which make the misfit as sensitive to the solver
+tolerance as the weakest observed field is small),
 
 and, since A_f = A_f^T (complex symmetric), one adjoint solve per (s,f) with
 the same operator gives
@@ -52,8 +57,9 @@ class InversionProblem:
 
     def __init__(self, topo, source_positions, frequencies, receivers, observed=None,
                  rtol=1e-10, precision="fp64", moment=(1.0, 0.0, 0.0), device=None,
-                 batched=False, max_batch=64):
+                 batched=False, max_batch=64, noise_floor=0.0):
         self.topo = topo
+        self.noise_floor = float(noise_floor)
         self.mesh = core.DeviceMesh.from_topology(topo, device)
         self.asm = core.Assembler(self.mesh)
         self.ent_bc, self.dof_bc = self.mesh.entity_mask(topo.boundary_dofs)
@@ -83,6 +89,9 @@ class InversionProblem:
             self.values = [torch.empty((nnz, 2), dtype=torch.float64, device=self.mesh.device)
                            for _ in self.freqs]
             self.solvers = [None] * len(self.freqs)
+            # element-by-element operators (p <= 2), one per frequency: the
+            # solvers' autotuners use them when they are the faster SpMV
+            self.eops = [None] * len(self.freqs)
         self.observed = None if observed is None else np.asarray(observed)
         self.x = None
         self.stats = dict(solves=0, iterations=0)
@@ -108,15 +117,22 @@ class InversionProblem:
                     self.batches[ci].refresh()
             return
         for fi, f in enumerate(self.freqs):
-            self.asm.assemble(core.omega_of(f), ent_bc=self.ent_bc, out=self.values[fi])
+            om = core.omega_of(f)
+            self.asm.assemble(om, ent_bc=self.ent_bc, out=self.values[fi])
             if self.solvers[fi] is None:
-                # COCG + entity-block Jacobi with the entity-row SpMV in Morton order
+                # COCG + entity-block Jacobi; the SpMV is the faster of the
+                # entity-row CSR kernels (Morton order) and the element products
+                order = self.mesh.entity_order()
+                if self.mesh.p <= 2:
+                    self.eops[fi] = core.ElementOperator(self.asm, om, self.ent_bc, order)
                 self.solvers[fi] = core.KrylovSolver(self.asm.plan, self.values[fi],
                                                      precision=self.precision, rtol=self.rtol,
                                                      pc="bjacobi",
                                                      block_ptr=self.mesh.block_ptr(),
-                                                     entity_order=self.mesh.entity_order())
+                                                     entity_order=order, eop=self.eops[fi])
             else:
+                if self.eops[fi] is not None:
+                    self.eops[fi].refresh(om)
                 self.solvers[fi].refresh()
 
     def _solve(self, fi, b):
@@ -196,10 +212,13 @@ class InversionProblem:
                 out[si, fi] = core.as_complex(fld)[:, 0:3]
         return out
 
+    def weights(self):
+        """w = 1 / (|E_obs|^2 + floor^2), per observed component."""
+        return 1.0 / np.maximum(np.abs(self.observed) ** 2 + self.noise_floor ** 2, 1e-300)
+
     def misfit(self, fields):
         d = fields - self.observed
-        w = 1.0 / np.maximum(np.abs(self.observed) ** 2, 1e-300)
-        return 0.5 * float(np.sum(w * np.abs(d) ** 2))
+        return 0.5 * float(np.sum(self.weights() * np.abs(d) ** 2))
 
     # --------------------------------------------------------------- gradient
     def misfit_and_gradient(self, log_sigma):
@@ -209,7 +228,7 @@ class InversionProblem:
         phi = self.misfit(fields)
         m = self.mesh
         g = torch.zeros(m.n_elems, dtype=torch.float64, device=m.device)
-        w = 1.0 / np.maximum(np.abs(self.observed) ** 2, 1e-300)
+        w = self.weights()
         L = nat.lib()
 
         def adjoint_source(si, fi):

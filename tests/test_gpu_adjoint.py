@@ -11,8 +11,8 @@ from petgem_amd.synthetic import Workload, receiver_line  # noqa: E402
 from oracle.adjoint import OracleInversion  # noqa: E402
 
 
-@pytest.mark.parametrize("p", [1, 2])
-def test_device_gradient_vs_oracle_and_fd(p):
+@pytest.mark.parametrize("p,floor", [(1, 0.0), (2, 0.0), (2, 0.3)])
+def test_device_gradient_vs_oracle_and_fd(p, floor):
     w = Workload(5, p, block=False)
     T = w.topo
     src = np.array([[0.0, 0.0, -950.0], [700.0, 300.0, -1200.0]])
@@ -22,9 +22,10 @@ def test_device_gradient_vs_oracle_and_fd(p):
     m0 = np.log(w.sigma[:, 0])
     ora = OracleInversion(T, src, freqs, rx)
     ora.observed = ora.forward(true)
+    ora.noise_floor = floor * np.abs(ora.observed).max()
     phi_ref, g_ref = ora.misfit_and_gradient(m0)
 
-    prob = InversionProblem(T, src, freqs, rx, rtol=1e-12)
+    prob = InversionProblem(T, src, freqs, rx, rtol=1e-12, noise_floor=ora.noise_floor)
     obs = prob.forward(true)
     assert np.abs(obs - ora.observed).max() <= 1e-8 * np.abs(ora.observed).max()
     prob.observed = obs

@@ -8,8 +8,10 @@ from oracle.adjoint import OracleInversion
 from petgem_amd.synthetic import Workload, receiver_line
 
 
-@pytest.mark.parametrize("p", [1, 2])
-def test_oracle_gradient_matches_central_fd(p):
+@pytest.mark.parametrize("p,floor", [(1, 0.0), (2, 0.0), (2, 0.3)])
+def test_oracle_gradient_matches_central_fd(p, floor):
+    """floor: absolute noise floor of the weights, as a fraction of the largest
+    observed field (0.3 changes most weights substantially)."""
     w = Workload(4, p, block=False)
     T = w.topo
     src = np.array([[0.0, 0.0, -950.0], [700.0, 300.0, -1200.0]])
@@ -17,6 +19,7 @@ def test_oracle_gradient_matches_central_fd(p):
     true = np.log(w.sigma[:, 0]) + 0.3 * np.random.default_rng(1).standard_normal(T.n_elems)
     inv = OracleInversion(T, src, [0.5, 1.0], rx)
     inv.observed = inv.forward(true)
+    inv.noise_floor = floor * np.abs(inv.observed).max()
     m0 = np.log(w.sigma[:, 0])
     phi, g = inv.misfit_and_gradient(m0)
     assert phi > 0
