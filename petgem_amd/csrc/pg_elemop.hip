@@ -31,6 +31,7 @@
 #include "pg_basis.h"
 #include "pg_common.h"
 #include "pg_eop.h"
+#include "pg_eop_mask.h"
 
 namespace pg {
 
@@ -58,6 +59,21 @@ struct Cx<float> {
   using T = float2;
 };
 
+// table components kept per orientation combination: those that are not
+// structural zeros (zero_mask); the kernel skips the zeros at compile time
+constexpr int popcount12(unsigned m) {
+  int n = 0;
+  for (int c = 0; c < 12; ++c) n += (m >> c) & 1;
+  return n;
+}
+template <int P>
+constexpr int n_table_entries() {
+  constexpr int N = P * (P + 2) * (P + 3) / 2;
+  int n = 0;
+  for (int k = 0; k < N * (N + 1) / 2; ++k) n += 12 - popcount12(zero_mask<P>(k));
+  return n;
+}
+
 __device__ inline double2 to_d(double2 v) { return v; }
 __device__ inline double2 to_d(float2 v) { return make_double2(v.x, v.y); }
 __device__ inline void store(double2* p, double2 v) { *p = v; }
@@ -84,10 +100,10 @@ __device__ inline void write_partials(double (&v)[NV], double* part) {
 
 // ---------------------------------------------------------------------------
 // local products.  coef[w][c][lane]: c < 6 the mass coefficients times
-// -omega mu0, c >= 6 the curl-curl coefficients; tab[combo][k][c] the table
-// entries of pair k = (i, j >= i) (row-major upper triangle), c < 6 mass,
-// c >= 6 curl-curl.  edofs[w][i][lane] = global DOF, -1 for a PEC DOF or an
-// empty lane (its x is taken as 0).
+// -omega mu0, c >= 6 the curl-curl coefficients; tab[combo][...] the table
+// components of the pairs k = (i, j >= i) (row-major upper triangle; per pair
+// c < 6 mass, c >= 6 curl-curl), structural zeros left out.  edofs[w][i][lane]
+// = global DOF, -1 for a PEC DOF or an empty lane (its x is taken as 0).
 // ---------------------------------------------------------------------------
 template <int P, typename R>
 __global__ void __launch_bounds__(local_bs<R>()) k_eop_local(int64_t nw, const int32_t* __restrict__ done,
@@ -99,7 +115,7 @@ __global__ void __launch_bounds__(local_bs<R>()) k_eop_local(int64_t nw, const i
                                                    typename Cx<R>::T* __restrict__ ye) {
   using C = typename Cx<R>::T;
   constexpr int N = P * (P + 2) * (P + 3) / 2;
-  constexpr int NP = N * (N + 1) / 2;
+  constexpr int NT = n_table_entries<P>();
   if (done && *done) return;
   // XCD-contiguous ranges of waves (workgroup b runs on XCD b % 8)
   const int64_t nb = gridDim.x;
@@ -110,7 +126,7 @@ __global__ void __launch_bounds__(local_bs<R>()) k_eop_local(int64_t nw, const i
   if (w >= nw) return;
   const int lane = threadIdx.x % kWave;
   const int combo = __builtin_amdgcn_readfirstlane(wcombo[w]);
-  const R* __restrict__ t = tab + (int64_t)combo * NP * 12;
+  const R* __restrict__ t = tab + (int64_t)combo * NT;
   // all index and coefficient loads first, then all gathers, unconditionally
   // (a masked DOF reads x[0] and is zeroed after), so that the wave waits for
   // memory twice rather than once per DOF
@@ -129,23 +145,45 @@ __global__ void __launch_bounds__(local_bs<R>()) k_eop_local(int64_t nw, const i
   C y[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) y[i] = C{0, 0};
+  // fully unrolled: the masks, the presence flags and the table offset are
+  // compile-time constants, so only the nonzero terms are emitted
+  int off = 0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
 #pragma unroll
     for (int j = i; j < N; ++j) {
-      const R* tk = t + (i * N - (i * (i - 1)) / 2 + (j - i)) * 12;
-      R im = cf[0] * tk[0];
-      R re = cf[6] * tk[6];
+      const unsigned zm = zero_mask<P>(i * N - (i * (i - 1)) / 2 + (j - i));
+      R im = 0, re = 0;
+      bool has_im = false, has_re = false;
 #pragma unroll
-      for (int c = 1; c < 6; ++c) {
-        im = fma(cf[c], tk[c], im);
-        re = fma(cf[6 + c], tk[6 + c], re);
+      for (int c = 0; c < 6; ++c)
+        if (!((zm >> c) & 1)) {
+          im = has_im ? fma(cf[c], t[off], im) : cf[c] * t[off];
+          has_im = true;
+          ++off;
+        }
+#pragma unroll
+      for (int c = 6; c < 12; ++c)
+        if (!((zm >> c) & 1)) {
+          re = has_re ? fma(cf[c], t[off], re) : cf[c] * t[off];
+          has_re = true;
+          ++off;
+        }
+      if (has_re) {
+        y[i].x = fma(re, xv[j].x, y[i].x);
+        y[i].y = fma(re, xv[j].y, y[i].y);
+        if (j != i) {
+          y[j].x = fma(re, xv[i].x, y[j].x);
+          y[j].y = fma(re, xv[i].y, y[j].y);
+        }
       }
-      y[i].x = fma(re, xv[j].x, fma(-im, xv[j].y, y[i].x));
-      y[i].y = fma(re, xv[j].y, fma(im, xv[j].x, y[i].y));
-      if (j != i) {
-        y[j].x = fma(re, xv[i].x, fma(-im, xv[i].y, y[j].x));
-        y[j].y = fma(re, xv[i].y, fma(im, xv[i].x, y[j].y));
+      if (has_im) {
+        y[i].x = fma(-im, xv[j].y, y[i].x);
+        y[i].y = fma(im, xv[j].x, y[i].y);
+        if (j != i) {
+          y[j].x = fma(-im, xv[i].y, y[j].x);
+          y[j].y = fma(im, xv[i].x, y[j].y);
+        }
       }
     }
   }
@@ -496,12 +534,16 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
     const int64_t g = order ? order[i] : i;
     gm[i] = make_int4(bp[g], bp[g + 1] - bp[g], (int)start[i], (int)start[i + 1]);
   }
-  // per-combination tables of the upper triangle
-  const int V = n_variants(p), NP = N * (N + 1) / 2, nc = (int)combos.size();
+  // per-combination tables of the upper triangle without the structural zeros
+  // (which must vanish here too: a stale mask would drop real terms)
+  const int V = n_variants(p), nc = (int)combos.size();
+  const int NT = p == 1 ? eop::n_table_entries<1>() : eop::n_table_entries<2>();
   std::vector<double> T((size_t)V * V * 12);
   int rc = pg_tables_host(p, T.data(), (int64_t)T.size());
   if (rc != PG_OK) return rc;
-  std::vector<double> tab((size_t)nc * NP * 12);
+  double tmax = 0.0;
+  for (double v : T) tmax = std::max(tmax, std::fabs(v));
+  std::vector<double> tab((size_t)nc * NT);
   std::vector<int> var(N);
   for (int c = 0; c < nc; ++c) {
     for (int la = 0; la < nle; ++la)
@@ -509,10 +551,20 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
         var[entity_local_offset(la, p) + k] =
             entity_variant_base(la, orient_of(combos[c], la), p) + k;
     int k = 0;
+    size_t o = (size_t)c * NT;
     for (int i = 0; i < N; ++i)
-      for (int j = i; j < N; ++j, ++k)
-        for (int q = 0; q < 12; ++q)
-          tab[((size_t)c * NP + k) * 12 + q] = T[((size_t)var[i] * V + var[j]) * 12 + q];
+      for (int j = i; j < N; ++j, ++k) {
+        const unsigned zm = p == 1 ? eop::zero_mask<1>(k) : eop::zero_mask<2>(k);
+        for (int q = 0; q < 12; ++q) {
+          const double v = T[((size_t)var[i] * V + var[j]) * 12 + q];
+          if ((zm >> q) & 1) {
+            if (std::fabs(v) > 1e-15 * tmax)
+              PG_FAIL(PG_ERR_UNSUPPORTED, "element table entry (%d, %d, %d) is not zero", i, j, q);
+          } else {
+            tab[o++] = v;
+          }
+        }
+      }
   }
   std::vector<float> tab32(tab.begin(), tab.end());
 

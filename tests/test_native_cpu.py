@@ -162,3 +162,15 @@ def test_entity_order_is_morton_permutation():
     # the Morton key interleaves x, y, z bits (x lowest)
     assert list(_morton3(np.array([1, 0, 0, 3]), np.array([0, 1, 0, 0]),
                          np.array([0, 0, 1, 0]))) == [1, 2, 4, 9]
+
+
+def test_eop_zero_mask_header_matches_tables():
+    """The structural-zero masks compiled into the element-product kernel are the
+    ones the current tables give (over every orientation combination)."""
+    import re
+    from petgem_amd import gen_eop_mask
+    text = open(gen_eop_mask.OUT).read()
+    for p in (1, 2):
+        body = re.search(r"kZero%d\[\d+\] = \{([^}]*)\}" % p, text).group(1)
+        header = [int(x, 16) for x in re.findall(r"0x[0-9a-f]+", body)]
+        assert header == [int(v) for v in gen_eop_mask.structural_zeros(p)], p
