@@ -41,13 +41,28 @@ def spmv_bytes(nnz, n, bv):
 
 
 def eop_flops(n_elems, p):
-    """Algorithmic flops of one element-by-element product: per element, each of
-    the n(n+1)/2 upper-triangle entries formed from 6 + 6 coefficients (2 MUL +
-    10 FMA = 22 flops) and applied to x_j (8 flops) and, off the diagonal, to
-    x_i (8 more)."""
+    """Algorithmic flops of one element-by-element product.  Per element and
+    upper-triangle pair (i, j >= i): the mass part of the entry from its m
+    nonzero table components (one MUL + m-1 FMA = 2m-1 flops), the curl-curl
+    part likewise, then each nonzero part applied to x_j (2 FMA = 4 flops) and,
+    off the diagonal, to x_i (4 more).  The components that vanish for every
+    orientation (csrc/pg_eop_mask.h) are not computed and not counted."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "petgem_amd", "csrc",
+                        "pg_eop_mask.h")
+    body = re.search(r"kZero%d\[\d+\] = \{([^}]*)\}" % p, open(path).read()).group(1)
+    masks = [int(v, 16) for v in re.findall(r"0x[0-9a-f]+", body)]
     n = p * (p + 2) * (p + 3) // 2
-    npairs = n * (n + 1) // 2
-    return n_elems * (npairs * 22 + n * 8 + (npairs - n) * 16)
+    total, k = 0, 0
+    for i in range(n):
+        for j in range(i, n):
+            z = masks[k]
+            k += 1
+            for part in (z & 0x3F, (z >> 6) & 0x3F):
+                m = 6 - bin(part).count("1")
+                if m:
+                    total += 2 * m - 1 + (4 if j == i else 8)
+    return n_elems * total
 
 
 def spmv_key(code, dmax, vt):
