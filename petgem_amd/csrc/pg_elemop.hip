@@ -217,13 +217,18 @@ __global__ void __launch_bounds__(kBS) k_eop_sum(int64_t nent, const int32_t* __
   for (int64_t ent = first + threadIdx.x; ent < end; ent += stride) {
     const int4 m = gm[ent];
     const int r0 = m.x, d = m.y, s0 = m.z, s1 = m.w;
+    // x at the entity's rows (identity rows; the MODE 1 dot product), loaded
+    // before the sums so that its latency overlaps theirs
+    C xr[D];
+    const bool need_x = MODE == 1 || s0 == s1;
+#pragma unroll
+    for (int t = 0; t < D; ++t) xr[t] = (need_x && t < d) ? x[r0 + t] : C{0, 0};
     C acc[D];
 #pragma unroll
     for (int t = 0; t < D; ++t) acc[t] = C{0, 0};
     if (s0 == s1) {
 #pragma unroll
-      for (int t = 0; t < D; ++t)
-        if (t < d) acc[t] = x[r0 + t];
+      for (int t = 0; t < D; ++t) acc[t] = xr[t];
     } else {
       // two incidences per step (their loads in flight together), summed in
       // ascending slot order
@@ -269,7 +274,7 @@ __global__ void __launch_bounds__(kBS) k_eop_sum(int64_t nent, const int32_t* __
         } else {
           y[row] = acc[t];
           if (MODE == 1) {
-            const double2 pv = to_d(x[row]), qv = to_d(acc[t]);
+            const double2 pv = to_d(xr[t]), qv = to_d(acc[t]);
             red[0] += pv.x * qv.x - pv.y * qv.y;
             red[1] += pv.x * qv.y + pv.y * qv.x;
           }
