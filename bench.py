@@ -180,8 +180,11 @@ def cpu_baseline(w, plan_rowptr, plan_cols, values_c128, b_c128, iters_fp64, bud
 
 
 def batched_leg(args, log):
-    """Batched multi-RHS solve: 64 HED positions (100 m apart along x at z = -950 m)
-    x 4 frequencies on the 30^3 p=2 mesh, solved in chunks of --batch-width columns."""
+    """Many right-hand sides: --batch-positions of 64 HED positions (100 m apart
+    along x at z = -950 m) x 4 frequencies on the 30^3 p=2 mesh, solved (a) by the
+    batched multi-RHS solver in chunks of --batch-width columns and (b) one after
+    another by the single-RHS solver (mixed precision, entity-block Jacobi,
+    autotuned SpMV: the element products at p = 2).  `rhs_per_s` is the faster."""
     import torch
     from petgem_amd import core
     from petgem_amd.synthetic import CONFIGS, Workload
@@ -237,13 +240,15 @@ def batched_leg(args, log):
         iters = [int(i.max()) for i in its]
     t = float(np.median(times))
     # the same right-hand sides one after another through the single-RHS solver
-    # (mixed precision, entity-block Jacobi), one solver per frequency
+    # (mixed precision, entity-block Jacobi, element products when faster), one
+    # solver per frequency
     seq_t = 0.0
     vals = torch.empty((asm.plan.nnz, 2), dtype=torch.float64, device=dev)
     for f in freqs:
         asm.assemble(core.omega_of(f), ent_bc=ent_bc, out=vals)
+        eop = core.ElementOperator(asm, core.omega_of(f), ent_bc, order) if T.p <= 2 else None
         S = core.KrylovSolver(asm.plan, vals, precision="mixed", pc="bjacobi", rtol=args.rtol,
-                              block_ptr=bptr, entity_order=order)
+                              block_ptr=bptr, entity_order=order, eop=eop)
         for si in range(len(xs)):
             b = core.source_vector(mesh, int(src_el[si]), srcs[si], (1.0, 0.0, 0.0),
                                    core.omega_of(f))
@@ -255,10 +260,12 @@ def batched_leg(args, log):
             seq_t += time.perf_counter() - t0
             if inf["status"] != 0:
                 raise RuntimeError(f"single solve did not converge: {inf}")
-        del S
+        del S, eop
     log(f"batched: {t:.1f} s per batch; one after another (mixed, block Jacobi) {seq_t:.1f} s")
-    return {"rhs_per_s": round(len(cols) / t, 3), "s_per_batch": round(t, 4), "nrhs": len(cols),
-            "width": width, "chunk_iterations": iters,
+    best = "batched_solver" if t <= seq_t else "one_after_another"
+    return {"rhs_per_s": round(len(cols) / min(t, seq_t), 3), "mode": best,
+            "batched_solver_rhs_per_s": round(len(cols) / t, 3), "s_per_batch": round(t, 4),
+            "nrhs": len(cols), "width": width, "chunk_iterations": iters,
             "sequential_mixed_bjacobi_rhs_per_s": round(len(cols) / seq_t, 3),
             "workload": (f"c4: p=2, 30^3 Kuhn x 6 = {T.n_elems} tets, {T.ndof} DOFs, "
                          f"{asm.plan.nnz} nnz; {len(xs)} of the 64 HED x-positions (100 m apart "
