@@ -69,8 +69,8 @@ def spmv_key(code, dmax, vt):
     """Kernel name (as the profile summaries print it) of the solve's SpMV from the
     solver's configuration code kind*1000 + G*10 + U."""
     kind, g, u = code // 1000, (code % 1000) // 10, code % 10
-    if kind == 3:
-        return "k_eop_local<2,{}>".format("float" if vt == "float2" else "double")
+    if kind == 3:  # element products (p = 2: paired gathers on the solver's aligned vectors)
+        return "k_eop_local<2,{},true>".format("float" if vt == "float2" else "double")
     if kind == 2:
         return f"k_spmv_er2<{g},{u},1,{vt},{vt}>"
     if kind == 1:

@@ -46,6 +46,12 @@ template <typename R>
 constexpr int local_bs() {
   return sizeof(R) == 4 ? 64 : 256;
 }
+// waves per SIMD the element products are compiled for: five in complex64
+// (at most 102 VGPRs; the register allocator otherwise settles on 104 and four)
+template <typename R>
+constexpr int min_waves() {
+  return sizeof(R) == 4 ? 5 : 1;
+}
 constexpr double kMu0 = 4.0 * 3.14159265358979323846 * 1e-7;  // petgem/solver.py:175
 
 template <typename R>
@@ -53,11 +59,19 @@ struct Cx;
 template <>
 struct Cx<double> {
   using T = double2;
+  using T2 = double4;  // two consecutive complex values
 };
 template <>
 struct Cx<float> {
   using T = float2;
+  using T2 = float4;
 };
+
+// local entities carrying DOFs (edges, then faces at p = 2) and their DOFs
+template <int P>
+constexpr int n_local_entities() {
+  return P == 1 ? 6 : 10;
+}
 
 // table components kept per orientation combination: those that are not
 // structural zeros (zero_mask); the kernel skips the zeros at compile time
@@ -102,11 +116,15 @@ __device__ inline void write_partials(double (&v)[NV], double* part) {
 // local products.  coef[w][c][lane]: c < 6 the mass coefficients times
 // -omega mu0, c >= 6 the curl-curl coefficients; tab[combo][...] the table
 // components of the pairs k = (i, j >= i) (row-major upper triangle; per pair
-// c < 6 mass, c >= 6 curl-curl), structural zeros left out.  edofs[w][i][lane]
-// = global DOF, -1 for a PEC DOF or an empty lane (its x is taken as 0).
+// c < 6 mass, c >= 6 curl-curl), structural zeros left out.  edofs[w][la][lane]
+// = first global DOF of local entity la (its P DOFs at p = 1 and 2 are
+// consecutive: 1 at p = 1, 2 at p = 2), -1 for a PEC entity or an empty lane
+// (its x is taken as 0).  PAIR: at p = 2 both DOFs of an entity are loaded as
+// one 16-byte (complex64) or 32-byte (complex128) value; the first DOF of
+// every entity is even, so this needs x aligned to that size.
 // ---------------------------------------------------------------------------
-template <int P, typename R>
-__global__ void __launch_bounds__(local_bs<R>()) k_eop_local(int64_t nw, const int32_t* __restrict__ done,
+template <int P, typename R, bool PAIR>
+__global__ void __launch_bounds__(local_bs<R>(), min_waves<R>()) k_eop_local(int64_t nw, const int32_t* __restrict__ done,
                                                    const int32_t* __restrict__ wcombo,
                                                    const int32_t* __restrict__ edofs,
                                                    const R* __restrict__ coef,
@@ -114,8 +132,11 @@ __global__ void __launch_bounds__(local_bs<R>()) k_eop_local(int64_t nw, const i
                                                    const typename Cx<R>::T* __restrict__ x,
                                                    typename Cx<R>::T* __restrict__ ye) {
   using C = typename Cx<R>::T;
+  using C2 = typename Cx<R>::T2;
   constexpr int N = P * (P + 2) * (P + 3) / 2;
+  constexpr int NLE = n_local_entities<P>();
   constexpr int NT = n_table_entries<P>();
+  static_assert(N == NLE * P, "entity DOFs");
   if (done && *done) return;
   // XCD-contiguous ranges of waves (workgroup b runs on XCD b % 8)
   const int64_t nb = gridDim.x;
@@ -128,20 +149,32 @@ __global__ void __launch_bounds__(local_bs<R>()) k_eop_local(int64_t nw, const i
   const int combo = __builtin_amdgcn_readfirstlane(wcombo[w]);
   const R* __restrict__ t = tab + (int64_t)combo * NT;
   // all index and coefficient loads first, then all gathers, unconditionally
-  // (a masked DOF reads x[0] and is zeroed after), so that the wave waits for
-  // memory twice rather than once per DOF
-  int dv[N];
+  // (a masked entity reads x[0] and is zeroed after), so that the wave waits
+  // for memory twice rather than once per DOF
+  int fv[NLE];
 #pragma unroll
-  for (int i = 0; i < N; ++i) dv[i] = edofs[(w * N + i) * kWave + lane];
+  for (int la = 0; la < NLE; ++la) fv[la] = edofs[(w * NLE + la) * kWave + lane];
   R cf[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) cf[c] = coef[(w * 12 + c) * kWave + lane];
   C xv[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) xv[i] = x[dv[i] >= 0 ? dv[i] : 0];
+  for (int la = 0; la < NLE; ++la) {
+    const int f = fv[la] >= 0 ? fv[la] : 0;
+    if (P == 2 && PAIR) {
+      const C2 v = *reinterpret_cast<const C2*>(x + f);
+      xv[2 * la] = C{v.x, v.y};
+      xv[2 * la + 1] = C{v.z, v.w};
+    } else {
 #pragma unroll
-  for (int i = 0; i < N; ++i)
-    if (dv[i] < 0) xv[i] = C{0, 0};
+      for (int k = 0; k < P; ++k) xv[P * la + k] = x[f + k];
+    }
+  }
+#pragma unroll
+  for (int la = 0; la < NLE; ++la)
+    if (fv[la] < 0)
+#pragma unroll
+      for (int k = 0; k < P; ++k) xv[P * la + k] = C{0, 0};
   C y[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) y[i] = C{0, 0};
@@ -309,6 +342,7 @@ __global__ void k_eop_coef(int64_t nslot, const int32_t* __restrict__ perm,
 
 struct ElemOp {
   int p = 0, N = 0, D = 1;
+  bool even_entities = false;  // every entity's first DOF even: paired gathers allowed
   int64_t E = 0, ndof = 0, nent = 0, nw = 0;
   int ncombo = 0, grid = 1;
   int32_t* wcombo = nullptr;
@@ -351,12 +385,18 @@ static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
   const int wpb = eop::local_bs<R>() / kWave;
   int64_t nbl = (op->nw + wpb - 1) / wpb;
   if (nbl >= 64) nbl = (nbl + 7) / 8 * 8;
+  const dim3 gl((unsigned)nbl), bll(wpb * kWave);
+  const bool pair_ok =
+      op->even_entities && reinterpret_cast<uintptr_t>(x) % (2 * sizeof(C)) == 0;
   if (op->p == 1)
-    hipLaunchKernelGGL((eop::k_eop_local<1, R>), dim3((unsigned)nbl), dim3(wpb * kWave), 0, st,
-                       op->nw, done, op->wcombo, op->edofs, coef, tab, x, ye);
+    hipLaunchKernelGGL((eop::k_eop_local<1, R, false>), gl, bll, 0, st, op->nw, done, op->wcombo,
+                       op->edofs, coef, tab, x, ye);
+  else if (pair_ok)
+    hipLaunchKernelGGL((eop::k_eop_local<2, R, true>), gl, bll, 0, st, op->nw, done, op->wcombo,
+                       op->edofs, coef, tab, x, ye);
   else
-    hipLaunchKernelGGL((eop::k_eop_local<2, R>), dim3((unsigned)nbl), dim3(wpb * kWave), 0, st,
-                       op->nw, done, op->wcombo, op->edofs, coef, tab, x, ye);
+    hipLaunchKernelGGL((eop::k_eop_local<2, R, false>), gl, bll, 0, st, op->nw, done, op->wcombo,
+                       op->edofs, coef, tab, x, ye);
   // a failed record only loses this timing sample (the elapsed-time query fails)
   if (mid) (void)hipEventRecord(mid, st);
   const dim3 g(op->grid), bl(eop::kBS);
@@ -503,14 +543,15 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
   }
   const int64_t nw = (int64_t)wcombo.size(), nslot = nw * kWave;
   if (nslot * N >= (1LL << 31)) PG_FAIL(PG_ERR_UNSUPPORTED, "too many elements");
-  std::vector<int32_t> hed((size_t)nslot * N, -1);
+  // per slot and local entity: the entity's first DOF (-1: PEC or empty)
+  std::vector<int32_t> hed((size_t)nslot * nle, -1);
   for (int64_t s = 0; s < nslot; ++s) {
     const int64_t e = perm[s];
     if (e < 0) continue;
     const int64_t w = s / kWave, l = s % kWave;
-    for (int i = 0; i < N; ++i) {
-      const int d = hd[e * N + i];
-      hed[(w * N + i) * kWave + l] = hbc[ent_of[d]] ? -1 : d;
+    for (int la = 0; la < nle; ++la) {
+      const int g = eg[e * nle + la];
+      hed[(w * nle + la) * kWave + l] = hbc[g] ? -1 : bp[g];
     }
   }
   // per-entity lists of local-result slots, in schedule order; within an
@@ -577,6 +618,9 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
   op->p = p;
   op->N = N;
   op->D = maxd;
+  op->even_entities = true;
+  for (int64_t g = 0; g < nblocks; ++g)
+    if (bp[g] % 2) op->even_entities = false;
   op->E = E;
   op->ndof = ndof;
   op->nent = nblocks;
