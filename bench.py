@@ -65,12 +65,13 @@ def eop_flops(n_elems, p):
     return n_elems * total
 
 
-def spmv_key(code, dmax, vt):
+def spmv_key(code, dmax, vt, p=2):
     """Kernel name (as the profile summaries print it) of the solve's SpMV from the
     solver's configuration code kind*1000 + G*10 + U."""
     kind, g, u = code // 1000, (code % 1000) // 10, code % 10
     if kind == 3:  # element products (p = 2: paired gathers on the solver's aligned vectors)
-        return "k_eop_local<2,{},true>".format("float" if vt == "float2" else "double")
+        return "k_eop_local<{},{},{}>".format(p, "float" if vt == "float2" else "double",
+                                              "true" if p == 2 else "false")
     if kind == 2:
         return f"k_spmv_er2<{g},{u},1,{vt},{vt}>"
     if kind == 1:
@@ -89,7 +90,7 @@ def roofline_entry(code, res, standalone_ms, vt, T, plan, dmax, traffic):
     HBM-bound and reported beside it with its algorithmic bytes."""
     bv = 8 if vt == "float2" else 16
     n, nnz = plan.ndof, plan.nnz
-    key = spmv_key(code, dmax, vt)
+    key = spmv_key(code, dmax, vt, T.p)
     out = {"samples": res["spmv_samples"], "spmv_ms": round(res["spmv_ms"], 5),
            "spmv_ms_standalone": round(standalone_ms, 5)}
     if code // 1000 == 3:
@@ -246,7 +247,7 @@ def batched_leg(args, log):
     vals = torch.empty((asm.plan.nnz, 2), dtype=torch.float64, device=dev)
     for f in freqs:
         asm.assemble(core.omega_of(f), ent_bc=ent_bc, out=vals)
-        eop = core.ElementOperator(asm, core.omega_of(f), ent_bc, order) if T.p <= 2 else None
+        eop = core.ElementOperator(asm, core.omega_of(f), ent_bc, order)
         S = core.KrylovSolver(asm.plan, vals, precision="mixed", pc="bjacobi", rtol=args.rtol,
                               block_ptr=bptr, entity_order=order, eop=eop)
         for si in range(len(xs)):
@@ -459,8 +460,9 @@ def main():
     assemble()
     bptr = mesh.block_ptr()
     order = mesh.entity_order()
-    # element-by-element operator (p <= 2): a candidate of the solvers' autotuners
-    eop = core.ElementOperator(asm, omega, ent_bc, order) if T.p <= 2 else None
+    # element-by-element operator: a candidate of the solvers' autotuners (at
+    # p = 3 for the complex64 iterations only)
+    eop = core.ElementOperator(asm, omega, ent_bc, order)
     solvers = {prec: core.KrylovSolver(plan, values, precision=prec, rtol=args.rtol, pc=args.pc,
                                        block_ptr=bptr, entity_order=order, eop=eop)
                for prec in ("fp64", "mixed")}

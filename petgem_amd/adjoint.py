@@ -89,8 +89,8 @@ class InversionProblem:
             self.values = [torch.empty((nnz, 2), dtype=torch.float64, device=self.mesh.device)
                            for _ in self.freqs]
             self.solvers = [None] * len(self.freqs)
-            # element-by-element operators (p <= 2), one per frequency: the
-            # solvers' autotuners use them when they are the faster SpMV
+            # element-by-element operators, one per frequency: the solvers'
+            # autotuners use them when they are the faster SpMV
             self.eops = [None] * len(self.freqs)
         self.observed = None if observed is None else np.asarray(observed)
         self.x = None
@@ -123,8 +123,7 @@ class InversionProblem:
                 # COCG + entity-block Jacobi; the SpMV is the faster of the
                 # entity-row CSR kernels (Morton order) and the element products
                 order = self.mesh.entity_order()
-                if self.mesh.p <= 2:
-                    self.eops[fi] = core.ElementOperator(self.asm, om, self.ent_bc, order)
+                self.eops[fi] = core.ElementOperator(self.asm, om, self.ent_bc, order)
                 self.solvers[fi] = core.KrylovSolver(self.asm.plan, self.values[fi],
                                                      precision=self.precision, rtol=self.rtol,
                                                      pc="bjacobi",

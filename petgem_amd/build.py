@@ -20,12 +20,18 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
          "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")]
 
 
+# per-source flags: the element products at p = 3 unroll 1035 entry pairs, past
+# the default size limit of a fully unrolled loop
+EXTRA = {"pg_elemop.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]}
+
+
 def build(verbose=False, jobs=4):
     os.makedirs(BUILD, exist_ok=True)
     objs, procs = [], []
     for src in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + FLAGS + EXTRA.get(src, []) + ["-x", "hip", "-c", os.path.join(CSRC, src),
+                                                      "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

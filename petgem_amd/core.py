@@ -431,9 +431,10 @@ class ElementOperator:
     """Element-by-element product y = A x of the system Assembler.assemble builds
     (A = sum_e K_e - i omega mu0 M_e, PEC entities as identity rows), without
     the assembled matrix: the element matrices are recomputed from the geometry
-    workspace and the tables at every product.  Orders p = 1, 2.  Passed to
-    KrylovSolver(eop=...), whose iterations then use it when it is the faster
-    kernel (or always, with spmv="element")."""
+    workspace and the tables at every product.  Orders p = 1, 2 (complex128 and
+    complex64) and 3 (complex64).  Passed to KrylovSolver(eop=...), whose
+    iterations then use it when it is the faster kernel (or always, with
+    spmv="element")."""
 
     def __init__(self, assembler, omega, ent_bc=None, entity_order=None, stream=None):
         m = assembler.mesh

@@ -46,11 +46,12 @@ template <typename R>
 constexpr int local_bs() {
   return sizeof(R) == 4 ? 64 : 256;
 }
-// waves per SIMD the element products are compiled for: five in complex64
-// (at most 102 VGPRs; the register allocator otherwise settles on 104 and four)
-template <typename R>
+// waves per SIMD the element products are compiled for: five in complex64 at
+// p <= 2 (at most 102 VGPRs; the register allocator otherwise settles on 104
+// and four); at p = 3 the 45 values of p and of the products need about 190
+template <int P, typename R>
 constexpr int min_waves() {
-  return sizeof(R) == 4 ? 5 : 1;
+  return sizeof(R) == 4 && P <= 2 ? 5 : 1;
 }
 constexpr double kMu0 = 4.0 * 3.14159265358979323846 * 1e-7;  // petgem/solver.py:175
 
@@ -67,10 +68,19 @@ struct Cx<float> {
   using T2 = float4;
 };
 
-// local entities carrying DOFs (edges, then faces at p = 2) and their DOFs
+// local entities carrying DOFs (edges; faces from p = 2; the volume at p = 3),
+// their first local DOF and their number of DOFs
 template <int P>
 constexpr int n_local_entities() {
-  return P == 1 ? 6 : 10;
+  return P == 1 ? 6 : (P == 2 ? 10 : 11);
+}
+template <int P>
+constexpr int loc_off(int la) {
+  return la < 6 ? la * P : (la < 10 ? 6 * P + (la - 6) * P * (P - 1) : 6 * P + 4 * P * (P - 1));
+}
+template <int P>
+constexpr int loc_nd(int la) {
+  return la < 6 ? P : (la < 10 ? P * (P - 1) : P * (P - 1) * (P - 2) / 2);
 }
 
 // table components kept per orientation combination: those that are not
@@ -124,7 +134,7 @@ __device__ inline void write_partials(double (&v)[NV], double* part) {
 // every entity is even, so this needs x aligned to that size.
 // ---------------------------------------------------------------------------
 template <int P, typename R, bool PAIR>
-__global__ void __launch_bounds__(local_bs<R>(), min_waves<R>()) k_eop_local(int64_t nw, const int32_t* __restrict__ done,
+__global__ void __launch_bounds__(local_bs<R>(), (min_waves<P, R>())) k_eop_local(int64_t nw, const int32_t* __restrict__ done,
                                                    const int32_t* __restrict__ wcombo,
                                                    const int32_t* __restrict__ edofs,
                                                    const R* __restrict__ coef,
@@ -136,7 +146,7 @@ __global__ void __launch_bounds__(local_bs<R>(), min_waves<R>()) k_eop_local(int
   constexpr int N = P * (P + 2) * (P + 3) / 2;
   constexpr int NLE = n_local_entities<P>();
   constexpr int NT = n_table_entries<P>();
-  static_assert(N == NLE * P, "entity DOFs");
+  static_assert(N == loc_off<P>(NLE - 1) + loc_nd<P>(NLE - 1), "entity DOFs");
   if (done && *done) return;
   // XCD-contiguous ranges of waves (workgroup b runs on XCD b % 8)
   const int64_t nb = gridDim.x;
@@ -167,14 +177,14 @@ __global__ void __launch_bounds__(local_bs<R>(), min_waves<R>()) k_eop_local(int
       xv[2 * la + 1] = C{v.z, v.w};
     } else {
 #pragma unroll
-      for (int k = 0; k < P; ++k) xv[P * la + k] = x[f + k];
+      for (int k = 0; k < loc_nd<P>(la); ++k) xv[loc_off<P>(la) + k] = x[f + k];
     }
   }
 #pragma unroll
   for (int la = 0; la < NLE; ++la)
     if (fv[la] < 0)
 #pragma unroll
-      for (int k = 0; k < P; ++k) xv[P * la + k] = C{0, 0};
+      for (int k = 0; k < loc_nd<P>(la); ++k) xv[loc_off<P>(la) + k] = C{0, 0};
   C y[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) y[i] = C{0, 0};
@@ -374,6 +384,7 @@ struct ElemOp {
 };
 
 int eop_grid(const ElemOp* op) { return op->grid; }
+bool eop_supports(const ElemOp* op, bool complex128) { return !(complex128 && op->p == 3); }
 int64_t eop_ndof(const ElemOp* op) { return op->ndof; }
 
 template <typename R>
@@ -388,10 +399,16 @@ static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
   const dim3 gl((unsigned)nbl), bll(wpb * kWave);
   const bool pair_ok =
       op->even_entities && reinterpret_cast<uintptr_t>(x) % (2 * sizeof(C)) == 0;
-  if (op->p == 1)
+  if (op->p == 1) {
     hipLaunchKernelGGL((eop::k_eop_local<1, R, false>), gl, bll, 0, st, op->nw, done, op->wcombo,
                        op->edofs, coef, tab, x, ye);
-  else if (pair_ok)
+  } else if (op->p == 3) {
+    // complex64 only (the complex128 kernel would need 360 registers for p and
+    // the products); eop_supports() keeps the solver from asking for it
+    if constexpr (sizeof(R) == 4)
+      hipLaunchKernelGGL((eop::k_eop_local<3, R, false>), gl, bll, 0, st, op->nw, done,
+                         op->wcombo, op->edofs, coef, tab, x, ye);
+  } else if (pair_ok)
     hipLaunchKernelGGL((eop::k_eop_local<2, R, true>), gl, bll, 0, st, op->nw, done, op->wcombo,
                        op->edofs, coef, tab, x, ye);
   else
@@ -407,10 +424,14 @@ static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
     if (mode == 0) PG_SUM(1, 0);
     else if (mode == 1) PG_SUM(1, 1);
     else PG_SUM(1, 2);
-  } else {
+  } else if (op->D == 2) {
     if (mode == 0) PG_SUM(2, 0);
     else if (mode == 1) PG_SUM(2, 1);
     else PG_SUM(2, 2);
+  } else {
+    if (mode == 0) PG_SUM(6, 0);
+    else if (mode == 1) PG_SUM(6, 1);
+    else PG_SUM(6, 2);
   }
 #undef PG_SUM
 }
@@ -448,11 +469,11 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
                   const uint8_t* ent_bc, double omega, void** out, void* stream) {
   if (!out || !ws || !dofs || !block_ptr || E <= 0 || ndof <= 0 || nblocks <= 0)
     PG_FAIL(PG_ERR_ARG, "bad arguments");
-  if (p < 1 || p > 2)
-    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator: order p=%d not supported (1, 2)", p);
+  if (p < 1 || p > 3)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator: order p=%d not supported (1..3)", p);
   const int N = ndof_element(p);
-  const int nle = p == 1 ? 6 : 10;  // local entities carrying DOFs
-  const int maxd = p == 1 ? 1 : 2;
+  const int nle = p == 1 ? 6 : (p == 2 ? 10 : 11);  // local entities carrying DOFs
+  const int maxd = p == 1 ? 1 : (p == 2 ? 2 : 6);
   hipStream_t st = as_stream(stream);
   const int32_t* bp = block_ptr;
   if (bp[0] != 0 || bp[nblocks] != ndof)
@@ -583,7 +604,9 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
   // per-combination tables of the upper triangle without the structural zeros
   // (which must vanish here too: a stale mask would drop real terms)
   const int V = n_variants(p), nc = (int)combos.size();
-  const int NT = p == 1 ? eop::n_table_entries<1>() : eop::n_table_entries<2>();
+  const int NT = p == 1   ? eop::n_table_entries<1>()
+                 : p == 2 ? eop::n_table_entries<2>()
+                          : eop::n_table_entries<3>();
   std::vector<double> T((size_t)V * V * 12);
   int rc = pg_tables_host(p, T.data(), (int64_t)T.size());
   if (rc != PG_OK) return rc;
@@ -600,7 +623,9 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
     size_t o = (size_t)c * NT;
     for (int i = 0; i < N; ++i)
       for (int j = i; j < N; ++j, ++k) {
-        const unsigned zm = p == 1 ? eop::zero_mask<1>(k) : eop::zero_mask<2>(k);
+        const unsigned zm = p == 1   ? eop::zero_mask<1>(k)
+                            : p == 2 ? eop::zero_mask<2>(k)
+                                     : eop::zero_mask<3>(k);
         for (int q = 0; q < 12; ++q) {
           const double v = T[((size_t)var[i] * V + var[j]) * 12 + q];
           if ((zm >> q) & 1) {
@@ -684,6 +709,8 @@ int pg_eop_refresh(void* op, const void* ws, double omega, void* stream) {
 
 int pg_eop_apply_z(const void* op, const double* x, double* y, void* stream) {
   if (!op || !x || !y) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  if (!eop_supports(static_cast<const ElemOp*>(op), true))
+    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator at p=3: complex64 only");
   eop_launch(static_cast<const ElemOp*>(op), 0, nullptr, reinterpret_cast<const double2*>(x),
              reinterpret_cast<double2*>(y), nullptr, nullptr, as_stream(stream));
   PG_CHECK_LAUNCH();

@@ -14,6 +14,9 @@ struct ElemOp;
 // partials (per quantity) that MODE 1 and MODE 2 leave in `part`.
 int eop_grid(const ElemOp* op);
 int64_t eop_ndof(const ElemOp* op);
+// whether the operator has a complex128 (true) or complex64 (false) product
+// (at p = 3 only complex64)
+bool eop_supports(const ElemOp* op, bool complex128);
 
 // MODE 0: y = A x.  MODE 1: y = A x and partials of x^T y (two per workgroup,
 // part[k * grid + b]).  MODE 2: y = b - A x and partials of |y|^2.  With `done`

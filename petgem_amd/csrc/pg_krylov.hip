@@ -1680,9 +1680,14 @@ static int autotune(Solver* S, hipStream_t st) {
   ce.G = ce.U = 0;
   if (S->eop) ce.grid = eop_grid(S->eop);
   S->c64 = S->cres;
-  if (only_eop) {
+  // (an operator without a complex128 product, p = 3, serves complex64 only;
+  // the FP64 recursion then keeps the CSR kernel)
+  const bool eop64 = S->eop && eop_supports(S->eop, true);
+  if (only_eop && !eop64 && !mixed)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "the element operator at p = 3 has no complex128 product");
+  if (only_eop && eop64) {
     S->c64 = ce;
-  } else if (S->eop && S->o.spmv == PG_SPMV_AUTO) {
+  } else if (eop64 && S->o.spmv == PG_SPMV_AUTO) {
     const float t_csr = time_spmv<double2, double2>(S, S->cres, st, S->vals, S->r64, S->q, 20);
     const float t_eop = time_spmv<double2, double2>(S, ce, st, S->vals, S->r64, S->q, 20);
     if (t_eop < t_csr) S->c64 = ce;

@@ -107,13 +107,14 @@ class Solver:
         for note in o.notes:
             Print.master(f"     solver option mapped: {note}")
         order = self.mesh.entity_order()
-        # element-by-element products (p <= 2, PEC on whole entities, as the
-        # reference's boundary lists are): used when the autotuner finds them faster
+        # element-by-element products (PEC on whole entities, as the reference's
+        # boundary lists are; complex64 only at p = 3): used when the autotuner
+        # finds them faster
         bp = self.mesh.block_ptr()
         whole = np.array_equal(np.repeat(ent_bc.cpu().numpy()[:bp.size - 1], np.diff(bp)),
                                dof_bc.cpu().numpy())
         eop = (core.ElementOperator(self.assembler, self.omega, ent_bc, order)
-               if self.mesh.p <= 2 and whole else None)
+               if whole else None)
         solver = core.KrylovSolver(self.assembler.plan, self.A, ksp=o.ksp_type, pc=o.pc_type,
                                    precision=o.precision, rtol=o.rtol, maxit=o.max_it,
                                    restart=o.restart, inner_rtol=o.inner_rtol,

@@ -170,7 +170,7 @@ def test_eop_zero_mask_header_matches_tables():
     import re
     from petgem_amd import gen_eop_mask
     text = open(gen_eop_mask.OUT).read()
-    for p in (1, 2):
+    for p in (1, 2, 3):
         body = re.search(r"kZero%d\[\d+\] = \{([^}]*)\}" % p, text).group(1)
         header = [int(x, 16) for x in re.findall(r"0x[0-9a-f]+", body)]
         assert header == [int(v) for v in gen_eop_mask.structural_zeros(p)], p
@@ -183,9 +183,9 @@ def test_element_operator_rejects_bad_arguments_before_device_work():
     h = ctypes.c_void_p()
     fake = ctypes.c_void_p(256)  # stands for a device pointer; not read
     bp = np.array([0, 2, 4], np.int32)
-    rc = L.pg_eop_create(3, 1, fake, fake, 4, nat.ptr(bp), 2, None, None, 1.0, ctypes.byref(h),
+    rc = L.pg_eop_create(4, 1, fake, fake, 4, nat.ptr(bp), 2, None, None, 1.0, ctypes.byref(h),
                          None)
-    assert rc == nat.PG_ERR_UNSUPPORTED and b"p=3" in L.pg_last_error()
+    assert rc == nat.PG_ERR_UNSUPPORTED and b"p=4" in L.pg_last_error()
     bad_end = np.array([0, 2, 5], np.int32)
     rc = L.pg_eop_create(2, 1, fake, fake, 4, nat.ptr(bad_end), 2, None, None, 1.0,
                          ctypes.byref(h), None)
