@@ -18,8 +18,9 @@
 //               in a fixed order (atomic-free, bitwise reproducible), and
 //               fuses the COCG dot product or the residual.
 //
-// Orders p = 1 and 2 (n = 6, 20).  The summation order differs from the
-// assembled CSR's, so results agree with the CSR SpMV to rounding.
+// Orders p = 1, 2 and 3 (n = 6, 20, 45; p = 3 in complex64 only).  The
+// summation order differs from the assembled CSR's, so results agree with the
+// CSR SpMV to rounding.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
