@@ -57,8 +57,11 @@ class InversionProblem:
 
     def __init__(self, topo, source_positions, frequencies, receivers, observed=None,
                  rtol=1e-10, precision="fp64", moment=(1.0, 0.0, 0.0), device=None,
-                 batched=False, max_batch=64, noise_floor=0.0):
+                 batched=False, max_batch=64, noise_floor=0.0, concurrency=2):
         self.topo = topo
+        # per-RHS mode: right-hand sides solved `concurrency` at a time, each
+        # worker with its own solver, element operator and stream
+        self.concurrency = max(1, int(concurrency))
         self.noise_floor = float(noise_floor)
         self.mesh = core.DeviceMesh.from_topology(topo, device)
         self.asm = core.Assembler(self.mesh)
@@ -89,8 +92,8 @@ class InversionProblem:
             self.values = [torch.empty((nnz, 2), dtype=torch.float64, device=self.mesh.device)
                            for _ in self.freqs]
             self.solvers = [None] * len(self.freqs)
-            # element-by-element operators, one per frequency: the solvers'
-            # autotuners use them when they are the faster SpMV
+            # element-by-element operators, one per frequency and worker: the
+            # solvers' autotuners use them when they are the faster SpMV
             self.eops = [None] * len(self.freqs)
         self.observed = None if observed is None else np.asarray(observed)
         self.x = None
@@ -123,24 +126,32 @@ class InversionProblem:
                 # COCG + entity-block Jacobi; the SpMV is the faster of the
                 # entity-row CSR kernels (Morton order) and the element products
                 order = self.mesh.entity_order()
-                self.eops[fi] = core.ElementOperator(self.asm, om, self.ent_bc, order)
-                self.solvers[fi] = core.KrylovSolver(self.asm.plan, self.values[fi],
-                                                     precision=self.precision, rtol=self.rtol,
-                                                     pc="bjacobi",
-                                                     block_ptr=self.mesh.block_ptr(),
-                                                     entity_order=order, eop=self.eops[fi])
+                self.eops[fi] = [core.ElementOperator(self.asm, om, self.ent_bc, order)
+                                 for _ in range(self.concurrency)]
+                self.solvers[fi] = [core.KrylovSolver(self.asm.plan, self.values[fi],
+                                                      precision=self.precision, rtol=self.rtol,
+                                                      pc="bjacobi",
+                                                      block_ptr=self.mesh.block_ptr(),
+                                                      entity_order=order, eop=e)
+                                    for e in self.eops[fi]]
             else:
-                if self.eops[fi] is not None:
-                    self.eops[fi].refresh(om)
-                self.solvers[fi].refresh()
+                for e, S in zip(self.eops[fi], self.solvers[fi]):
+                    e.refresh(om)
+                    S.refresh()
 
-    def _solve(self, fi, b):
-        x, info = self.solvers[fi].solve(b)
-        if info["status"] != 0:
-            raise RuntimeError(f"Krylov solve failed: {info}")
-        self.stats["solves"] += 1
-        self.stats["iterations"] += info["iterations"]
-        return x
+    def _solve_all(self, fi, bs):
+        """Solutions of A_f x = b for every b in bs (concurrently when configured)."""
+        solvers = self.solvers[fi]
+        if len(solvers) > 1 and len(bs) > 1:
+            xs, infos = core.solve_concurrently(solvers, bs)
+        else:
+            xs, infos = zip(*[solvers[0].solve(b) for b in bs]) if bs else ((), ())
+        for info in infos:
+            if info["status"] != 0:
+                raise RuntimeError(f"Krylov solve failed: {info}")
+            self.stats["solves"] += 1
+            self.stats["iterations"] += info["iterations"]
+        return list(xs)
 
     def _solve_batch(self, ci, column):
         """Solve chunk ci with right-hand side column(si, fi) -> (ndof, 2) per pair."""
@@ -201,11 +212,8 @@ class InversionProblem:
         self.x = [[None] * F for _ in range(S)]
         for fi, f in enumerate(self.freqs):
             om = core.omega_of(f)
-            for si in range(S):
-                b = core.source_vector(self.mesh, int(self.src_el[si]), self.sources[si],
-                                       self.moment, om)
-                self._mask(b)
-                x = self._solve(fi, b)
+            xs = self._solve_all(fi, [self._source(si, fi) for si in range(S)])
+            for si, x in enumerate(xs):
                 self.x[si][fi] = x
                 fld = core.receiver_fields(self.mesh, self.rx, self.rx_el, x, om)
                 out[si, fi] = core.as_complex(fld)[:, 0:3]
@@ -248,8 +256,9 @@ class InversionProblem:
             return phi, g
         for fi, f in enumerate(self.freqs):
             om = core.omega_of(f)
-            for si in range(self.sources.shape[0]):
-                lam = self._solve(fi, adjoint_source(si, fi))
+            S = self.sources.shape[0]
+            lams = self._solve_all(fi, [adjoint_source(si, fi) for si in range(S)])
+            for si, lam in enumerate(lams):
                 omega = np.array([om], dtype=np.float64)
                 nat.check(L.pg_adjoint_gradient(self.asm.tables.handle, m.n_elems,
                                                 nat.ptr(self.asm.workspace), nat.ptr(m.dofs), 1,

@@ -540,6 +540,46 @@ class KrylovSolver:
             pass
 
 
+def solve_concurrently(solvers, rhs, streams=None):
+    """Solve A x_i = rhs[i] for every right-hand side, len(solvers) at a time:
+    worker k drives solvers[k] on its own HIP stream from its own host thread
+    (the library call releases the interpreter lock), taking right-hand sides
+    k, k + K, ...  The solvers must be independent objects of the same system
+    (each with its own element operator, whose local-result buffer is per
+    operator).  A solve's arithmetic does not depend on what runs beside it,
+    so the solutions are bitwise those of one solve after another.  Returns
+    (solutions, infos) in the order of `rhs`; raises the first failure."""
+    import threading
+    K = len(solvers)
+    if streams is None:
+        streams = [torch.cuda.Stream() for _ in range(K)]
+    out, infos, errors = [None] * len(rhs), [None] * len(rhs), []
+    torch.cuda.synchronize()  # right-hand sides and operators complete
+
+    def work(k):
+        try:
+            with torch.cuda.stream(streams[k]):
+                for i in range(k, len(rhs), K):
+                    x = torch.zeros_like(rhs[i])
+                    out[i], infos[i] = solvers[k].solve(rhs[i], x, stream=streams[k])
+            streams[k].synchronize()
+        except Exception as e:  # reported after every worker has stopped
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(K)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    if errors:
+        raise errors[0]
+    cur = torch.cuda.current_stream()
+    for x in out:
+        x.record_stream(cur)  # allocated on a worker stream, used from here on
+    return out, infos
+
+
 def spmv(plan, values, x, stream=None):
     y = torch.empty_like(x)
     nat.check(nat.lib().pg_spmv_z(plan.ndof, nat.ptr(plan.rowptr), nat.ptr(plan.cols),
