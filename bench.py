@@ -249,8 +249,11 @@ def batched_leg(args, log):
     for f in freqs:
         asm.assemble(core.omega_of(f), ent_bc=ent_bc, out=vals)
         eops = [core.ElementOperator(asm, core.omega_of(f), ent_bc, order) for _ in range(2)]
-        Ss = [core.KrylovSolver(asm.plan, vals, precision="mixed", pc="bjacobi", rtol=args.rtol,
-                                block_ptr=bptr, entity_order=order, eop=e) for e in eops]
+        Ss = []
+        for e in eops:  # the second takes the first one's kernel choices
+            Ss.append(core.KrylovSolver(asm.plan, vals, precision="mixed", pc="bjacobi",
+                                        rtol=args.rtol, block_ptr=bptr, entity_order=order,
+                                        eop=e, like=Ss[0] if Ss else None))
         rhs = []
         for si in range(len(xs)):
             b = core.source_vector(mesh, int(src_el[si]), srcs[si], (1.0, 0.0, 0.0),

@@ -237,6 +237,12 @@ typedef struct pg_solve_opts {
                          always; the true residual always uses the CSR matrix.  The
                          operator must outlive the solver, and after a reassembly
                          pg_eop_refresh must accompany pg_solver_refresh. */
+  const void *like;   /* optional solver of the same system whose SpMV choices (the
+                         iterations' kernels in both precisions and the true
+                         residual's) are copied instead of timed: solvers with the
+                         same kernels give bitwise identical solves (the timing of
+                         the autotuner can differ between instances); read at
+                         creation */
 } pg_solve_opts;
 
 typedef struct pg_solve_info {

@@ -32,7 +32,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("ksp", _I32), ("pc", _I32), ("precision", _I32), ("restart", _I32),
                 ("maxit", _I32), ("check_every", _I32), ("rtol", _D), ("inner_rtol", _D),
                 ("block_ptr", _P), ("nblocks", _I64), ("spmv", _I32), ("stall_iters", _I32),
-                ("entity_order", _P), ("eop", _P)]
+                ("entity_order", _P), ("eop", _P), ("like", _P)]
 
 
 class SolveInfo(ctypes.Structure):

@@ -128,12 +128,14 @@ class InversionProblem:
                 order = self.mesh.entity_order()
                 self.eops[fi] = [core.ElementOperator(self.asm, om, self.ent_bc, order)
                                  for _ in range(self.concurrency)]
-                self.solvers[fi] = [core.KrylovSolver(self.asm.plan, self.values[fi],
-                                                      precision=self.precision, rtol=self.rtol,
-                                                      pc="bjacobi",
-                                                      block_ptr=self.mesh.block_ptr(),
-                                                      entity_order=order, eop=e)
-                                    for e in self.eops[fi]]
+                # the workers' solvers take the first one's kernel choices, so that
+                # every right-hand side is solved with the same arithmetic
+                self.solvers[fi] = []
+                for e in self.eops[fi]:
+                    self.solvers[fi].append(core.KrylovSolver(
+                        self.asm.plan, self.values[fi], precision=self.precision, rtol=self.rtol,
+                        pc="bjacobi", block_ptr=self.mesh.block_ptr(), entity_order=order, eop=e,
+                        like=self.solvers[fi][0] if self.solvers[fi] else None))
             else:
                 for e, S in zip(self.eops[fi], self.solvers[fi]):
                     e.refresh(om)

@@ -480,7 +480,8 @@ class KrylovSolver:
 
     def __init__(self, plan, values, ksp="cocg", pc="jacobi", precision="fp64", rtol=1e-8,
                  maxit=100000, check_every=25, inner_rtol=1e-4, restart=30, block_ptr=None,
-                 spmv="auto", entity_order=None, stall_iters=0, eop=None, stream=None):
+                 spmv="auto", entity_order=None, stall_iters=0, eop=None, like=None,
+                 stream=None):
         self.plan = plan
         self.values = values
         self.eop = eop  # kept alive as long as the solver
@@ -499,7 +500,8 @@ class KrylovSolver:
                                   None if self._bptr is None else self._bptr.ctypes.data, nb,
                                   _SPMV[spmv], int(stall_iters),
                                   None if self._order is None else self._order.ctypes.data,
-                                  None if eop is None else eop.handle)
+                                  None if eop is None else eop.handle,
+                                  None if like is None else like.handle)
         h = ctypes.c_void_p()
         nat.check(nat.lib().pg_solver_create(plan.ndof, plan.nnz, nat.ptr(plan.rowptr),
                                              nat.ptr(plan.cols), nat.ptr(values),
@@ -546,8 +548,9 @@ def solve_concurrently(solvers, rhs, streams=None):
     (the library call releases the interpreter lock), taking right-hand sides
     k, k + K, ...  The solvers must be independent objects of the same system
     (each with its own element operator, whose local-result buffer is per
-    operator).  A solve's arithmetic does not depend on what runs beside it,
-    so the solutions are bitwise those of one solve after another.  Returns
+    operator) with the same kernel choices (``KrylovSolver(like=...)``).  A
+    solve's arithmetic does not depend on what runs beside it, so the
+    solutions are then bitwise those of one solve after another.  Returns
     (solutions, infos) in the order of `rhs`; raises the first failure."""
     import threading
     K = len(solvers)

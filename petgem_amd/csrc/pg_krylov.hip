@@ -1639,9 +1639,30 @@ static int setup_entities(Solver* S, const int32_t* bp, const int32_t* order, hi
   return PG_OK;
 }
 
-// pick the fastest SpMV configuration for this matrix (row / entity-row, G, U)
+// pick the fastest SpMV configuration for this matrix (row / entity-row, G, U,
+// element operator), or copy another solver's choices
 static int autotune(Solver* S, hipStream_t st) {
   const bool mixed = S->o.precision == PG_PREC_MIXED;
+  if (S->o.like) {
+    const Solver* L = static_cast<const Solver*>(S->o.like);
+    if (L->n != S->n || L->nnz != S->nnz || L->er_ok != S->er_ok || L->pairs_ok != S->pairs_ok)
+      PG_FAIL(PG_ERR_ARG, "`like` is a solver of another system or layout");
+    const auto usable = [&](const SpmvCfg& c, bool c128) {
+      return c.kind != 3 || (S->eop && eop_supports(S->eop, c128));
+    };
+    if (!usable(L->c64, true) || (mixed && !usable(L->c32, false)))
+      PG_FAIL(PG_ERR_ARG, "`like` uses an element operator this solver does not have");
+    if (mixed && L->o.precision != PG_PREC_MIXED)
+      PG_FAIL(PG_ERR_ARG, "`like` has no complex64 configuration");
+    S->cres = L->cres;
+    S->c64 = L->c64;
+    if (S->c64.kind == 3) S->c64.grid = eop_grid(S->eop);
+    if (mixed) {
+      S->c32 = L->c32;
+      if (S->c32.kind == 3) S->c32.grid = eop_grid(S->eop);
+    }
+    return PG_OK;
+  }
   if ((S->o.spmv == PG_SPMV_ENTITY || S->o.spmv == PG_SPMV_PAIRS) && !S->er_ok)
     PG_FAIL(PG_ERR_ARG, "entity-row SpMV needs block_ptr ranges whose rows share one column list");
   if (S->o.spmv == PG_SPMV_PAIRS && !S->pairs_ok)
