@@ -583,6 +583,41 @@ def solve_concurrently(solvers, rhs, streams=None):
     return out, infos
 
 
+def solve_each(solvers, rhs, workers=4):
+    """Solve solvers[i] x_i = rhs[i] (independent systems, e.g. many meshes),
+    `workers` at a time, each worker on its own host thread and HIP stream
+    taking i = k, k + workers, ...  Returns (solutions, infos); raises the
+    first failure.  Each solve is bitwise what it is on its own."""
+    import threading
+    K = max(1, min(int(workers), len(rhs)))
+    streams = [torch.cuda.Stream() for _ in range(K)]
+    out, infos, errors = [None] * len(rhs), [None] * len(rhs), []
+    torch.cuda.synchronize()
+
+    def work(k):
+        try:
+            with torch.cuda.stream(streams[k]):
+                for i in range(k, len(rhs), K):
+                    x = torch.zeros_like(rhs[i])
+                    out[i], infos[i] = solvers[i].solve(rhs[i], x, stream=streams[k])
+            streams[k].synchronize()
+        except Exception as e:  # reported after every worker has stopped
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(K)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    torch.cuda.synchronize()
+    if errors:
+        raise errors[0]
+    cur = torch.cuda.current_stream()
+    for x in out:
+        x.record_stream(cur)
+    return out, infos
+
+
 def spmv(plan, values, x, stream=None):
     y = torch.empty_like(x)
     nat.check(nat.lib().pg_spmv_z(plan.ndof, nat.ptr(plan.rowptr), nat.ptr(plan.cols),
