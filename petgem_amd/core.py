@@ -542,20 +542,14 @@ class KrylovSolver:
             pass
 
 
-def solve_concurrently(solvers, rhs, streams=None):
-    """Solve A x_i = rhs[i] for every right-hand side, len(solvers) at a time:
-    worker k drives solvers[k] on its own HIP stream from its own host thread
-    (the library call releases the interpreter lock), taking right-hand sides
-    k, k + K, ...  The solvers must be independent objects of the same system
-    (each with its own element operator, whose local-result buffer is per
-    operator) with the same kernel choices (``KrylovSolver(like=...)``).  A
-    solve's arithmetic does not depend on what runs beside it, so the
-    solutions are then bitwise those of one solve after another.  Returns
-    (solutions, infos) in the order of `rhs`; raises the first failure."""
+def _run_on_streams(nworkers, rhs, pick):
+    """Solve rhs[i] with the solver pick(k, i) for i = k, k + K, ... on worker k
+    (K = nworkers host threads, one HIP stream each; the library call releases
+    the interpreter lock).  Returns (solutions, infos); raises the first
+    failure once every worker has stopped."""
     import threading
-    K = len(solvers)
-    if streams is None:
-        streams = [torch.cuda.Stream() for _ in range(K)]
+    K = max(1, min(int(nworkers), len(rhs)))
+    streams = [torch.cuda.Stream() for _ in range(K)]
     out, infos, errors = [None] * len(rhs), [None] * len(rhs), []
     torch.cuda.synchronize()  # right-hand sides and operators complete
 
@@ -564,7 +558,7 @@ def solve_concurrently(solvers, rhs, streams=None):
             with torch.cuda.stream(streams[k]):
                 for i in range(k, len(rhs), K):
                     x = torch.zeros_like(rhs[i])
-                    out[i], infos[i] = solvers[k].solve(rhs[i], x, stream=streams[k])
+                    out[i], infos[i] = pick(k, i).solve(rhs[i], x, stream=streams[k])
             streams[k].synchronize()
         except Exception as e:  # reported after every worker has stopped
             errors.append(e)
@@ -583,39 +577,23 @@ def solve_concurrently(solvers, rhs, streams=None):
     return out, infos
 
 
+def solve_concurrently(solvers, rhs):
+    """Solve A x_i = rhs[i] for every right-hand side, len(solvers) at a time:
+    worker k drives solvers[k] on its own HIP stream, taking right-hand sides
+    k, k + K, ...  The solvers must be independent objects of the same system
+    (each with its own element operator, whose local-result buffer is per
+    operator) with the same kernel choices (``KrylovSolver(like=...)``).  A
+    solve's arithmetic does not depend on what runs beside it, so the
+    solutions are then bitwise those of one solve after another.  Returns
+    (solutions, infos) in the order of `rhs`; raises the first failure."""
+    return _run_on_streams(len(solvers), rhs, lambda k, i: solvers[k])
+
+
 def solve_each(solvers, rhs, workers=4):
     """Solve solvers[i] x_i = rhs[i] (independent systems, e.g. many meshes),
-    `workers` at a time, each worker on its own host thread and HIP stream
-    taking i = k, k + workers, ...  Returns (solutions, infos); raises the
-    first failure.  Each solve is bitwise what it is on its own."""
-    import threading
-    K = max(1, min(int(workers), len(rhs)))
-    streams = [torch.cuda.Stream() for _ in range(K)]
-    out, infos, errors = [None] * len(rhs), [None] * len(rhs), []
-    torch.cuda.synchronize()
-
-    def work(k):
-        try:
-            with torch.cuda.stream(streams[k]):
-                for i in range(k, len(rhs), K):
-                    x = torch.zeros_like(rhs[i])
-                    out[i], infos[i] = solvers[i].solve(rhs[i], x, stream=streams[k])
-            streams[k].synchronize()
-        except Exception as e:  # reported after every worker has stopped
-            errors.append(e)
-
-    threads = [threading.Thread(target=work, args=(k,)) for k in range(K)]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
-    torch.cuda.synchronize()
-    if errors:
-        raise errors[0]
-    cur = torch.cuda.current_stream()
-    for x in out:
-        x.record_stream(cur)
-    return out, infos
+    `workers` at a time on their own streams.  Each solve is bitwise what it
+    is on its own.  Returns (solutions, infos); raises the first failure."""
+    return _run_on_streams(workers, rhs, lambda k, i: solvers[i])
 
 
 def spmv(plan, values, x, stream=None):
