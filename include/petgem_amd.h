@@ -225,8 +225,10 @@ typedef struct pg_solve_opts {
   int64_t nblocks;
   int32_t spmv;       /* PG_SPMV_*: SpMV kernel family (AUTO = fastest by timing) */
   int32_t stall_iters;/* mixed: iterations without a 10 % drop of a complex64 round's best
-                         residual after which the round counts as stagnated and the solve
-                         continues in FP64 (0 = 400) */
+                         residual after which the round counts as stagnated (0 = 400):
+                         a round that has cut the true residual at least tenfold is
+                         followed by another complex64 round from the FP64 residual,
+                         otherwise the solve continues in FP64 */
   const int32_t *entity_order; /* optional host permutation of the nblocks entities: the
                                   order in which the entity-row SpMV visits them (a
                                   spatially local order keeps the gathers of x in cache);

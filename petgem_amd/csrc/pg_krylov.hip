@@ -1315,6 +1315,7 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
     return PG_OK;
   }
   const double delta2 = S->o.inner_rtol * S->o.inner_rtol;
+  double round_res64 = res64;  // |b - A x|^2 when the current complex64 round began
   if (low) {
     hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
     hipLaunchKernelGGL(k_set_stall, dim3(1), dim3(64), 0, st, S->s,
@@ -1368,8 +1369,15 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
       break;
     }
     if (hs.iter >= S->o.maxit) break;
-    if (low && hs.stalled) {
-      // the complex64 recursion stagnated: continue in FP64 from x
+    if (low && hs.stalled && res64 <= 1e-2 * round_res64) {
+      // the complex64 round stagnated after reducing the true residual at least
+      // tenfold: one more round from the FP64 residual (iterative refinement
+      // whose rounds are limited by complex64 accuracy)
+      launch_reset<float2>(S, st, S->r32, S->dinv32, S->binv32, S->z32, S->p32, nullptr);
+      hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
+      round_res64 = res64;
+    } else if (low && hs.stalled) {
+      // the complex64 recursion stagnated without progress: continue in FP64 from x
       low = false;
       info->fp64_fallback = 1;
       hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, 0.0);
@@ -1384,6 +1392,7 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
       // search direction across a residual replacement stalls COCG)
       launch_reset<float2>(S, st, S->r32, S->dinv32, S->binv32, S->z32, S->p32, nullptr);
       hipLaunchKernelGGL(k_set_ref2, dim3(1), dim3(64), 0, st, S->s, res64 * delta2);
+      round_res64 = res64;
     }
     PG_CHECK_LAUNCH();
   }
