@@ -550,6 +550,8 @@ def main():
                             dmax, traffic)
     roof64 = roofline_entry(prof64["g64"], results["fp64"], prof64["ms64"], "double2", T, plan,
                             dmax, traffic)
+    k32_name = spmv_key(prof["g32"], dmax, "float2", T.p)
+    k64_name = spmv_key(prof64["g64"], dmax, "double2", T.p)
     log(f"SpMV mixed: {roof32}")
     log(f"SpMV fp64: {roof64}")
     log(f"assembly {t_asm_ms:.2f} ms")
@@ -590,6 +592,9 @@ def main():
                          f"{' + log-normal noise 0.1' if cfg.get('noise') else ''}, "
                          f"HED at (0,0,-950) m, {w.frequency} Hz, rtol {args.rtol}"),
             "solver": f"COCG + {args.pc}, true-residual stop",
+            "spmv": {"mixed": k32_name, "fp64": k64_name,
+                     "choice": "autotuned per precision among CSR kernels and the "
+                               "element-by-element operator"},
         },
         "fp64_s": round(t_fp64, 6),
         "mixed_s": round(t_mixed, 6),
