@@ -542,21 +542,28 @@ class KrylovSolver:
             pass
 
 
-def _run_on_streams(nworkers, rhs, pick):
-    """Solve rhs[i] with the solver pick(k, i) for i = k, k + K, ... on worker k
-    (K = nworkers host threads, one HIP stream each; the library call releases
-    the interpreter lock).  Returns (solutions, infos); raises the first
+def _run_on_streams(nworkers, rhs, pick, order=None):
+    """Solve rhs[i] with the solver pick(k, i) on worker k (K = nworkers host
+    threads, one HIP stream each; the library call releases the interpreter
+    lock); each worker takes the next unsolved index of `order` (default
+    0, 1, ...) when it is free.  Returns (solutions, infos); raises the first
     failure once every worker has stopped."""
     import threading
     K = max(1, min(int(nworkers), len(rhs)))
     streams = [torch.cuda.Stream() for _ in range(K)]
     out, infos, errors = [None] * len(rhs), [None] * len(rhs), []
+    queue = list(range(len(rhs))) if order is None else list(order)
+    lock = threading.Lock()
     torch.cuda.synchronize()  # right-hand sides and operators complete
 
     def work(k):
         try:
             with torch.cuda.stream(streams[k]):
-                for i in range(k, len(rhs), K):
+                while True:
+                    with lock:
+                        if not queue:
+                            break
+                        i = queue.pop(0)
                     x = torch.zeros_like(rhs[i])
                     out[i], infos[i] = pick(k, i).solve(rhs[i], x, stream=streams[k])
             streams[k].synchronize()
@@ -579,8 +586,8 @@ def _run_on_streams(nworkers, rhs, pick):
 
 def solve_concurrently(solvers, rhs):
     """Solve A x_i = rhs[i] for every right-hand side, len(solvers) at a time:
-    worker k drives solvers[k] on its own HIP stream, taking right-hand sides
-    k, k + K, ...  The solvers must be independent objects of the same system
+    worker k drives solvers[k] on its own HIP stream, taking the next right-hand
+    side whenever it is free.  The solvers must be independent objects of the same system
     (each with its own element operator, whose local-result buffer is per
     operator) with the same kernel choices (``KrylovSolver(like=...)``).  A
     solve's arithmetic does not depend on what runs beside it, so the
@@ -591,9 +598,11 @@ def solve_concurrently(solvers, rhs):
 
 def solve_each(solvers, rhs, workers=4):
     """Solve solvers[i] x_i = rhs[i] (independent systems, e.g. many meshes),
-    `workers` at a time on their own streams.  Each solve is bitwise what it
+    `workers` at a time on their own streams, largest system first, each
+    worker taking the next one when it is free.  Each solve is bitwise what it
     is on its own.  Returns (solutions, infos); raises the first failure."""
-    return _run_on_streams(workers, rhs, lambda k, i: solvers[i])
+    order = sorted(range(len(rhs)), key=lambda i: -rhs[i].shape[0])
+    return _run_on_streams(workers, rhs, lambda k, i: solvers[i], order)
 
 
 def spmv(plan, values, x, stream=None):
