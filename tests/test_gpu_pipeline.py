@@ -83,7 +83,7 @@ output:
 
 
 @pytest.mark.parametrize("p,ksp,precision", [(1, "gmres", None), (1, "cocg", "mixed"),
-                                             (2, "cocg", None)])
+                                             (2, "cocg", None), (2, "cocg", "mixed")])
 def test_kernel_pipeline_vs_oracle(tmp_path, p, ksp, precision):
     from petgem_amd import kernel
     from petgem_amd.common import Timers
