@@ -11,7 +11,9 @@ this build's scope and are rejected with a clear error).
 ``SolverOptions`` reads the PETSc-style flags the reference passes through
 ``-options_file`` (``kernel.py:15``, ``solver.py:588``): ``-ksp_type``,
 ``-pc_type``, ``-ksp_rtol``, ``-ksp_max_it``, ``-ksp_gmres_restart``, plus
-``-pg_precision fp64|mixed`` and ``-pg_inner_rtol``.
+``-pg_precision fp64|mixed``, ``-pg_inner_rtol`` and
+``-pg_spmv auto|row|entity|entity_pairs|element`` (the product kernel family;
+auto = the fastest by timing, element = the element-by-element operator).
 
 ``Timers`` keeps named wall-clock timers with the reference's names
 (``Setup``, ``Assembly``, ``SetBoundaries``, ``Solver``, ...;
@@ -124,6 +126,7 @@ class SolverOptions:
         self.restart = 30
         self.precision = "fp64"
         self.inner_rtol = 1e-4
+        self.spmv = "auto"
         self.notes = []
         flags = []
         if options_file:
@@ -178,6 +181,11 @@ class SolverOptions:
             self.precision = v
         elif k == "-pg_inner_rtol" and v:
             self.inner_rtol = float(v)
+        elif k == "-pg_spmv" and v:
+            if v not in ("auto", "row", "entity", "entity_pairs", "element"):
+                raise ParamsError(f"-pg_spmv must be auto, row, entity, entity_pairs or "
+                                  f"element, not {v}")
+            self.spmv = v
 
     def ksp_name(self):
         return self.ksp_type

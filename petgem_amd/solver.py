@@ -114,11 +114,13 @@ class Solver:
         whole = np.array_equal(np.repeat(ent_bc.cpu().numpy()[:bp.size - 1], np.diff(bp)),
                                dof_bc.cpu().numpy())
         eop = (core.ElementOperator(self.assembler, self.omega, ent_bc, order)
-               if whole else None)
+               if whole and o.spmv in ("auto", "element") else None)
+        if o.spmv == "element" and eop is None:
+            raise ValueError("-pg_spmv element needs the PEC condition on whole entities")
         solver = core.KrylovSolver(self.assembler.plan, self.A, ksp=o.ksp_type, pc=o.pc_type,
                                    precision=o.precision, rtol=o.rtol, maxit=o.max_it,
                                    restart=o.restart, inner_rtol=o.inner_rtol,
-                                   block_ptr=bp, entity_order=order, eop=eop)
+                                   block_ptr=bp, entity_order=order, eop=eop, spmv=o.spmv)
         self.x, self.info = solver.solve(self.b, self.x)
         device_sync()
         Timers()["Solver"].stop()

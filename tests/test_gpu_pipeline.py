@@ -47,7 +47,7 @@ def test_gmres_on_reference_matrix_fixture(ksp, restart):
     assert np.abs(xg - xs).max() <= 1e-7 * np.abs(xs).max()
 
 
-def _write_case(tmp, p, ksp="gmres", pc="sor", precision=None):
+def _write_case(tmp, p, ksp="gmres", pc="sor", precision=None, spmv=None):
     rx = np.load(os.path.join(GOLDEN, "receiver_pos.npy"))
     h5.write_data(os.path.join(tmp, "receiver_pos.h5"), rx)
     params = f"""
@@ -79,17 +79,22 @@ output:
         f.write(f"# Solver options for PETSc\n-ksp_type {ksp}\n-pc_type {pc}\n-ksp_rtol 1e-10\n")
         if precision:
             f.write(f"-pg_precision {precision}\n")
+        if spmv:
+            f.write(f"-pg_spmv {spmv}\n")
     return os.path.join(tmp, "params.yaml"), os.path.join(tmp, "petsc.opts")
 
 
-@pytest.mark.parametrize("p,ksp,precision", [(1, "gmres", None), (1, "cocg", "mixed"),
-                                             (2, "cocg", None), (2, "cocg", "mixed")])
-def test_kernel_pipeline_vs_oracle(tmp_path, p, ksp, precision):
+@pytest.mark.parametrize("p,ksp,precision,spmv", [(1, "gmres", None, None),
+                                                  (1, "cocg", "mixed", None),
+                                                  (2, "cocg", None, None),
+                                                  (2, "cocg", "mixed", None),
+                                                  (2, "cocg", "mixed", "element")])
+def test_kernel_pipeline_vs_oracle(tmp_path, p, ksp, precision, spmv):
     from petgem_amd import kernel
     from petgem_amd.common import Timers
     from petgem_amd.mesh import read_gmsh_v2, Topology
     Timers().reset()
-    params, opts = _write_case(str(tmp_path), p, ksp=ksp, precision=precision)
+    params, opts = _write_case(str(tmp_path), p, ksp=ksp, precision=precision, spmv=spmv)
     path = kernel.main(["kernel.py", "-options_file", opts, params])
     assert os.path.exists(path)
     out = os.path.join(str(tmp_path), "out")

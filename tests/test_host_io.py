@@ -93,4 +93,7 @@ def test_solver_options_from_reference_opts_file(tmp_path):
     assert (o.ksp_type, o.pc_type, o.rtol) == ("gmres", "bjacobi", 1e-8)
     assert any("sor" in n for n in o.notes)
     o = SolverOptions(["-ksp_type", "cocg", "-pg_precision", "mixed", "-ksp_max_it", "50"])
-    assert (o.ksp_type, o.precision, o.max_it) == ("cocg", "mixed", 50)
+    assert (o.ksp_type, o.precision, o.max_it, o.spmv) == ("cocg", "mixed", 50, "auto")
+    assert SolverOptions(["-pg_spmv", "element"]).spmv == "element"
+    with pytest.raises(ParamsError):
+        SolverOptions(["-pg_spmv", "magic"])
