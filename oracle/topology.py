@@ -17,7 +17,12 @@ owner = first element in element order) rather than any shortcut:
   element, their edges, and the DOFs on both (returned as a sorted set).
 
 Pinned by the reference's own golden values ``tests/test_mesh.py:31-36`` on
-``tests/data/test_mesh.msh`` (counts and element 0's edges and faces).
+``tests/data/test_mesh.msh`` (counts and element 0's edges and faces), and
+array-for-array by the outputs of the reference's ``computeEdges``,
+``computeFaces``, ``computeFacesEdges``, ``computeConnectivityDOFS``,
+``computeBoundaries`` and ``computeElementOrientation`` on that mesh and on a
+Kuhn grid (golden vectors, ``tests/test_topology_golden.py``), plus the
+tangential-continuity check of assembled fields across shared faces.
 """
 import numpy as np
 

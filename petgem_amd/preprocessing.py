@@ -16,9 +16,9 @@ Differences from the reference, by design: the mesh reader is our own (no
 meshio), HDF5 goes through ``petgem_amd.h5`` (no h5py needed) and the source
 is located with the device kernel ``pg_locate_points`` (same rule as
 ``Delaunay.find_simplex(bruteforce=True, tol=1e-12)``: lowest index).
-``boundaries.dat`` lists the boundary edge DOFs then the boundary face DOFs
-in ascending order (the reference's order within each block differs; only
-the set is used).
+``boundaries.dat`` lists the boundary edge DOFs then the boundary face DOFs,
+each block ascending, which is the order ``computeBoundaries``
+(``mesh.py:280-321``) produces.
 """
 import os
 

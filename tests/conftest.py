@@ -25,3 +25,9 @@ def test_mesh():
 @pytest.fixture(scope="session")
 def golden():
     return np.load(os.path.join(GOLDEN, "hvfem_vectors.npz"))
+
+
+@pytest.fixture(scope="session")
+def topology_golden():
+    """Outputs of the reference's integer topology routines (make_topology_vectors.py)."""
+    return np.load(os.path.join(GOLDEN, "topology_vectors.npz"))

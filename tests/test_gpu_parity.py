@@ -88,10 +88,14 @@ def test_element_matrices_vs_reference_golden(golden, p):
     assert rel_err_per_element(K, golden[f"elem_K_p{p}"]) <= 1e-12
 
 
-def test_orientation_bit_exact(test_mesh):
+def test_orientation_bit_exact(test_mesh, topology_golden):
+    """Against the reference's own computeElementOrientation on its test mesh
+    (tests/golden/topology_vectors.npz, hvfem.py:122-220) and the oracle."""
     pts, en, _ = test_mesh
     T = Topology(pts, en, 2)
     eo_ref, fo_ref = otopo.orientations(en, T.elems_edges, T.edges_nodes, T.elem_faces_edges)
+    assert np.array_equal(eo_ref, topology_golden["mesh_edge_orient"])
+    assert np.array_equal(fo_ref, topology_golden["mesh_face_orient"])
     E = T.n_elems
     eo = torch.empty((E, 6), dtype=torch.int32, device=DEV)
     fo = torch.empty((E, 4), dtype=torch.int32, device=DEV)
