@@ -25,11 +25,25 @@ FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
 EXTRA = {"pg_elemop.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]}
 
 
-def build(verbose=False, jobs=4):
+def _stale(obj, src):
+    """True when obj is missing or older than its source or any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC)
+                                        if h.endswith(".h")]
+    deps.append(os.path.join(HERE, "..", "include", "petgem_amd.h"))
+    return any(os.path.getmtime(d) > os.path.getmtime(obj) for d in deps)
+
+
+def build(verbose=False, jobs=4, incremental=False):
+    """Compile every source (incremental=True: only those whose object is stale) and link."""
     os.makedirs(BUILD, exist_ok=True)
     objs, procs = [], []
     for src in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        if incremental and not _stale(obj, src):
+            objs.append(obj)
+            continue
         cmd = [HIPCC] + FLAGS + EXTRA.get(src, []) + ["-x", "hip", "-c", os.path.join(CSRC, src),
                                                       "-o", obj]
         if verbose:
@@ -56,4 +70,4 @@ def build(verbose=False, jobs=4):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))
