@@ -12,9 +12,12 @@ set-up + Krylov solve to a true relative residual <= 1e-8, with every input
 already resident in HBM.  The Krylov iterations apply A either with a CSR
 SpMV of the assembled matrix or element by element (the element matrices
 recomputed from the tables at every product), whichever the solver's
-autotuner measures faster; the true residual always uses the assembled CSR.  The headline value is the mixed-precision step
-time (complex64 Krylov with FP64 residual replacement); the FP64 step time is
-reported beside it.  Prints one JSON line.
+autotuner measures faster; the true residual always uses the assembled CSR.
+The headline value is the FP64 step time (complex128 throughout, the
+workload as stated); the mixed-precision step (complex64 Krylov iterations
+under FP64 residual replacement) is the metric's second leg, reported beside
+it with its speed-up and its receiver-field difference from FP64.  Prints one
+JSON line.
 """
 import argparse
 import json
@@ -110,8 +113,11 @@ def roofline_entry(code, res, standalone_ms, vt, T, plan, dmax, traffic):
             "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
             "frac": round(ach / peak, 4), "traffic": traffic.get(key),
             "algorithmic_flops": fl, "avg_ms": round(ms, 5),
-            "note": "vector-ALU FMA kernel (no matrix-core work); packed FP32 is disabled "
-                    "in this build, so FP32 issues at most half the published peak",
+            "note": ("vector-ALU FMA kernel (no matrix-core work); packed FP32 is disabled "
+                     "in this build, so FP32 issues at most half the published peak"
+                     if vt == "float2" else
+                     "vector-ALU FP64 FMA kernel (no matrix-core work); peak = MI355X FP64 "
+                     "vector rate"),
             "sum_kernel": {"bound": "hbm", "avg_ms": round(sum_ms, 5),
                            "algorithmic_bytes": sum_bytes,
                            "achieved": round(sum_bytes / (sum_ms * 1e-3) / 1e9, 1),
@@ -575,16 +581,16 @@ def main():
     t_mixed, t_fp64 = results["mixed"]["t"], results["fp64"]["t"]
     line = {
         "metric": METRIC,
-        "value": round(t_mixed, 6),
+        "value": round(t_fp64, 6),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_mixed * 1e3, 3),
+        "ms_per_step": round(t_fp64 * 1e3, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex64 Krylov/SpMV + complex128 residual, solution, assembly",
+        "dtype": "complex128",
         "data": "synthetic",
         "config": {
             "workload": (f"{args.config}: p={T.p} Nedelec, {w.n}^3 Kuhn hexes x 6 = {T.n_elems} "
@@ -597,6 +603,10 @@ def main():
                                "element-by-element operator"},
         },
         "fp64_s": round(t_fp64, 6),
+        "mixed": {"s": round(t_mixed, 6), "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
+                  "dtype": "complex64 Krylov iterations and SpMV; complex128 assembly, "
+                           "residual b - Ax, solution update",
+                  "roofline": roof32},
         "mixed_s": round(t_mixed, 6),
         "mixed_speedup": round(t_fp64 / t_mixed, 3),
         "iterations": {"fp64": results["fp64"]["info"]["iterations"],
@@ -606,7 +616,7 @@ def main():
                    "mixed": results["mixed"]["info"]["relres"]},
         "receiver_field_diff_mixed_vs_fp64": field_diff,
         "assembly_ms": round(t_asm_ms, 3),
-        "roofline": dict(roof32, fp64_spmv=roof64),
+        "roofline": roof64,
         "batched": batched,
         "multimesh": multimesh,
         "adjoint": adjoint,

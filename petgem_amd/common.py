@@ -231,7 +231,23 @@ _TIMERS = _Timers()
 
 
 def Timers(opath=None):  # noqa: N802 - same name as the reference's singleton
+    """The timers singleton.  When the reference's ``petgem.common`` is loaded
+    (its ``kernel.py`` imports it before creating the solver), its own
+    ``Timers()`` singleton is returned, so that the reference's postprocessing
+    (``postprocessing.py:381-383``: ``Timers()["Assembly"].elapsed +
+    Timers()["Solver"].elapsed``) reads the times this solver recorded."""
+    ref = sys.modules.get("petgem.common")
+    if ref is not None and callable(getattr(ref, "Timers", None)):
+        return ref.Timers()
     return _TIMERS
+
+
+def stop_timer(name, sync=True):
+    """Stop timer `name` after the device has drained (the reference's Timer.stop
+    takes no argument, so the drain happens here)."""
+    if sync:
+        device_sync()
+    Timers()[name].stop()
 
 
 class Print:

@@ -542,6 +542,27 @@ class KrylovSolver:
             pass
 
 
+def _check_independent(solvers):
+    """Solvers that may run at the same time must not share device state: each
+    KrylovSolver owns its vectors and scalars, and each ElementOperator its
+    local-result buffer.  Raises ValueError when the list names one solver, or
+    one element operator, twice (the two would race and give wrong solutions
+    without an error)."""
+    seen_s, seen_e = set(), set()
+    for S in solvers:
+        h = S.handle.value
+        if h in seen_s:
+            raise ValueError("the same solver object would run on two streams at once: give "
+                             "every concurrent worker its own KrylovSolver")
+        seen_s.add(h)
+        if getattr(S, "eop", None) is not None:
+            he = S.eop.handle.value
+            if he in seen_e:
+                raise ValueError("two concurrent solvers share one ElementOperator (its "
+                                 "local-result buffer is per operator): give each its own")
+            seen_e.add(he)
+
+
 def _run_on_streams(nworkers, rhs, pick, order=None):
     """Solve rhs[i] with the solver pick(k, i) on worker k (K = nworkers host
     threads, one HIP stream each; the library call releases the interpreter
@@ -593,6 +614,7 @@ def solve_concurrently(solvers, rhs):
     solve's arithmetic does not depend on what runs beside it, so the
     solutions are then bitwise those of one solve after another.  Returns
     (solutions, infos) in the order of `rhs`; raises the first failure."""
+    _check_independent(solvers)
     return _run_on_streams(len(solvers), rhs, lambda k, i: solvers[k])
 
 
@@ -601,6 +623,10 @@ def solve_each(solvers, rhs, workers=4):
     `workers` at a time on their own streams, largest system first, each
     worker taking the next one when it is free.  Each solve is bitwise what it
     is on its own.  Returns (solutions, infos); raises the first failure."""
+    if len(solvers) != len(rhs):
+        raise ValueError("solve_each needs one solver per right-hand side")
+    if workers > 1:
+        _check_independent(solvers)
     order = sorted(range(len(rhs)), key=lambda i: -rhs[i].shape[0])
     return _run_on_streams(workers, rhs, lambda k, i: solvers[i], order)
 

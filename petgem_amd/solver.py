@@ -18,12 +18,13 @@ scratch-file inputs and ``x{i}.dat`` output, CSEM mode:
 There is no CPU fallback: without the HIP library every method raises.
 """
 import os
+import sys
 
 import numpy as np
 
-from . import _native as nat
+from . import _native as nat  # noqa: F401  (fails loudly when the library is missing)
 from . import core, petsc_io
-from .common import Print, SolverOptions, Timers, device_sync
+from .common import Print, SolverOptions, Timers, stop_timer
 from .mesh import dof_counts
 
 
@@ -35,7 +36,10 @@ class Solver:
     """Class for solver (CSEM mode, one polarization)."""
 
     def __init__(self, options=None):
-        self.options = options if options is not None else SolverOptions()
+        # without explicit options the PETSc-style flags come from the command
+        # line, -options_file included, as the reference's ksp.setFromOptions()
+        # reads them (solver.py:588)
+        self.options = options if options is not None else SolverOptions(sys.argv[1:])
         self.info = None
 
     # ------------------------------------------------------------------ setup
@@ -67,8 +71,7 @@ class Solver:
                                     fe.reshape(E, 4, 3), dofs, n_edges, n_faces, ef)
         if self.mesh.ndof != self.total_num_dofs:
             raise ValueError(f"nnz.dat length {self.total_num_dofs} != {self.mesh.ndof} DOFs")
-        device_sync()
-        Timers()["Setup"].stop()
+        stop_timer("Setup")
 
     # --------------------------------------------------------------- assembly
     def assembly(self, inputSetup):
@@ -92,7 +95,7 @@ class Solver:
         self.b = torch.zeros((m.ndof, 2), dtype=torch.float64, device=m.device)
         core.source_vector(rec, 0, src["position"], core.dipole_moment(src), self.omega, b=self.b)
         self.x = torch.zeros_like(self.b)
-        Timers()["Assembly"].stop(device_sync)
+        stop_timer("Assembly")
 
     # -------------------------------------------------------------------- run
     def run(self, inputSetup):
@@ -101,7 +104,7 @@ class Solver:
         Timers()["SetBoundaries"].start()
         ent_bc, dof_bc = self.mesh.entity_mask(self.boundaries)
         core.apply_dirichlet(self.assembler.plan, self.A, dof_bc, self.b)
-        Timers()["SetBoundaries"].stop(device_sync)
+        stop_timer("SetBoundaries")
         Timers()["Solver"].start()
         o = self.options
         for note in o.notes:
@@ -122,8 +125,7 @@ class Solver:
                                    restart=o.restart, inner_rtol=o.inner_rtol,
                                    block_ptr=bp, entity_order=order, eop=eop, spmv=o.spmv)
         self.x, self.info = solver.solve(self.b, self.x)
-        device_sync()
-        Timers()["Solver"].stop()
+        stop_timer("Solver")
         Print.master(f"     {o.ksp_type}/{o.pc_type} ({o.precision}): {self.info['iterations']} "
                      f"iterations, relative residual {self.info['relres']:.3e}")
         if self.info["status"] != 0:

@@ -8,6 +8,7 @@ torch = pytest.importorskip("torch")
 
 from petgem_amd import core  # noqa: E402
 from petgem_amd.synthetic import Workload  # noqa: E402
+from oracle import fem, topology as otopo  # noqa: E402
 
 
 def _setup(n, p):
@@ -96,3 +97,54 @@ def test_mixed_batch_stagnating_columns_finish_in_fp64():
         assert (info["status"] == 0).all() and info["relres"].max() <= 1e-10
         Xh, Rh = core.as_complex(X), core.as_complex(ref)
         assert np.abs(Xh - Rh).max() <= 1e-7 * np.abs(Rh).max()
+
+
+def _oracle_column(w, position, frequency):
+    """SuperLU solution and receiver fields of one column (oracle.fem.forward)."""
+    T = w.topo
+    eo, fo = otopo.orientations(T.elems_nodes, T.elems_edges, T.edges_nodes, T.elem_faces_edges)
+    src = dict(position=list(position), azimuth=0.0, dip=0.0, current=1.0, length=1.0,
+               frequency=frequency)
+    return fem.forward(dict(edge_orient=eo, face_orient=fo, dofs=T.dofs, ndof=T.ndof,
+                            boundary_dofs=T.boundary_dofs), T.coords, w.sigma, T.p, src,
+                       w.receivers)
+
+
+@pytest.mark.parametrize("precision,pc", [("fp64", "bjacobi"), ("mixed", "bjacobi"),
+                                          ("fp64", "jacobi")])
+@pytest.mark.parametrize("n,p,nsrc,freqs", [(4, 2, 4, [0.1, 0.25, 0.5, 1.0]),
+                                            (3, 3, 2, [0.25, 1.0])])
+def test_batch_columns_match_oracle(n, p, nsrc, freqs, precision, pc):
+    """Batched columns (several sources and frequencies in one solve) against the
+    oracle's SuperLU solution of each column's own system: solution and
+    receiver fields within 1e-6."""
+    w, mesh, asm, ent_bc, dof_bc = _setup(n, p)
+    K, M = core.assemble_km(asm, ent_bc)
+    xs = np.linspace(-600.0, 600.0, nsrc)
+    srcs = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
+    el = core.locate_points(mesh, srcs).cpu().numpy()
+    cols, meta = [], []
+    for f in freqs:
+        for s in range(nsrc):
+            b = core.source_vector(mesh, int(el[s]), srcs[s], (1.0, 0.0, 0.0), core.omega_of(f))
+            core.apply_dirichlet(asm.plan, None, dof_bc, b)
+            cols.append(b)
+            meta.append((srcs[s], f))
+    B = torch.stack(cols, dim=1).contiguous()
+    bs = core.BatchSolver(asm.plan, K, M, [f for _, f in meta], rtol=1e-10,
+                          block_ptr=mesh.block_ptr(), entity_order=mesh.entity_order(),
+                          precision=precision, pc=pc)
+    X, info = bs.solve(B)
+    assert (info["status"] == 0).all() and info["relres"].max() <= 1e-10
+    rx_el = core.locate_points(mesh, w.receivers)
+    for r in (0, len(meta) // 2, len(meta) - 1):
+        pos, f = meta[r]
+        ref = _oracle_column(w, pos, f)
+        xr = X[:, r, :].contiguous()
+        fields = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, xr,
+                                                      core.omega_of(f)))
+        xerr = np.abs(core.as_complex(xr) - ref["x"]).max() / np.abs(ref["x"]).max()
+        ferr = np.abs(fields - ref["fields"]).max() / np.abs(ref["fields"]).max()
+        print(f"batch {precision} {pc} column {r} ({f} Hz): x err {xerr:.2e}, field err "
+              f"{ferr:.2e}", flush=True)
+        assert xerr <= 1e-6 and ferr <= 1e-6

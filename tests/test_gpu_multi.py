@@ -8,6 +8,7 @@ torch = pytest.importorskip("torch")
 
 from petgem_amd import core  # noqa: E402
 from petgem_amd.synthetic import Workload  # noqa: E402
+from oracle import fem, topology as otopo  # noqa: E402
 
 
 def _system(n, p, freq, seed):
@@ -39,6 +40,38 @@ def test_multi_mesh_matches_single_solves():
         x1, inf1 = core.KrylovSolver(pl, v, rtol=1e-10).solve(b)
         xh, x1h = core.as_complex(x), core.as_complex(x1)
         assert np.abs(xh - x1h).max() <= 1e-7 * np.abs(x1h).max()
+
+
+def test_multi_mesh_matches_oracle():
+    """Three meshes of the batch (p = 1, 2, 3, different frequencies) against
+    the oracle's SuperLU solution of each mesh's system: solution and receiver
+    fields within 1e-6."""
+    picks = [0, 2, 3]
+    ws, systems = [], []
+    for i in picks:
+        n, p, f = CASES[i]
+        w = Workload(n, p, noise=0.2, seed=i, n_rx=21, frequency=f)
+        ws.append(w)
+        systems.append(_system(n, p, f, seed=i))
+    ms = core.MultiMeshSolver([(pl, v) for pl, v, _ in systems], rtol=1e-10, check_every=10)
+    xs, info = ms.solve([b for _, _, b in systems])
+    assert (info["status"] == 0).all() and info["relres"].max() <= 1e-10
+    for w, x in zip(ws, xs):
+        T = w.topo
+        eo, fo = otopo.orientations(T.elems_nodes, T.elems_edges, T.edges_nodes,
+                                    T.elem_faces_edges)
+        ref = fem.forward(dict(edge_orient=eo, face_orient=fo, dofs=T.dofs, ndof=T.ndof,
+                               boundary_dofs=T.boundary_dofs), T.coords, w.sigma, T.p,
+                          dict(w.source, position=[0.0, 0.0, -950.0]), w.receivers)
+        mesh = core.DeviceMesh.from_topology(T, "cuda")
+        rx_el = core.locate_points(mesh, w.receivers)
+        fields = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, x,
+                                                      core.omega_of(w.frequency)))
+        xerr = np.abs(core.as_complex(x) - ref["x"]).max() / np.abs(ref["x"]).max()
+        ferr = np.abs(fields - ref["fields"]).max() / np.abs(ref["fields"]).max()
+        print(f"multi-mesh p={T.p} n={w.n} {w.frequency} Hz: x err {xerr:.2e}, field err "
+              f"{ferr:.2e}", flush=True)
+        assert xerr <= 1e-6 and ferr <= 1e-6
 
 
 def test_graph_replays_match_eager_launches():

@@ -296,6 +296,26 @@ def test_forward_solve_vs_superlu(p, n, precision, pc, spmv):
     assert ferr <= 1e-6
 
 
+@pytest.mark.parametrize("n,spmv", [(3, "entity"), (4, "entity"), (4, "element")])
+def test_p3_mixed_at_quarter_hertz_vs_superlu(n, spmv):
+    """C3's second frequency: p = 3 at 0.25 Hz in mixed precision (complex64
+    iterations under FP64 refinement, with the stagnation watch free to hand
+    the solve over to FP64) against SuperLU, for the CSR and element-product
+    iterations."""
+    w = Workload(n, 3, noise=0.1, seed=9, n_rx=21, frequency=0.25)
+    g = _forward_gpu(w, precision="mixed", pc="bjacobi", spmv=spmv)
+    r = _oracle_forward(w)
+    info = g["info"]
+    true_rel = np.linalg.norm(g["b"] - g["A"] @ g["x"]) / np.linalg.norm(g["b"])
+    ferr = np.abs(g["fields"] - r["fields"]).max() / np.abs(r["fields"]).max()
+    xerr = np.abs(g["x"] - r["x"]).max() / np.abs(r["x"]).max()
+    print(f"p=3 n={n} 0.25 Hz mixed {spmv}: iters={info['iterations']} "
+          f"refinements={info['refinements']} fp64_fallback={info['fp64_fallback']} "
+          f"true relres={true_rel:.2e} field err={ferr:.2e} x err={xerr:.2e}", flush=True)
+    assert info["status"] == 0 and info["relres"] <= 1e-8 and true_rel <= 1e-8
+    assert ferr <= 1e-6
+
+
 def test_c1_full_size_fp64_and_mixed():
     """C1 (20^3 Kuhn, p=1, 59,660 DOFs) at full size: true residual and FP64/mixed agreement."""
     w = Workload(20, 1, n_rx=41)

@@ -129,3 +129,69 @@ def test_kernel_pipeline_vs_oracle(tmp_path, p, ksp, precision, spmv):
     f, keep = field_interpolator(mesh, core.from_complex(x, DEV), rx, 2.0)
     assert keep.size == rx.shape[0]
     assert np.abs(f - fref).max() <= 1e-6 * np.abs(fref).max()
+
+
+class _RefTimer:
+    """The reference's Timer (common.py:391-419): stop() takes no argument."""
+
+    def __init__(self):
+        self._start, self.elapsed = 0.0, 0.0
+
+    def start(self):
+        import time
+        self._start = time.time()
+
+    def stop(self):
+        import time
+        if self._start > 0:
+            self.elapsed += time.time() - self._start
+            self._start = 0.0
+
+
+class _RefTimers:
+    def __init__(self):
+        self._elems = {}
+
+    def __getitem__(self, key):
+        return self._elems.setdefault(key, _RefTimer())
+
+
+def test_reference_driver_with_only_the_solver_import_changed(tmp_path, monkeypatch):
+    """The reference's kernel.py sequence (kernel.py:22-88) with one line
+    changed, ``from petgem_amd.solver import Solver``: ``Solver()`` takes the
+    options file from the command line as ``ksp.setFromOptions()`` does
+    (solver.py:588), and the reference's own ``Timers()`` singleton -- a
+    stand-in ``petgem.common`` module here, loaded first as the reference's
+    kernel.py loads it -- receives the Assembly and Solver times that the
+    reference's postprocessing adds up (postprocessing.py:381-383)."""
+    import sys
+    import types
+    inst = _RefTimers()
+    common = types.ModuleType("petgem.common")
+    common.Timers = lambda opath=None: inst
+    pkg = types.ModuleType("petgem")
+    pkg.common = common
+    monkeypatch.setitem(sys.modules, "petgem", pkg)
+    monkeypatch.setitem(sys.modules, "petgem.common", common)
+    params, opts = _write_case(str(tmp_path), 2, ksp="cocg", pc="jacobi", precision="mixed")
+    monkeypatch.setattr(sys, "argv", ["kernel.py", "-options_file", opts, params])
+
+    from petgem_amd.common import InputParameters
+    from petgem_amd.postprocessing import Postprocessing
+    from petgem_amd.preprocessing import Preprocessing
+    from petgem_amd.solver import Solver  # the changed import
+    input_setup = InputParameters(sys.argv[3])
+    Preprocessing().run(input_setup)
+    solver = Solver()
+    solver.setup(input_setup)
+    solver.assembly(input_setup)
+    solver.run(input_setup)
+    o = solver.options
+    assert (o.ksp_type, o.pc_type, o.precision, o.rtol) == ("cocg", "jacobi", "mixed", 1e-10)
+    assert solver.info["status"] == 0 and solver.info["relres"] <= 1e-10
+    assert solver.info["refinements"] >= 1  # complex64 inner solves under FP64 refinement
+    del solver
+    path = Postprocessing().run(input_setup)
+    assert inst["Assembly"].elapsed > 0 and inst["Solver"].elapsed > 0
+    rt = h5.read_dataset(path, "model/run-time (s)")
+    assert rt == pytest.approx(inst["Assembly"].elapsed + inst["Solver"].elapsed)
