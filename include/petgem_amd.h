@@ -182,6 +182,19 @@ int pg_adjoint_gradient(const void *tables, int64_t n_elems, const void *workspa
 int pg_adjoint_gradient_rows(const void *tables, int64_t n_elems, const void *workspace,
                              const int32_t *dofs, int nrhs, int64_t ndof, const double *x,
                              const double *lam, const double *omega, double *g, void *stream);
+/* pg_adjoint_gradient without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-right-hand-side omega*mu0 values are staged in a fixed 32 KB device
+ * array (one per GPU, kept until the process ends) instead of a stream-ordered
+ * allocation, so nrhs is at most 4096 (larger: PG_ERR_ARG).  Same arguments,
+ * kernel and results as pg_adjoint_gradient; the call returns once the stream
+ * has drained.  Every call on a GPU shares that GPU's staging array: calls that
+ * overlap on one GPU (from several host threads) race. */
+int pg_adjoint_gradient_noalloc(const void *tables, int64_t n_elems, const void *workspace,
+                                const int32_t *dofs, int nrhs, int64_t ndof, const double *x,
+                                const double *lam, const double *omega, double *g,
+                                void *stream);
 
 /* ------------------------------------------------------------------------
  * Sparse kernels and Krylov solve.  Replace MatMult and KSPSolve

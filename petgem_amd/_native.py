@@ -70,6 +70,8 @@ SIGNATURES = {
                                            _P]),
     "pg_adjoint_gradient_rows": (ctypes.c_int, [_P, _I64, _P, _P, ctypes.c_int, _I64, _P, _P, _P,
                                                 _P, _P]),
+    "pg_adjoint_gradient_noalloc": (ctypes.c_int, [_P, _I64, _P, _P, ctypes.c_int, _I64, _P, _P,
+                                                   _P, _P, _P]),
     "pg_eop_create": (ctypes.c_int, [ctypes.c_int, _I64, _P, _P, _I64, _P, _I64, _P, _P, _D,
                                      ctypes.POINTER(_P), _P]),
     "pg_eop_refresh": (ctypes.c_int, [_P, _P, _D, _P]),

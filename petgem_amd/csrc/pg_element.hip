@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pg_basis.h"
@@ -590,6 +591,36 @@ __global__ void __launch_bounds__(256) k_adjoint_grad(
 
 static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// ---------------------------------------------------------------------------
+// Staging of the adjoint gradient's per-right-hand-side omega*mu0 for the
+// entry point that allocates nothing (pg_adjoint_gradient_noalloc).  Every
+// device holds its own copy of `scratch`, at its own address; in host code the
+// name is only a stand-in, so the address is resolved once per device with
+// hipGetSymbolAddress (that device current) and only that address is used.
+// Each call overwrites the first nrhs entries before its kernel reads them;
+// calls that overlap on one device (several host threads) race on it.
+// ---------------------------------------------------------------------------
+constexpr int kAdjointMaxRhs = 4096;
+static __device__ __align__(256) double scratch[kAdjointMaxRhs];
+
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;  // by device ordinal; nullptr = not resolved yet
+
+static int scratch_address(double** out) {
+  int dev = 0;
+  PG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) PG_FAIL(PG_ERR_HIP, "hipGetSymbolAddress returned a null staging address");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PG_OK;
+}
+
 }  // namespace pg
 
 using namespace pg;
@@ -815,19 +846,28 @@ int pg_adjoint_rhs(int p, int64_t npts, const double* pts, const int32_t* elem,
   return PG_OK;
 }
 
+// static_staging: stage omega*mu0 in the file-scope device array instead of a
+// stream-ordered allocation (pg_adjoint_gradient_noalloc)
 static int adjoint_gradient(const void* tables, int64_t E, const void* ws, const int32_t* dofs,
                             int nrhs, int64_t ndof, int64_t sd, int64_t sr, const double* x,
-                            const double* lam, const double* omega, double* g, void* stream) {
+                            const double* lam, const double* omega, double* g, void* stream,
+                            bool static_staging = false) {
   if (!tables || !ws || !dofs || nrhs <= 0 || ndof <= 0 || !x || !lam || !omega || !g || E <= 0)
     PG_FAIL(PG_ERR_ARG, "bad arguments");
-  if (nrhs > 4096) PG_FAIL(PG_ERR_ARG, "at most 4096 right-hand sides per call");
+  if (nrhs > kAdjointMaxRhs)
+    PG_FAIL(PG_ERR_ARG, "at most %d right-hand sides per call (got %d)", kAdjointMaxRhs, nrhs);
   const Tables* t = static_cast<const Tables*>(tables);
-  std::vector<double> om(nrhs);
+  double om[kAdjointMaxRhs];
   for (int r = 0; r < nrhs; ++r) om[r] = omega[r] * kMu0;
   double* d_om = nullptr;
   hipStream_t s = as_stream(stream);
-  PG_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_om), nrhs * sizeof(double), s));
-  PG_HIP(hipMemcpyAsync(d_om, om.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, s));
+  if (static_staging) {
+    const int rc = scratch_address(&d_om);
+    if (rc != PG_OK) return rc;
+  } else {
+    PG_HIP(hipMallocAsync(reinterpret_cast<void**>(&d_om), nrhs * sizeof(double), s));
+  }
+  PG_HIP(hipMemcpyAsync(d_om, om, nrhs * sizeof(double), hipMemcpyHostToDevice, s));
   void* w = const_cast<void*>(ws);
   const dim3 gr(nblocks(E, 256)), bl(256);
   const double2* xx = reinterpret_cast<const double2*>(x);
@@ -840,7 +880,7 @@ static int adjoint_gradient(const void* tables, int64_t E, const void* ws, const
   PG_CHECK_LAUNCH();
   // the host copy above is synchronous with respect to `om` only once the stream drains
   PG_HIP(hipStreamSynchronize(s));
-  PG_HIP(hipFreeAsync(d_om, s));
+  if (!static_staging) PG_HIP(hipFreeAsync(d_om, s));
   return PG_OK;
 }
 
@@ -854,6 +894,14 @@ int pg_adjoint_gradient_rows(const void* tables, int64_t E, const void* ws, cons
                              int nrhs, int64_t ndof, const double* x, const double* lam,
                              const double* omega, double* g, void* stream) {
   return adjoint_gradient(tables, E, ws, dofs, nrhs, ndof, nrhs, 1, x, lam, omega, g, stream);
+}
+
+int pg_adjoint_gradient_noalloc(const void* tables, int64_t E, const void* ws,
+                                const int32_t* dofs, int nrhs, int64_t ndof, const double* x,
+                                const double* lam, const double* omega, double* g,
+                                void* stream) {
+  return adjoint_gradient(tables, E, ws, dofs, nrhs, ndof, 1, ndof, x, lam, omega, g, stream,
+                          true);
 }
 
 }  // extern "C"
