@@ -358,6 +358,16 @@ int pg_batch_set_mixed(void *batch, double inner_rtol, int stall_iters, void *st
  * Jacobi.  For every distinct frequency of the batch and every entity, the
  * inverse of the entity's block of K - i omega mu0 M (at most 6 x 6). */
 int pg_batch_set_block_jacobi(void *batch, void *stream);
+/* Optional: apply A_r with the element-by-element operator `eop` of the same
+ * system (pg_eop_create on the geometry workspace `workspace`, p = 1 or 2,
+ * PEC entities as in K and M) instead of the CSR products: per element and
+ * batch of columns the entries of K_e and M_e are formed once and applied to
+ * every column with its own frequency.  nrhs must be 64, 128, 192 or 256.
+ * The true residuals still use K and M.  The operator must outlive the batch;
+ * after new element geometry, pg_eop_refresh must accompany pg_batch_refresh.
+ * Replaces, like the batch itself, one KSP per right-hand side
+ * (petgem/solver.py:584-590). */
+int pg_batch_set_element_operator(void *batch, void *eop, const void *workspace, void *stream);
 /* Re-derive the Jacobi diagonals (block inverses, schedule-order copies of K
  * and M) after the caller has reassembled K and M in place. */
 int pg_batch_refresh(void *batch, void *stream);

@@ -93,6 +93,7 @@ SIGNATURES = {
     "pg_batch_set_mixed": (ctypes.c_int, [_P, _D, ctypes.c_int, _P]),
     "pg_batch_set_block_jacobi": (ctypes.c_int, [_P, _P]),
     "pg_batch_set_entities": (ctypes.c_int, [_P, _P, _I64, _P, _P]),
+    "pg_batch_set_element_operator": (ctypes.c_int, [_P, _P, _P, _P]),
     "pg_batch_destroy": (ctypes.c_int, [_P]),
     "pg_solver_profile": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P, _P]),
     "pg_multi_create": (ctypes.c_int, [ctypes.c_int, _P, _P, _P, _P, _P, _D, ctypes.c_int,

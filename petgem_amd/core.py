@@ -259,7 +259,10 @@ class BatchSolver:
 
     def __init__(self, plan, K, M, frequencies, rtol=1e-8, maxit=100000, block_ptr=None,
                  entity_order=None, precision="fp64", inner_rtol=1e-4, stall_iters=0,
-                 pc="jacobi", stream=None):
+                 pc="jacobi", eop=None, stream=None):
+        """eop: an ElementOperator of the same system (p = 1, 2; 64, 128, 192 or
+        256 right-hand sides): the iterations apply A_r element by element,
+        forming each element's entries once for all columns."""
         self.plan, self.K, self.M = plan, K, M
         self.omega = np.array([omega_of(f) for f in frequencies], dtype=np.float64)
         self.nrhs = self.omega.size
@@ -290,9 +293,15 @@ class BatchSolver:
                                                    int(stall_iters), nat.stream_handle(stream)))
         elif precision != "fp64":
             raise ValueError(f"precision must be fp64 or mixed, not {precision}")
+        self.eop = eop  # kept alive as long as the batch
+        if eop is not None:
+            nat.check(nat.lib().pg_batch_set_element_operator(self.handle, eop.handle,
+                                                              nat.ptr(eop._ws),
+                                                              nat.stream_handle(stream)))
 
     def refresh(self, stream=None):
-        """Re-derive the Jacobi diagonals after K and M were reassembled in place."""
+        """Re-derive the Jacobi diagonals after K and M were reassembled in place
+        (with an element operator, refresh the operator first)."""
         nat.check(nat.lib().pg_batch_refresh(self.handle, nat.stream_handle(stream)))
 
     def solve(self, B, stream=None):

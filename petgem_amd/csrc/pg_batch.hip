@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "pg_common.h"
+#include "pg_eop.h"
 
 namespace pg {
 namespace batch {
@@ -257,6 +258,94 @@ __global__ void __launch_bounds__(kBS) k_spmm_er(const Ctl* ctl, int64_t nent, i
             v[t][0] += pv.x * qv.x - pv.y * qv.y;
             v[t][1] += pv.x * qv.y + pv.y * qv.x;
           }
+  }
+  column_partials<CT, 2, TMAX>(R, v, part);
+}
+
+// q_r = A_r p_r from the element operator's local results: for every entity of
+// its schedule the sum of the entity's local results over its elements, in
+// ascending slot order (atomic-free, fixed order), PEC entities identity rows;
+// with the partials of mu_r = p_r^T q_r.  gm[i] = {r0, d, s0, s1}; the local
+// result of slot g, entity DOF t, column r is ye[(g + 64 t) R + r].
+template <int CT, int TMAX, int D, typename VT>
+__global__ void __launch_bounds__(kBS) k_meop_sum(const Ctl* ctl, int64_t nent, int R,
+                                                  const int4* __restrict__ gm,
+                                                  const int32_t* __restrict__ gslot,
+                                                  const VT* __restrict__ ye,
+                                                  const VT* __restrict__ P,
+                                                  VT* __restrict__ Q, double* part) {
+  if (ctl->all_done) return;
+  constexpr int EPB = kBS / CT;
+  const int c = threadIdx.x % CT, es = threadIdx.x / CT;
+  const int T = R / CT;
+  const Sweep sw = xcd_sweep(nent, EPB);
+  double v[TMAX][2];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) v[t][0] = v[t][1] = 0.0;
+  for (int64_t ent = sw.first + es; ent < sw.end; ent += sw.stride) {
+    const int4 m = gm[ent];
+    const int r0 = m.x, d = m.y, s0 = m.z, s1 = m.w;
+    // all column groups of an incidence together (up to TMAX * D independent
+    // loads per step), two incidences per step, in ascending slot order
+    VT acc[TMAX][D];
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+#pragma unroll
+      for (int k = 0; k < D; ++k) acc[t][k] = VT{0, 0};
+    if (s0 == s1) {
+#pragma unroll
+      for (int t = 0; t < TMAX; ++t)
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+          if (t < T && k < d) acc[t][k] = P[(int64_t)(r0 + k) * R + c + t * CT];
+    } else {
+      int q = s0;
+      for (; q + 2 <= s1; q += 2) {
+        const int64_t g0 = gslot[q], g1 = gslot[q + 1];
+        VT a[TMAX][D], b[TMAX][D];
+#pragma unroll
+        for (int t = 0; t < TMAX; ++t)
+#pragma unroll
+          for (int k = 0; k < D; ++k)
+            if (t < T && k < d) {
+              a[t][k] = ye[(g0 + (int64_t)kWave * k) * R + c + t * CT];
+              b[t][k] = ye[(g1 + (int64_t)kWave * k) * R + c + t * CT];
+            }
+#pragma unroll
+        for (int t = 0; t < TMAX; ++t)
+#pragma unroll
+          for (int k = 0; k < D; ++k)
+            if (t < T && k < d) {
+              acc[t][k].x += a[t][k].x;
+              acc[t][k].y += a[t][k].y;
+              acc[t][k].x += b[t][k].x;
+              acc[t][k].y += b[t][k].y;
+            }
+      }
+      if (q < s1) {
+        const int64_t g0 = gslot[q];
+#pragma unroll
+        for (int t = 0; t < TMAX; ++t)
+#pragma unroll
+          for (int k = 0; k < D; ++k)
+            if (t < T && k < d) {
+              const VT a = ye[(g0 + (int64_t)kWave * k) * R + c + t * CT];
+              acc[t][k].x += a.x;
+              acc[t][k].y += a.y;
+            }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        if (t < T && k < d) {
+          const int64_t idx = (int64_t)(r0 + k) * R + c + t * CT;
+          Q[idx] = acc[t][k];
+          const double2 pv = to_d(P[idx]), qv = to_d(acc[t][k]);
+          v[t][0] += pv.x * qv.x - pv.y * qv.y;
+          v[t][1] += pv.x * qv.y + pv.y * qv.x;
+        }
   }
   column_partials<CT, 2, TMAX>(R, v, part);
 }
@@ -1026,6 +1115,14 @@ struct Batch {
   std::vector<double> hom;  // omega * mu0 per column (host)
   float2 *R32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *Z32 = nullptr, *X32 = nullptr;
   float *Ks32 = nullptr, *Ms32 = nullptr;
+  // element-by-element products (optional, pg_batch_set_element_operator):
+  // the operator, its summation schedule, the local results per precision and
+  // the summation grid
+  const ElemOp* eop = nullptr;
+  EopSchedule esched{};
+  double2* ye64 = nullptr;
+  float2* ye32 = nullptr;
+  int sgrid = 1;
   std::vector<void*> owned;
   ~Batch() {  // release errors cannot be reported from here
     for (void* v : owned) (void)hipFree(v);
@@ -1049,7 +1146,21 @@ static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q,
   const dim3 g(B->grid), b(kBS), gr(B->R);
   hipLaunchKernelGGL((k_pupdate<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->n, B->R, Z, P);
   int np = B->grid;
-  if (B->em) {
+  if (B->eop) {
+    const int32_t* done = &B->ctl->all_done;
+    VT* ye = nullptr;
+    if constexpr (sizeof(VT) == sizeof(float2)) ye = B->ye32;
+    else ye = B->ye64;
+    eop_multi_local(B->eop, B->R, done, B->omu, P, ye, st);
+    const EopSchedule& es = B->esched;
+    np = B->sgrid;
+    if (es.D <= 1)
+      hipLaunchKernelGGL((k_meop_sum<CT, TM, 1, VT>), dim3(B->sgrid), b, 0, st, B->ctl, es.nent,
+                         B->R, es.gm, es.gslot, ye, P, Q, B->part);
+    else
+      hipLaunchKernelGGL((k_meop_sum<CT, TM, 2, VT>), dim3(B->sgrid), b, 0, st, B->ctl, es.nent,
+                         B->R, es.gm, es.gslot, ye, P, Q, B->part);
+  } else if (B->em) {
     const dim3 ge(B->egrid);
     np = B->egrid;
     if (B->dmax <= 1)
@@ -1430,6 +1541,9 @@ int pg_batch_set_mixed(void* batch, double inner_rtol, int stall_iters, void* st
       (rc = B->alloc(&B->X32, nr)) != PG_OK || (rc = B->alloc(&B->Ks32, B->nnz)) != PG_OK ||
       (rc = B->alloc(&B->Ms32, B->nnz)) != PG_OK)
     return rc;
+  if (B->eop &&
+      (rc = B->alloc(&B->ye32, (size_t)B->esched.nw * kWave * B->esched.N * B->R)) != PG_OK)
+    return rc;
   const double d = (inner_rtol > 0.0 && inner_rtol < 1.0) ? inner_rtol : 1e-4;
   B->delta2 = d * d;
   B->stall = stall_iters > 0 ? stall_iters : 400;
@@ -1468,6 +1582,32 @@ int pg_batch_set_block_jacobi(void* batch, void* stream) {
   const int rc2 = refresh_diag(B, as_stream(stream));
   if (rc2 != PG_OK) B->binv = nullptr;  // stays allocated; the batch keeps point Jacobi
   return rc2;
+}
+
+int pg_batch_set_element_operator(void* batch, void* op, const void* workspace, void* stream) {
+  if (!batch || !op || !workspace) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  Batch* B = static_cast<Batch*>(batch);
+  ElemOp* E = static_cast<ElemOp*>(op);
+  if (B->eop) PG_FAIL(PG_ERR_ARG, "element operator already set");
+  if (B->R < 64 || B->R % 64 != 0)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "element products in a batch need 64, 128, 192 or 256 columns "
+                                "(got %d)", B->R);
+  if (eop_ndof(E) != B->n) PG_FAIL(PG_ERR_ARG, "element operator of another system");
+  hipStream_t st = as_stream(stream);
+  int rc = eop_multi_prepare(E, workspace, st);
+  if (rc != PG_OK) return rc;
+  const EopSchedule es = eop_schedule(E);
+  const size_t nye = (size_t)es.nw * kWave * es.N * B->R;
+  if ((rc = B->alloc(&B->ye64, nye)) != PG_OK) return rc;
+  if (B->mixed && (rc = B->alloc(&B->ye32, nye)) != PG_OK) return rc;
+  const int64_t epb = kBS / B->CT;
+  int64_t g = std::min<int64_t>(kMaxGrid, (es.nent + epb - 1) / epb);
+  if (g >= 64) g = (g / 8) * 8;
+  B->sgrid = (int)std::max<int64_t>(1, g);
+  B->esched = es;
+  B->eop = E;
+  PG_HIP(hipStreamSynchronize(st));
+  return PG_OK;
 }
 
 int pg_batch_refresh(void* batch, void* stream) {

@@ -27,4 +27,25 @@ void eop_launch(const ElemOp* op, int mode, const int32_t* done, const double2* 
 void eop_launch(const ElemOp* op, int mode, const int32_t* done, const float2* x, float2* y,
                 const double2* b, double* part, hipStream_t st, hipEvent_t mid = nullptr);
 
+// Multi-right-hand-side products for the batched solver (p = 1, 2): column r
+// of a row-major (ndof, R) block gets its own frequency.  eop_multi_prepare
+// (once; `ws` = the geometry workspace the operator was created from) keeps
+// raw coefficients that pg_eop_refresh updates from then on.
+// eop_multi_local writes the local results ye (nw * 64 * N * R values) of
+// every element and column; the batch sums them per entity with the schedule
+// below (local result of slot g, entity DOF t, column r at ye[(g + 64 t) R + r];
+// s0 == s1 marks a PEC entity, an identity row).  R = 64, 128, 192 or 256.
+struct EopSchedule {
+  const int4* gm;        // per scheduled entity {first DOF, d, s0, s1}
+  const int32_t* gslot;  // local-result slots, ascending per entity
+  int64_t nent, nw;
+  int p, N, D;
+};
+int eop_multi_prepare(ElemOp* op, const void* ws, hipStream_t st);
+EopSchedule eop_schedule(const ElemOp* op);
+void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
+                     const double2* x, double2* ye, hipStream_t st);
+void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
+                     const float2* x, float2* ye, hipStream_t st);
+
 }  // namespace pg

@@ -148,3 +148,52 @@ def test_batch_columns_match_oracle(n, p, nsrc, freqs, precision, pc):
         print(f"batch {precision} {pc} column {r} ({f} Hz): x err {xerr:.2e}, field err "
               f"{ferr:.2e}", flush=True)
         assert xerr <= 1e-6 and ferr <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n,p,nsrc,freqs", [(4, 2, 16, [0.1, 0.25, 0.5, 1.0]),
+                                            (4, 1, 32, [0.25, 0.5, 1.0, 2.0]),
+                                            (3, 2, 32, [0.5, 1.0])])
+def test_batch_on_element_operator_matches_csr_and_oracle(n, p, nsrc, freqs, precision):
+    """The batched solver applying A_r element by element (entries formed once
+    per element for all columns, each column with its own frequency) against
+    the same batch on the CSR products, and three columns against SuperLU."""
+    w, mesh, asm, ent_bc, dof_bc = _setup(n, p)
+    K, M = core.assemble_km(asm, ent_bc)
+    xs = np.linspace(-600.0, 600.0, nsrc)
+    srcs = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
+    el = core.locate_points(mesh, srcs).cpu().numpy()
+    cols, meta = [], []
+    for f in freqs:
+        for s in range(nsrc):
+            b = core.source_vector(mesh, int(el[s]), srcs[s], (1.0, 0.0, 0.0), core.omega_of(f))
+            core.apply_dirichlet(asm.plan, None, dof_bc, b)
+            cols.append(b)
+            meta.append((srcs[s], f))
+    B = torch.stack(cols, dim=1).contiguous()
+    kw = dict(rtol=1e-10, block_ptr=mesh.block_ptr(), entity_order=mesh.entity_order(),
+              precision=precision, pc="bjacobi")
+    fr = [f for _, f in meta]
+    ref, info_r = core.BatchSolver(asm.plan, K, M, fr, **kw).solve(B)
+    eop = core.ElementOperator(asm, core.omega_of(1.0), ent_bc, mesh.entity_order())
+    bs = core.BatchSolver(asm.plan, K, M, fr, eop=eop, **kw)
+    X, info = bs.solve(B)
+    print(f"batch on element products ({precision}, R={len(fr)}, p={p}): iterations "
+          f"{info['iterations'].min()}..{info['iterations'].max()} (CSR "
+          f"{info_r['iterations'].min()}..{info_r['iterations'].max()}), relres "
+          f"{info['relres'].max():.2e}", flush=True)
+    assert (info["status"] == 0).all() and info["relres"].max() <= 1e-10
+    Xh, Rh = core.as_complex(X), core.as_complex(ref)
+    assert np.abs(Xh - Rh).max() <= 1e-7 * np.abs(Rh).max()
+    rx_el = core.locate_points(mesh, w.receivers)
+    for r in (0, len(meta) // 2, len(meta) - 1):
+        pos, f = meta[r]
+        o = _oracle_column(w, pos, f)
+        xr = X[:, r, :].contiguous()
+        fields = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, xr,
+                                                      core.omega_of(f)))
+        assert np.abs(core.as_complex(xr) - o["x"]).max() <= 1e-6 * np.abs(o["x"]).max()
+        assert np.abs(fields - o["fields"]).max() <= 1e-6 * np.abs(o["fields"]).max()
+    # a second solve on the same batch (local-result buffers reused)
+    X2, info2 = bs.solve(B)
+    assert torch.equal(X2, X)
