@@ -148,51 +148,64 @@ def measured_traffic(config):
         return parse_traffic_table(f.read())
 
 
-def cpu_baseline(w, plan_rowptr, plan_cols, values_c128, b_c128, iters_fp64, budget_s):
-    """Oracle restatement timed on the host (single thread) on a bounded sample."""
-    from threadpoolctl import threadpool_limits
-    import scipy.sparse as sp
-    from oracle import fem, krylov, topology as otopo
+def cpu_baseline(w, plan_rowptr, plan_cols):
+    """The CPU restatement in C + OpenMP (oracle/cpu_baseline.c) on every host
+    core OpenMP is given: element matrices in the reference's quadrature form,
+    CSR assembly, PEC condition and COCG + Jacobi to a true relative residual
+    of 1e-8 on the full workload, timed end to end (no sampling, no
+    extrapolation).  Beside it, the same on C1 (p = 1, 20^3) and SciPy's
+    SuperLU on C1."""
+    import scipy.sparse.linalg as spla
+    from oracle import cpu, fem, topology as otopo
+    from petgem_amd.synthetic import CONFIGS, Workload
 
-    T = w.topo
-    with threadpool_limits(1):
-        # (a) element matrices + assembly: a sample of elements, extrapolated linearly
-        ns = 3000
-        sel = np.arange(ns)
-        eo, fo = otopo.orientations(T.elems_nodes[sel], T.elems_edges[sel], T.edges_nodes,
-                                    T.elem_faces_edges[sel])
-        t0 = time.perf_counter()
-        M, K = fem.all_element_matrices(T.coords[sel], eo, fo, w.sigma[sel], T.p)
-        fem.assemble(T.dofs[sel], M, K, w.frequency, T.ndof)
-        t_asm = (time.perf_counter() - t0) * T.n_elems / ns
-        # (b) COCG + Jacobi iterations with scipy CSR SpMV on the full system
-        A = sp.csr_matrix((values_c128, plan_cols, plan_rowptr), shape=(T.ndof, T.ndof))
-        dinv = 1.0 / A.diagonal()
-        k = 3
-        while True:
-            t0 = time.perf_counter()
-            krylov.cocg(A, b_c128, dinv, tol=1e-300, maxit=k)
-            dt = time.perf_counter() - t0
-            if dt > 0.3 * budget_s or k >= 400:
-                break
-            k = max(k + 1, int(k * min(8.0, 0.35 * budget_s / max(dt, 1e-3))))
-        t_it = dt / k
-    total = t_asm + t_it * iters_fp64
-    return dict(value=total, unit="s", cores=1, kind="port",
-                sample=(f"oracle (numpy/scipy) on 1 host thread: element matrices+assembly of "
-                        f"{ns} of {T.n_elems} elements extrapolated linearly ({t_asm:.1f} s), plus "
-                        f"{k} COCG+Jacobi iterations with scipy CSR SpMV on the full system "
-                        f"({t_it*1e3:.1f} ms/iter) x {iters_fp64} iterations (the GPU FP64 count)"),
-                assembly_s=t_asm, per_iteration_s=t_it)
+    def run(wl, rowptr, cols):
+        T = wl.topo
+        eo, fo = otopo.orientations(T.elems_nodes, T.elems_edges, T.edges_nodes,
+                                    T.elem_faces_edges)
+        S = cpu.System(T.coords, wl.sigma, eo, fo, T.dofs, T.ndof, T.p, rowptr, cols)
+        b = cpu.source(T.coords, T.dofs, eo, fo, T.p, T.ndof, wl.source)
+        return S, b, cpu.forward(S, wl.frequency, b, T.boundary_dofs, tol=1e-8)
+
+    S, b, r = run(w, plan_rowptr, plan_cols)
+    if r["relres"] > 1e-8:
+        raise RuntimeError(f"CPU baseline did not converge: {r['relres']}")
+    # C1: the same restatement, and SuperLU on the same system
+    w1 = Workload(**CONFIGS["c1"], seed=2026)
+    T1 = w1.topo
+    ip, ci = fem.csr_pattern(T1.dofs, T1.ndof)
+    S1, b1, r1 = run(w1, ip, ci)
+    bb = S1.dirichlet(T1.boundary_dofs, b1)
+    t0 = time.perf_counter()
+    x1 = spla.spsolve(S1.matrix().tocsc(), bb[:, 0] + 1j * bb[:, 1])
+    t_lu = time.perf_counter() - t0
+    lu_diff = float(np.abs(x1 - r1["x"]).max() / np.abs(x1).max())
+    return dict(value=r["total_s"], unit="s", cores=r["threads"], kind="port",
+                sample=(f"full workload on {r['threads']} OpenMP threads (oracle/cpu_baseline.c): "
+                        f"assembly {r['assembly_s']:.2f} s, PEC {r['bc_s']:.2f} s, COCG + Jacobi "
+                        f"{r['iterations']} iterations to true relres {r['relres']:.1e} in "
+                        f"{r['solve_s']:.1f} s; no extrapolation"),
+                assembly_s=r["assembly_s"], solve_s=r["solve_s"], iterations=r["iterations"],
+                c1={"total_s": round(r1["total_s"], 3), "iterations": r1["iterations"],
+                    "relres": r1["relres"], "superlu_s": round(t_lu, 3),
+                    "superlu_vs_cocg_rel_diff": lu_diff})
 
 
 def batched_leg(args, log):
-    """Many right-hand sides: --batch-positions of 64 HED positions (100 m apart
-    along x at z = -950 m) x 4 frequencies on the 30^3 p=2 mesh, solved (a) by the
-    batched multi-RHS solver in chunks of --batch-width columns and (b) by the
-    single-RHS solver (mixed precision, entity-block Jacobi, autotuned SpMV: the
-    element products at p = 2), two right-hand sides at a time on two streams.
-    `rhs_per_s` is the faster."""
+    """C4 batching: 64 HED positions (100 m apart along x at z = -950 m) x 4
+    frequencies = 256 right-hand sides on the 30^3 p=2 mesh.
+
+    (a) The batched multi-RHS solver, one --batch-width-column batch per chunk
+        of one frequency's positions: COCG per column (--batch-precision),
+        entity-block Jacobi, the columns' products applied element by element
+        with each element's entries formed once for all columns (the element
+        operator in the batch).  Every right-hand side is solved and timed,
+        K and M assembled once per batch.
+    (b) For comparison, the single-RHS solver (mixed precision, entity-block
+        Jacobi, element products) two right-hand sides at a time on two
+        streams, timed on --single-positions of the positions x 4 frequencies
+        (the rate of independent solves).
+    `rhs_per_s` is the batched solver's rate on all right-hand sides."""
     import torch
     from petgem_amd import core
     from petgem_amd.synthetic import CONFIGS, Workload
@@ -206,51 +219,53 @@ def batched_leg(args, log):
     sigma = torch.from_numpy(w.sigma).to(dev)
     K = torch.empty(asm.plan.nnz, dtype=torch.float64, device=dev)
     M = torch.empty_like(K)
-    xs = -3150.0 + 100.0 * np.arange(64)
-    xs = xs[np.linspace(0, 63, args.batch_positions).round().astype(int)]
+    xs_all = -3150.0 + 100.0 * np.arange(64)
+    xs = xs_all[np.linspace(0, 63, args.batch_positions).round().astype(int)]
     srcs = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
     src_el = core.locate_points(mesh, srcs).cpu().numpy()
     freqs = [0.1, 0.25, 0.5, 1.0]
-    cols = [(si, f) for f in freqs for si in range(len(xs))]
-    width = args.batch_width or min(64, len(xs))
-    chunks = [cols[i:i + width] for i in range(0, len(cols), width)]
+    width = args.batch_width
+    chunks = []
+    for f in freqs:  # one frequency per batch: each batch stops when its columns converge
+        cf = [(si, f) for si in range(len(xs))]
+        chunks += [cf[i:i + width] for i in range(0, len(cf), width)]
+    ncols = sum(len(ch) for ch in chunks)
     asm.geometry(sigma)
     core.assemble_km(asm, ent_bc, out=(K, M))
     bptr, order = mesh.block_ptr(), mesh.entity_order()
+    eop = core.ElementOperator(asm, core.omega_of(1.0), ent_bc, order)
+    use_eop = width in (64, 128, 192, 256)
     solvers = [core.BatchSolver(asm.plan, K, M, [f for _, f in ch], rtol=args.rtol,
                                 block_ptr=bptr, entity_order=order, pc="bjacobi",
-                                precision=args.batch_precision) for ch in chunks]
+                                precision=args.batch_precision,
+                                eop=eop if use_eop and len(ch) == width else None)
+               for ch in chunks]
     B = torch.zeros((T.ndof, width, 2), dtype=torch.float64, device=dev)
-    times, iters = [], []
-    for it in range(1):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        asm.geometry(sigma)
-        core.assemble_km(asm, ent_bc, out=(K, M))
-        its, worst = [], 0.0
-        for ch, bs in zip(chunks, solvers):
-            bs.refresh()
-            for c, (si, f) in enumerate(ch):
-                b = core.source_vector(mesh, int(src_el[si]), srcs[si], (1.0, 0.0, 0.0),
-                                       core.omega_of(f))
-                core.apply_dirichlet(asm.plan, None, dof_bc, b)
-                B[:, c, :] = b
-            X, info = bs.solve(B)
-            if (info["status"] != 0).any():
-                raise RuntimeError(f"batched solve did not converge: {info}")
-            its.append(info["iterations"])
-            worst = max(worst, float(info["relres"].max()))
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        log(f"batched step {it}: {dt*1e3:.1f} ms for {len(cols)} RHS, iterations per chunk "
-            f"{[int(i.max()) for i in its]}, worst relres {worst:.2e}")
-        times.append(dt)
-        iters = [int(i.max()) for i in its]
-    t = float(np.median(times))
-    # the same right-hand sides through the single-RHS solver (mixed precision,
-    # entity-block Jacobi, element products when faster), two at a time on two
-    # streams (each with its own solver and element operator), per frequency
-    seq_t = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    asm.geometry(sigma)
+    core.assemble_km(asm, ent_bc, out=(K, M))
+    eop.refresh(core.omega_of(1.0))
+    its, worst = [], 0.0
+    for ch, bs in zip(chunks, solvers):
+        bs.refresh()
+        for c, (si, f) in enumerate(ch):
+            b = core.source_vector(mesh, int(src_el[si]), srcs[si], (1.0, 0.0, 0.0),
+                                   core.omega_of(f))
+            core.apply_dirichlet(asm.plan, None, dof_bc, b)
+            B[:, c, :] = b
+        X, info = bs.solve(B[:, :len(ch)].contiguous() if len(ch) < width else B)
+        if (info["status"] != 0).any():
+            raise RuntimeError(f"batched solve did not converge: {info}")
+        its.append(int(info["iterations"].max()))
+        worst = max(worst, float(info["relres"].max()))
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    log(f"batched: {t:.2f} s for {ncols} RHS ({ncols / t:.2f} RHS/s), iterations per batch "
+        f"{its}, worst relres {worst:.2e}")
+    # the single-RHS solver, two at a time on two streams, on a subset of positions
+    sel = np.linspace(0, len(xs) - 1, min(args.single_positions, len(xs))).round().astype(int)
+    seq_t, nseq = 0.0, 0
     vals = torch.empty((asm.plan.nnz, 2), dtype=torch.float64, device=dev)
     for f in freqs:
         asm.assemble(core.omega_of(f), ent_bc=ent_bc, out=vals)
@@ -261,7 +276,7 @@ def batched_leg(args, log):
                                         rtol=args.rtol, block_ptr=bptr, entity_order=order,
                                         eop=e, like=Ss[0] if Ss else None))
         rhs = []
-        for si in range(len(xs)):
+        for si in sel:
             b = core.source_vector(mesh, int(src_el[si]), srcs[si], (1.0, 0.0, 0.0),
                                    core.omega_of(f))
             core.apply_dirichlet(asm.plan, None, dof_bc, b)
@@ -271,22 +286,26 @@ def batched_leg(args, log):
         _, infos = core.solve_concurrently(Ss, rhs)
         torch.cuda.synchronize()
         seq_t += time.perf_counter() - t0
+        nseq += len(rhs)
         for inf in infos:
             if inf["status"] != 0:
                 raise RuntimeError(f"single solve did not converge: {inf}")
         del Ss, eops
-    log(f"batched: {t:.1f} s per batch; single-RHS solves two at a time (mixed, block Jacobi) "
-        f"{seq_t:.1f} s")
-    best = "batched_solver" if t <= seq_t else "single_rhs_two_streams"
-    return {"rhs_per_s": round(len(cols) / min(t, seq_t), 3), "mode": best,
-            "batched_solver_rhs_per_s": round(len(cols) / t, 3), "s_per_batch": round(t, 4),
-            "nrhs": len(cols), "width": width, "chunk_iterations": iters,
-            "single_rhs_two_streams_rhs_per_s": round(len(cols) / seq_t, 3),
+    log(f"single-RHS solves two at a time (mixed, block Jacobi, element products): {nseq} RHS in "
+        f"{seq_t:.2f} s ({nseq / seq_t:.2f} RHS/s)")
+    return {"rhs_per_s": round(ncols / t, 3), "mode": "batched_solver",
+            "s_total": round(t, 3), "nrhs": ncols, "width": width,
+            "precision": args.batch_precision, "element_products": use_eop,
+            "batch_iterations": its, "worst_relres": worst,
+            "single_rhs_two_streams_rhs_per_s": round(nseq / seq_t, 3),
+            "single_rhs_sample": int(nseq),
+            "speedup_vs_single_rhs_two_streams": round((ncols / t) / (nseq / seq_t), 3),
             "workload": (f"c4: p=2, 30^3 Kuhn x 6 = {T.n_elems} tets, {T.ndof} DOFs, "
-                         f"{asm.plan.nnz} nnz; {len(xs)} of the 64 HED x-positions (100 m apart "
-                         f"at z=-950 m) x {freqs} Hz, rtol {args.rtol}, COCG + entity-block "
-                         f"Jacobi per column ({args.batch_precision}), K/M assembled once per "
-                         "batch")}
+                         f"{asm.plan.nnz} nnz; {len(xs)} HED x-positions (100 m apart at "
+                         f"z=-950 m) x {freqs} Hz = {ncols} RHS, rtol {args.rtol}; batches of "
+                         f"{width} columns of one frequency, COCG + entity-block Jacobi per "
+                         f"column ({args.batch_precision}), element products; K/M assembled "
+                         "once")}
 
 
 def multimesh_leg(args, log):
@@ -328,7 +347,7 @@ def multimesh_leg(args, log):
     ndof = int(sum(pl.ndof for pl, _ in systems))
     ms = core.MultiMeshSolver(systems, rtol=args.rtol, check_every=25, use_graph=True)
     times = []
-    for it in range(1 + max(1, args.steps)):
+    for it in range(1 + min(2, max(1, args.steps))):  # a warm-up and at most two timed runs
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         xs, info = ms.solve(bs)
@@ -438,8 +457,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--cpu-budget", type=float, default=20.0,
-                    help="seconds of host work for the CPU baseline sample (0 = skip)")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline (a full solve on the host cores, about a minute)")
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--frequency", type=float, default=None,
                     help="source frequency of the solve leg (default: the workload's, 1 Hz)")
@@ -447,12 +466,14 @@ def main():
     ap.add_argument("--legs", default="solve,batch,multimesh,adjoint",
                     help="comma list of: solve (headline), batch (RHS/s), multimesh (RHS/s), "
                          "adjoint (grad/s)")
-    ap.add_argument("--batch-width", type=int, default=0,
-                    help="columns per batched solve (0: one frequency per batch, at most 64)")
-    ap.add_argument("--batch-precision", default="fp64", choices=["fp64", "mixed"],
+    ap.add_argument("--batch-width", type=int, default=64,
+                    help="columns per batched solve (chunks of one frequency's positions)")
+    ap.add_argument("--batch-precision", default="mixed", choices=["fp64", "mixed"],
                     help="batched and adjoint legs: FP64, or complex64 under FP64 refinement")
-    ap.add_argument("--batch-positions", type=int, default=16,
+    ap.add_argument("--batch-positions", type=int, default=64,
                     help="HED positions of the batched leg (of 64; x 4 frequencies)")
+    ap.add_argument("--single-positions", type=int, default=16,
+                    help="positions (x 4 frequencies) of the single-RHS comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU sharding of the "
@@ -563,13 +584,10 @@ def main():
     log(f"assembly {t_asm_ms:.2f} ms")
 
     cpu = None
-    if args.cpu_budget > 0:
-        log("cpu baseline ...")
-        vals_h = core.as_complex(values)
-        b_h = core.as_complex(b)
-        cpu = cpu_baseline(w, plan.h_rowptr, plan.h_cols, vals_h, b_h,
-                           results["fp64"]["info"]["iterations"], args.cpu_budget)
-        log(f"cpu baseline {cpu['value']:.1f} s")
+    if not args.no_cpu:
+        log("cpu baseline (full solve on the host cores) ...")
+        cpu = cpu_baseline(w, plan.h_rowptr, plan.h_cols)
+        log(f"cpu baseline {cpu['value']:.1f} s: {cpu['sample']}; C1 {cpu['c1']}")
 
     legs = set(args.legs.split(","))
     del solvers, values
@@ -621,7 +639,8 @@ def main():
         "multimesh": multimesh,
         "adjoint": adjoint,
         "cpu_baseline": None if cpu is None else {
-            k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+            k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample", "assembly_s",
+                                "solve_s", "iterations", "c1")},
         "host": platform.processor() or platform.machine(),
     }
     if cpu is not None:
