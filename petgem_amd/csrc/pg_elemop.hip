@@ -383,9 +383,10 @@ __global__ void __launch_bounds__(kBS) k_eop_sum(int64_t nent, const int32_t* __
 // ---------------------------------------------------------------------------
 // multi-right-hand-side local products for the batched solver: column r of the
 // (ndof, R) row-major block x gets y_e,r = (K_e - i omega_r mu0 M_e) x_e,r.
-// One workgroup of R threads (thread = column; R = 64 .. 256) per slot group w
-// of the single-RHS layout (64 elements of one orientation combination), which
-// takes its elements one after another.  Per element the threads first form
+// One workgroup of R threads (thread = column; R = 64 .. 256) per part of a
+// slot group w of the single-RHS layout (64 elements of one orientation
+// combination, split into 256 / R parts), which takes its elements one after
+// another.  Per element the threads first form
 // the n(n+1)/2 entry pairs (K_ij, M_ij) from the raw coefficients and the
 // combination's table (pdesc[k] = {table offset, zero mask} of pair k) into
 // LDS, double buffered so that one barrier per element suffices; then every
@@ -397,7 +398,8 @@ __global__ void __launch_bounds__(kBS) k_eop_sum(int64_t nent, const int32_t* __
 // ---------------------------------------------------------------------------
 template <int P, typename RT>
 __global__ void __launch_bounds__(256) k_meop_local(
-    int64_t nw, int R, const int32_t* __restrict__ done, const int32_t* __restrict__ wcombo,
+    int64_t nw, int R, int split, const int32_t* __restrict__ done,
+    const int32_t* __restrict__ wcombo,
     const int32_t* __restrict__ edofs, const double* __restrict__ coefm,
     const double* __restrict__ tab, const int2* __restrict__ pdesc,
     const double* __restrict__ omu, const typename Cx<RT>::T* __restrict__ x,
@@ -407,17 +409,25 @@ __global__ void __launch_bounds__(256) k_meop_local(
   constexpr int NP = N * (N + 1) / 2;
   constexpr int NLE = n_local_entities<P>();
   if (done && *done) return;
+  constexpr int NTS = table_stride<P>();
   __shared__ C ent[2][NP];
+  __shared__ double tl[NTS];  // the combination's table, staged once per workgroup
+  // workgroup b: part b % split of slot group b / split (`split` workgroups per
+  // group, so that narrow batches still launch several waves per SIMD)
   const int64_t nb = gridDim.x;
-  int64_t w = blockIdx.x;
-  if (nb % 8 == 0) w = (w % 8) * (nb / 8) + w / 8;  // XCD-contiguous ranges of slot groups
+  int64_t b = blockIdx.x;
+  if (nb % 8 == 0) b = (b % 8) * (nb / 8) + b / 8;  // XCD-contiguous ranges of slot groups
+  const int64_t w = b / split;
+  const int part = (int)(b - w * split);
   if (w >= nw) return;
   const int r = threadIdx.x;
   const RT om = (RT)omu[r];
   const int combo = wcombo[w];
-  constexpr int NTS = table_stride<P>();
   const double* __restrict__ t = tab + (int64_t)combo * NTS;
-  for (int l = 0; l < kWave; ++l) {
+  for (int k = threadIdx.x; k < NTS; k += blockDim.x) tl[k] = t[k];
+  __syncthreads();
+  const int l0 = part * (kWave / split), l1 = l0 + kWave / split;
+  for (int l = l0; l < l1; ++l) {
     int fv[NLE];
     bool any = false;
 #pragma unroll
@@ -427,14 +437,17 @@ __global__ void __launch_bounds__(256) k_meop_local(
     }
     if (!any) continue;  // empty slot (or every entity PEC): no products; uniform branch
     C* e = ent[l & 1];
+    double cfl[12];  // the element's raw coefficients (uniform: scalar loads)
+#pragma unroll
+    for (int c = 0; c < 12; ++c) cfl[c] = coefm[(w * 12 + c) * kWave + l];
     for (int k = threadIdx.x; k < NP; k += blockDim.x) {
       const int2 pd = pdesc[k];
       int o = pd.x;
       double m = 0.0, kk = 0.0;
       for (int c = 0; c < 6; ++c)
-        if (!((pd.y >> c) & 1)) m = fma(coefm[(w * 12 + c) * kWave + l], t[o++], m);
+        if (!((pd.y >> c) & 1)) m = fma(cfl[c], tl[o++], m);
       for (int c = 6; c < 12; ++c)
-        if (!((pd.y >> c) & 1)) kk = fma(coefm[(w * 12 + c) * kWave + l], t[o++], kk);
+        if (!((pd.y >> c) & 1)) kk = fma(cfl[c], tl[o++], kk);
       e[k] = C{(RT)kk, (RT)m};
     }
     C xv[N];
@@ -649,14 +662,17 @@ template <typename RT>
 static void multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
                         const typename eop::Cx<RT>::T* x, typename eop::Cx<RT>::T* ye,
                         hipStream_t st) {
-  const dim3 g((unsigned)op->nw), b(R);
+  // workgroups of R threads; a slot group split over 256 / R workgroups (at
+  // most 4), about 4 waves per slot group at every width
+  const int split = std::max(1, std::min(4, 256 / R));
+  const dim3 g((unsigned)(op->nw * split)), b(R);
   const double* tab = op->tab64;
   if (op->p == 1)
-    hipLaunchKernelGGL((eop::k_meop_local<1, RT>), g, b, 0, st, op->nw, R, done, op->wcombo,
-                       op->edofs, op->coefm, tab, op->pdesc, omu, x, ye);
+    hipLaunchKernelGGL((eop::k_meop_local<1, RT>), g, b, 0, st, op->nw, R, split, done,
+                       op->wcombo, op->edofs, op->coefm, tab, op->pdesc, omu, x, ye);
   else
-    hipLaunchKernelGGL((eop::k_meop_local<2, RT>), g, b, 0, st, op->nw, R, done, op->wcombo,
-                       op->edofs, op->coefm, tab, op->pdesc, omu, x, ye);
+    hipLaunchKernelGGL((eop::k_meop_local<2, RT>), g, b, 0, st, op->nw, R, split, done,
+                       op->wcombo, op->edofs, op->coefm, tab, op->pdesc, omu, x, ye);
 }
 void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
                      const double2* x, double2* ye, hipStream_t st) {
