@@ -73,6 +73,8 @@ def spmv_key(code, dmax, vt, p=2):
     solver's configuration code kind*1000 + G*10 + U."""
     kind, g, u = code // 1000, (code % 1000) // 10, code % 10
     if kind == 3:  # element products (p = 2: paired gathers on the solver's aligned vectors)
+        if p == 3 and vt != "float2":
+            return "k_eop_mfma3"  # p = 3 complex128: the matrix-core kernel
         return "k_eop_local<{},{},{}>".format(p, "float" if vt == "float2" else "double",
                                               "true" if p == 2 else "false")
     if kind == 2:

@@ -300,9 +300,8 @@ int pg_solver_profile(void *solver, int reps, double *ms64, double *ms32, int32_
  * assembled matrix (the element matrices are recomputed from the geometry
  * workspace and the reference tables at every product).  The same product as
  * MatMult on the assembled system (petgem/solver.py:584-590, inside KSPSolve),
- * to rounding.  Orders p = 1, 2 (complex128 and complex64) and 3 (complex64:
- * pg_eop_apply_z returns PG_ERR_UNSUPPORTED, and a solver uses the operator
- * for its complex64 iterations only).
+ * to rounding.  Orders p = 1, 2 and 3, complex128 and complex64 (the p = 3
+ * complex128 products on the FP64 matrix cores).
  *   ws        pg_element_geometry workspace (read at creation and refresh)
  *   dofs      device (E, n) element DOFs, as for the assembly plan
  *   block_ptr host (nblocks+1) DOF ranges of the mesh entities

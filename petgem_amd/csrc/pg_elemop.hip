@@ -18,7 +18,8 @@
 //               in a fixed order (atomic-free, bitwise reproducible), and
 //               fuses the COCG dot product or the residual.
 //
-// Orders p = 1, 2 and 3 (n = 6, 20, 45; p = 3 in complex64 only).  The
+// Orders p = 1, 2 and 3 (n = 6, 20, 45), complex128 and complex64; the p = 3
+// complex128 products run on the FP64 matrix cores (k_eop_mfma3).  The
 // summation order differs from the assembled CSR's, so results agree with the
 // CSR SpMV to rounding.
 #include <hip/hip_runtime.h>
@@ -284,6 +285,110 @@ __global__ void __launch_bounds__(local_bs<R>(), (min_waves<P, R>())) k_eop_loca
 }
 
 // ---------------------------------------------------------------------------
+// p = 3 complex128 element products on the FP64 matrix cores.  With 45 DOFs
+// the values of x and of the products take 360 registers per lane in
+// complex128, so the lane-per-element kernel above does not fit; here one
+// wave takes one slot group w (64 elements of one combination) and forms
+//     Y (48 x 128) = Tf (48 x K) * B (K x 128)
+// with v_mfma_f64_16x16x4f64: column n < 64 of Y is element n's real part,
+// column 64 + n its imaginary part (rows 45..47 padding); column k = (c, j)
+// of Tf is table component c of the pairs (i, j) (both triangles), and
+//     B[(c, j)][n]      = cf_n[c] re x_n[j]   (c curl-curl) or -cf_n[c] im x_n[j] (c mass)
+//     B[(c, j)][64 + n] = cf_n[c] im x_n[j]   (c curl-curl) or  cf_n[c] re x_n[j] (c mass)
+// (mass coefficients carry -omega mu0, as in k_eop_local).  Only the columns
+// (c, j) that are not zero for every orientation are kept (kcol, padded to a
+// multiple of 4 with -1).  The element's x and coefficients are staged in LDS
+// (52 KB per wave); the fragments of Tf are per combination in fragment order
+// afrag[combo][ks][m-tile][lane] (L2-resident).  Operand layout of the
+// instruction (measured): A lane l = (row l % 16, k l / 16), B lane l =
+// (k l / 16, column l % 16), D lane l, value r = (row 4 r + l / 16, column l % 16).
+// ---------------------------------------------------------------------------
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaXStride = 46;  // complex values per element row in LDS (45 + 1 pad)
+
+__global__ void __launch_bounds__(kWave) k_eop_mfma3(int64_t nw, const int32_t* __restrict__ done,
+                                                     const int32_t* __restrict__ wcombo,
+                                                     const int32_t* __restrict__ edofs,
+                                                     const double* __restrict__ coef,
+                                                     const double* __restrict__ afrag,
+                                                     const int32_t* __restrict__ kcol, int nks,
+                                                     const double2* __restrict__ x,
+                                                     double2* __restrict__ ye) {
+  constexpr int P = 3, N = 45, NLE = n_local_entities<P>();
+  __shared__ double2 xs[kWave * kMfmaXStride];
+  __shared__ double cfs[kWave * 12];
+  if (done && *done) return;
+  const int64_t nb = gridDim.x;
+  int64_t w = blockIdx.x;
+  if (nb % 8 == 0) w = (w % 8) * (nb / 8) + w / 8;  // XCD-contiguous ranges of slot groups
+  if (w >= nw) return;
+  const int lane = threadIdx.x;
+  const int combo = __builtin_amdgcn_readfirstlane(wcombo[w]);
+  // stage this lane's element: coefficients and x (zero for PEC / empty)
+#pragma unroll
+  for (int c = 0; c < 12; ++c) cfs[lane * 12 + c] = coef[(w * 12 + c) * kWave + lane];
+#pragma unroll
+  for (int la = 0; la < NLE; ++la) {
+    const int f = edofs[(w * NLE + la) * kWave + lane];
+#pragma unroll
+    for (int k = 0; k < loc_nd<P>(la); ++k)
+      xs[lane * kMfmaXStride + loc_off<P>(la) + k] = f >= 0 ? x[f + k] : double2{0.0, 0.0};
+  }
+  __syncthreads();
+  mfma_d4 acc[3][8];
+#pragma unroll
+  for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) acc[mt][nt] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+  const double* __restrict__ af = afrag + (int64_t)combo * nks * 3 * kWave;
+  const int kr = lane / 16, en = lane % 16;
+  for (int ks = 0; ks < nks; ++ks) {
+    const double a0 = af[(ks * 3 + 0) * kWave + lane];
+    const double a1 = af[(ks * 3 + 1) * kWave + lane];
+    const double a2 = af[(ks * 3 + 2) * kWave + lane];
+    const int kc = kcol[ks * 4 + kr];
+    double bre[4], bim[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bre[q] = bim[q] = 0.0;
+    if (kc >= 0) {
+      const int c = kc >> 8, j = kc & 255;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = q * 16 + en;
+        const double2 xv = xs[e * kMfmaXStride + j];
+        const double cv = cfs[e * 12 + c];
+        if (c >= 6) {
+          bre[q] = cv * xv.x;
+          bim[q] = cv * xv.y;
+        } else {
+          bre[q] = -(cv * xv.y);
+          bim[q] = cv * xv.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      acc[0][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bre[q], acc[0][q], 0, 0, 0);
+      acc[1][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bre[q], acc[1][q], 0, 0, 0);
+      acc[2][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bre[q], acc[2][q], 0, 0, 0);
+      acc[0][q + 4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bim[q], acc[0][q + 4], 0, 0, 0);
+      acc[1][q + 4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bim[q], acc[1][q + 4], 0, 0, 0);
+      acc[2][q + 4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bim[q], acc[2][q + 4], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < 3; ++mt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = mt * 16 + 4 * r + kr;
+      if (i < N)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          ye[((int64_t)w * N + i) * kWave + q * 16 + en] = double2{acc[mt][q][r], acc[mt][q + 4][r]};
+    }
+}
+
+// ---------------------------------------------------------------------------
 // per-entity sums.  gm[i] = {r0, d, s0, s1} for the i-th scheduled entity:
 // rows r0 .. r0+d, local results at gslot[s0 .. s1) (+ t * 64 for DOF t);
 // s0 == s1 marks a PEC entity (identity rows).  MODE as in the CSR SpMV.
@@ -533,6 +638,11 @@ struct ElemOp {
   // the combination's table, zero mask}
   double* coefm = nullptr;
   int2* pdesc = nullptr;
+  // p = 3 complex128 products on the matrix cores (k_eop_mfma3): Tf fragments
+  // per combination and the kept (c, j) columns
+  double* afrag = nullptr;
+  int32_t* kcol = nullptr;
+  int nks = 0;
   std::vector<void*> owned;
 
   template <typename T>
@@ -551,7 +661,9 @@ struct ElemOp {
 };
 
 int eop_grid(const ElemOp* op) { return op->grid; }
-bool eop_supports(const ElemOp* op, bool complex128) { return !(complex128 && op->p == 3); }
+bool eop_supports(const ElemOp* op, bool complex128) {
+  return !(complex128 && op->p == 3 && !op->afrag);
+}
 int64_t eop_ndof(const ElemOp* op) { return op->ndof; }
 
 template <typename R>
@@ -570,11 +682,17 @@ static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
     hipLaunchKernelGGL((eop::k_eop_local<1, R, false>), gl, bll, 0, st, op->nw, done, op->wcombo,
                        op->edofs, coef, tab, x, ye);
   } else if (op->p == 3) {
-    // complex64 only (the complex128 kernel would need 360 registers for p and
-    // the products); eop_supports() keeps the solver from asking for it
-    if constexpr (sizeof(R) == 4)
+    // complex64: one lane per element; complex128: the matrix-core kernel (the
+    // values of x and of the products do not fit one lane)
+    if constexpr (sizeof(R) == 4) {
       hipLaunchKernelGGL((eop::k_eop_local<3, R, false>), gl, bll, 0, st, op->nw, done,
                          op->wcombo, op->edofs, coef, tab, x, ye);
+    } else {
+      int64_t nbm = op->nw;
+      if (nbm >= 64) nbm = (nbm + 7) / 8 * 8;
+      hipLaunchKernelGGL(eop::k_eop_mfma3, dim3((unsigned)nbm), dim3(kWave), 0, st, op->nw,
+                         done, op->wcombo, op->edofs, coef, op->afrag, op->kcol, op->nks, x, ye);
+    }
   } else if (pair_ok)
     hipLaunchKernelGGL((eop::k_eop_local<2, R, true>), gl, bll, 0, st, op->nw, done, op->wcombo,
                        op->edofs, coef, tab, x, ye);
@@ -867,6 +985,44 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
     if (o != (size_t)c * NTS + NT) PG_FAIL(PG_ERR_UNSUPPORTED, "element table size mismatch");
   }
   std::vector<float> tab32(tab.begin(), tab.end());
+  // p = 3: matrix-core fragments.  kcol = the (c, j) columns of Tf that are not
+  // zero for every orientation (c << 8 | j), padded to a multiple of 4 with -1;
+  // afrag[combo][ks][mt][lane] = Tf[mt * 16 + lane % 16][kcol[4 ks + lane / 16]]
+  std::vector<int32_t> kcol;
+  std::vector<double> afrag;
+  int nks = 0;
+  if (p == 3) {
+    const int NPp = N * (N + 1) / 2;
+    std::vector<int> poff(NPp + 1, 0);
+    for (int k = 0; k < NPp; ++k) poff[k + 1] = poff[k] + 12 - eop::popcount12(eop::zero_mask<3>(k));
+    auto pair_of = [&](int i, int j) {
+      const int a = std::min(i, j), b2 = std::max(i, j);
+      return a * N - (a * (a - 1)) / 2 + (b2 - a);
+    };
+    for (int c = 0; c < 12; ++c)
+      for (int j = 0; j < N; ++j) {
+        bool any = false;
+        for (int i = 0; i < N && !any; ++i) any = !((eop::zero_mask<3>(pair_of(i, j)) >> c) & 1);
+        if (any) kcol.push_back((c << 8) | j);
+      }
+    while (kcol.size() % 4) kcol.push_back(-1);
+    nks = (int)kcol.size() / 4;
+    afrag.assign((size_t)nc * nks * 3 * kWave, 0.0);
+    for (int cb = 0; cb < nc; ++cb)
+      for (int ks = 0; ks < nks; ++ks)
+        for (int mt = 0; mt < 3; ++mt)
+          for (int l = 0; l < kWave; ++l) {
+            const int i = mt * 16 + l % 16, kc = kcol[ks * 4 + l / 16];
+            if (i >= N || kc < 0) continue;
+            const int c = kc >> 8, j = kc & 255, k = pair_of(i, j);
+            const unsigned zm = eop::zero_mask<3>(k);
+            if ((zm >> c) & 1) continue;
+            int rank = 0;  // position of component c among the pair's kept components
+            for (int q = 0; q < c; ++q) rank += !((zm >> q) & 1);
+            afrag[(((size_t)cb * nks + ks) * 3 + mt) * kWave + l] =
+                tab[(size_t)cb * NTS + poff[k] + rank];
+          }
+  }
 
   ElemOp* op = new ElemOp();
   op->p = p;
@@ -902,6 +1058,11 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
   PG_A(op->tab32, tab.size());
   PG_A(op->ye64, nslot * N);
   PG_A(op->ye32, nslot * N);
+  if (p == 3) {
+    PG_A(op->afrag, afrag.size());
+    PG_A(op->kcol, kcol.size());
+    op->nks = nks;
+  }
 #undef PG_A
   const struct {
     void* dst;
@@ -913,7 +1074,9 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
             {op->gm, gm.data(), gm.size() * sizeof(int4)},
             {op->gslot, gslot.data(), gslot.size() * sizeof(int32_t)},
             {op->tab64, tab.data(), tab.size() * sizeof(double)},
-            {op->tab32, tab32.data(), tab32.size() * sizeof(float)}};
+            {op->tab32, tab32.data(), tab32.size() * sizeof(float)},
+            {op->afrag, afrag.data(), afrag.size() * sizeof(double)},
+            {op->kcol, kcol.data(), kcol.size() * sizeof(int32_t)}};
   for (const auto& u : up)
     if (u.bytes && hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
       delete op;
@@ -939,7 +1102,7 @@ int pg_eop_refresh(void* op, const void* ws, double omega, void* stream) {
 int pg_eop_apply_z(const void* op, const double* x, double* y, void* stream) {
   if (!op || !x || !y) PG_FAIL(PG_ERR_ARG, "bad arguments");
   if (!eop_supports(static_cast<const ElemOp*>(op), true))
-    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator at p=3: complex64 only");
+    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator without a complex128 product");
   eop_launch(static_cast<const ElemOp*>(op), 0, nullptr, reinterpret_cast<const double2*>(x),
              reinterpret_cast<double2*>(y), nullptr, nullptr, as_stream(stream));
   PG_CHECK_LAUNCH();

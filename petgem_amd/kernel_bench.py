@@ -5,7 +5,7 @@ kernel-trace or hardware-counter profiler sees a short, regular program:
 
 * the solve's SpMV (``q = A p`` plus the ``p^T q`` partials) in complex128 and
   complex64, with the CSR configuration the solver's autotuner picked, and
-  (p = 1, 2) the element-by-element operator's two kernels doing the same;
+  the element-by-element operator's kernels doing the same;
 * the numeric assembly (element geometry + fused CSR assembly);
 * a plain device copy of ``--copy-mib`` MiB (torch's copy kernel), whose
   counter readings calibrate the HBM byte counters (bytes read and written
@@ -67,8 +67,6 @@ def main(argv=None):
     kinds = [("spmv", args.spmv)] + ([("eop", "element")] if eop is not None else [])
     for key, kind in kinds:
         for prec in ("fp64", "mixed"):
-            if kind == "element" and prec == "fp64" and T.p == 3:
-                continue  # complex64 only at p = 3
             S = core.KrylovSolver(plan, values, precision=prec, pc=args.pc, spmv=kind,
                                   block_ptr=bptr, entity_order=order,
                                   eop=eop if kind == "element" else None)

@@ -279,8 +279,6 @@ def _oracle_forward(w):
                                                ("fp64", "bjacobi", "element"),
                                                ("mixed", "bjacobi", "element")])
 def test_forward_solve_vs_superlu(p, n, precision, pc, spmv):
-    if spmv == "element" and p == 3 and precision == "fp64":
-        pytest.skip("the element-by-element operator has no complex128 product at p = 3")
     w = Workload(n, p, noise=0.1, seed=7, n_rx=21)
     g = _forward_gpu(w, precision=precision, pc=pc, spmv=spmv)
     r = _oracle_forward(w)
