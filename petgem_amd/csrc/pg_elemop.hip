@@ -342,11 +342,18 @@ __global__ void __launch_bounds__(kWave) k_eop_mfma3(int64_t nw, const int32_t* 
     for (int nt = 0; nt < 8; ++nt) acc[mt][nt] = mfma_d4{0.0, 0.0, 0.0, 0.0};
   const double* __restrict__ af = afrag + (int64_t)combo * nks * 3 * kWave;
   const int kr = lane / 16, en = lane % 16;
+  // the fragments of Tf and the column of step ks + 1 are loaded during step ks
+  double na0 = af[lane], na1 = af[kWave + lane], na2 = af[2 * kWave + lane];
+  int nkc = kcol[kr];
   for (int ks = 0; ks < nks; ++ks) {
-    const double a0 = af[(ks * 3 + 0) * kWave + lane];
-    const double a1 = af[(ks * 3 + 1) * kWave + lane];
-    const double a2 = af[(ks * 3 + 2) * kWave + lane];
-    const int kc = kcol[ks * 4 + kr];
+    const double a0 = na0, a1 = na1, a2 = na2;
+    const int kc = nkc;
+    if (ks + 1 < nks) {
+      na0 = af[((ks + 1) * 3 + 0) * kWave + lane];
+      na1 = af[((ks + 1) * 3 + 1) * kWave + lane];
+      na2 = af[((ks + 1) * 3 + 2) * kWave + lane];
+      nkc = kcol[(ks + 1) * 4 + kr];
+    }
     double bre[4], bim[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) bre[q] = bim[q] = 0.0;
