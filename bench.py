@@ -465,9 +465,9 @@ def main():
     ap.add_argument("--frequency", type=float, default=None,
                     help="source frequency of the solve leg (default: the workload's, 1 Hz)")
     ap.add_argument("--pc", default="bjacobi", choices=["jacobi", "bjacobi"])
-    ap.add_argument("--legs", default="solve,batch,multimesh,adjoint",
-                    help="comma list of: solve (headline), batch (RHS/s), multimesh (RHS/s), "
-                         "adjoint (grad/s)")
+    ap.add_argument("--legs", default="solve,dropin,batch,multimesh,adjoint",
+                    help="comma list of: solve (headline), dropin (the reference's petsc.opts "
+                         "solver), batch (RHS/s), multimesh (RHS/s), adjoint (grad/s)")
     ap.add_argument("--batch-width", type=int, default=64,
                     help="columns per batched solve (chunks of one frequency's positions)")
     ap.add_argument("--batch-precision", default="mixed", choices=["fp64", "mixed"],
@@ -592,6 +592,33 @@ def main():
         log(f"cpu baseline {cpu['value']:.1f} s: {cpu['sample']}; C1 {cpu['c1']}")
 
     legs = set(args.legs.split(","))
+    dropin = None
+    if "dropin" in legs:
+        # the reference's own example options (petsc.opts: -ksp_type gmres -pc_type
+        # sor -ksp_rtol 1e-8) as the drop-in Solver maps them: GMRES(30), FP64,
+        # entity-block Jacobi for SOR, products autotuned (element operator)
+        Sg = core.KrylovSolver(plan, values, ksp="gmres", pc="bjacobi", precision="fp64",
+                               rtol=args.rtol, restart=30, maxit=30000, block_ptr=bptr,
+                               entity_order=order, eop=eop)
+        xg = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        assemble()
+        eop.refresh(omega)
+        Sg.refresh()
+        xg, ig = Sg.solve(b, xg)
+        torch.cuda.synchronize()
+        tg = time.perf_counter() - t0
+        fg = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, xg, omega))
+        dropin = {"s": round(tg, 4), "iterations": ig["iterations"], "relres": ig["relres"],
+                  "status": ig["status"], "spmv": spmv_key(Sg.profile(reps=3)["g64"], dmax,
+                                                           "double2", T.p),
+                  "receiver_field_diff_vs_cocg_fp64": float(np.abs(fg - f64).max()
+                                                            / np.abs(f64).max()),
+                  "options": "-ksp_type gmres -pc_type sor -ksp_rtol 1e-8 (petsc.opts:2-4) -> "
+                             "GMRES(30) FP64, entity-block Jacobi, true-residual stop"}
+        log(f"drop-in petsc.opts (GMRES(30) + bjacobi): {dropin}")
+        del Sg
     del solvers, values
     torch.cuda.empty_cache()
     batched = batched_leg(args, log) if "batch" in legs else None
@@ -637,6 +664,7 @@ def main():
         "receiver_field_diff_mixed_vs_fp64": field_diff,
         "assembly_ms": round(t_asm_ms, 3),
         "roofline": roof64,
+        "dropin_petsc_opts": dropin,
         "batched": batched,
         "multimesh": multimesh,
         "adjoint": adjoint,

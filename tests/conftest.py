@@ -13,6 +13,16 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X) and the built library")
+    config.addinivalue_line("markers", "slow: minutes of GPU time; runs when PETGEM_AMD_SLOW=1")
+
+
+def pytest_collection_modifyitems(config, items):
+    if os.environ.get("PETGEM_AMD_SLOW") == "1":
+        return
+    skip = pytest.mark.skip(reason="slow test: set PETGEM_AMD_SLOW=1 to run it")
+    for item in items:
+        if "slow" in item.keywords:
+            item.add_marker(skip)
 
 
 @pytest.fixture(scope="session")

@@ -116,6 +116,41 @@ def test_adjoint_fd_twenty_elements_c5_setup():
     print(f"central differences on 20 elements: worst |fd - g| / |g| = {worst:.2e}", flush=True)
 
 
+@pytest.mark.slow
+@pytest.mark.timeout(1800)
+def test_adjoint_fd_twenty_elements_c5_full_size():
+    """C5 at its full size: p = 2 on 30^3 Kuhn hexes (162 000 parameters), 8
+    HED x 2 frequencies, 101 receivers, observations from the model with the
+    resistive block, gradient at the model without it; central differences
+    (h = 1e-4) on 20 seeded elements drawn from those whose gradient is above
+    1 % of the largest.  (The FP64 CPU oracle gradient is checked on 8^3 above:
+    SuperLU at 1 M DOFs is out of reach of a test.)"""
+    w_true = Workload(30, 2, block=True)
+    w0 = Workload(30, 2, block=False)
+    T = w_true.topo
+    xs = np.linspace(-1750.0, 1750.0, 8)
+    src = np.stack([xs, np.zeros_like(xs), np.full_like(xs, -950.0)], axis=1)
+    rx = receiver_line(101, z=-1000.0)
+    m0 = np.log(w0.sigma[:, 0])
+    prob = InversionProblem(T, src, [0.25, 1.0], rx, rtol=1e-12, precision="fp64")
+    prob.observed = prob.forward(np.log(w_true.sigma[:, 0]))
+    phi, g = prob.misfit_and_gradient(m0)
+    g = g.cpu().numpy()
+    h = 1e-4
+    worst = 0.0
+    cand = np.flatnonzero(np.abs(g) >= 1e-2 * np.abs(g).max())
+    for k, e in enumerate(np.random.default_rng(30).choice(cand, 20, replace=False)):
+        mp, mm = m0.copy(), m0.copy()
+        mp[e] += h
+        mm[e] -= h
+        fd = (prob.misfit(prob.forward(mp)) - prob.misfit(prob.forward(mm))) / (2 * h)
+        worst = max(worst, abs(fd - g[e]) / abs(g[e]))
+        print(f"element {e}: fd {fd:.9e} adjoint {g[e]:.9e} ({k + 1}/20)", flush=True)
+        assert abs(fd - g[e]) <= 1e-3 * abs(g[e]) + 2e-4 * np.abs(g).max(), (e, fd, g[e])
+    print(f"C5 30^3: phi {phi:.9e}, central differences on 20 elements: worst "
+          f"|fd - g| / |g| = {worst:.2e}", flush=True)
+
+
 def test_concurrent_per_rhs_solves_are_bitwise_sequential():
     """Right-hand sides three at a time on three streams (per-RHS mode; the
     workers' solvers copy the first one's kernel choices) give bitwise the
