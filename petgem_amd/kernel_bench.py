@@ -36,7 +36,8 @@ def main(argv=None):
                     help="entity-row SpMV in natural entity order (default: Morton order)")
     ap.add_argument("--batch-width", type=int, default=0,
                     help="also run the batched solver on the 30^3 p=2 mesh with this many "
-                         "columns for --batch-iters iterations (0 = skip)")
+                         "columns for --batch-iters iterations (0 = skip); CSR products, and "
+                         "the element operator at 64, 128, 192 or 256 columns")
     ap.add_argument("--batch-iters", type=int, default=50)
     ap.add_argument("--batch-precision", default="fp64", choices=["fp64", "mixed"])
     args = ap.parse_args(argv)
@@ -131,13 +132,17 @@ def batch_run(width, iters, precision="fp64"):
         B[:, c, :] = b
     out = {}
     bptr = mesh.block_ptr()
-    runs = (("row", None, None), ("entity_natural", bptr, None),
-            ("entity_morton", bptr, mesh.entity_order()))
+    order = mesh.entity_order()
+    runs = (("row", None, None, False), ("entity_natural", bptr, None, False),
+            ("entity_morton", bptr, order, False))
     if precision == "mixed":
         runs = runs[1:]
-    for name, bp, od in runs:
+    if width in (64, 128, 192, 256):  # the element-by-element operator in the batch
+        runs = runs + (("element", bptr, order, True),)
+    eop = core.ElementOperator(asm, core.omega_of(1.0), ent_bc, order)
+    for name, bp, od, use_eop in runs:
         bs = core.BatchSolver(asm.plan, K, M, [1.0] * width, rtol=1e-30, maxit=iters, block_ptr=bp,
-                              entity_order=od, precision=precision)
+                              entity_order=od, precision=precision, eop=eop if use_eop else None)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
