@@ -313,17 +313,19 @@ def batched_leg(args, log):
 def multimesh_leg(args, log):
     """32 independent meshes, 8^3 .. 24^3 Kuhn hexes (p=2), seeded random layered
     conductivities (0.01 - 3.3 S/m), one HED each, solved in one multi-mesh batch
-    (HIP-graph replay; COCG + Jacobi, FP64, CSR); beside it the same systems one
-    after another with the same method, and through single-RHS solvers (mixed,
-    entity-block Jacobi, element products) four meshes at a time on four
-    streams.  `rhs_per_s` is the faster of the batch and the latter."""
+    (HIP-graph replay; COCG per mesh, FP64) with the element products of all
+    meshes in one launch and entity-block Jacobi; beside it the same batch on
+    CSR products and point Jacobi, the latter's systems one after another,
+    and single-RHS solvers (mixed, entity-block Jacobi, element products)
+    four meshes at a time on four streams.  `rhs_per_s` is the faster of the
+    element-operator batch and the four-stream single solves."""
     import torch
     from petgem_amd import core
     from petgem_amd.synthetic import Workload
 
     sizes = np.round(np.linspace(8, 24, 32)).astype(int)
     freqs = [0.1, 0.25, 0.5, 1.0]
-    systems, bs, fast = [], [], []
+    systems, bs, fast, elem = [], [], [], []
     for i, n in enumerate(sizes):
         w = Workload(int(n), 2, block=False, seed=i)
         rng = np.random.default_rng(1000 + i)
@@ -342,26 +344,37 @@ def multimesh_leg(args, log):
         systems.append((asm.plan, vals))
         bs.append(b)
         order = mesh.entity_order()
+        elem.append((asm, om, ent_bc, order))
         eop = core.ElementOperator(asm, om, ent_bc, order)
         fast.append(core.KrylovSolver(asm.plan, vals, precision="mixed", pc="bjacobi",
                                       rtol=args.rtol, block_ptr=mesh.block_ptr(),
                                       entity_order=order, eop=eop))
     ndof = int(sum(pl.ndof for pl, _ in systems))
+
+    def timed(ms, name):
+        times = []
+        for it in range(1 + min(2, max(1, args.steps))):  # a warm-up and at most two timed runs
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xs, info = ms.solve(bs)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if (info["status"] != 0).any():
+                raise RuntimeError(f"multi-mesh solve did not converge: {info}")
+            log(f"multi-mesh ({name}) step {it}: {dt*1e3:.1f} ms, iterations "
+                f"{info['iterations'].min()}..{info['iterations'].max()}, graph launches "
+                f"{info['launches']}")
+            if it >= 1:
+                times.append(dt)
+        return float(np.median(times)), info
+
+    ms_e = core.MultiMeshSolver(systems, rtol=args.rtol, check_every=25, use_graph=True,
+                                element=elem)
+    t_e, info_e = timed(ms_e, "element products, block Jacobi")
+    del ms_e
     ms = core.MultiMeshSolver(systems, rtol=args.rtol, check_every=25, use_graph=True)
-    times = []
-    for it in range(1 + min(2, max(1, args.steps))):  # a warm-up and at most two timed runs
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        xs, info = ms.solve(bs)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        if (info["status"] != 0).any():
-            raise RuntimeError(f"multi-mesh solve did not converge: {info}")
-        log(f"multi-mesh step {it}: {dt*1e3:.1f} ms, iterations {info['iterations'].min()}.."
-            f"{info['iterations'].max()}, graph launches {info['launches']}")
-        if it >= 1:
-            times.append(dt)
-    t = float(np.median(times))
+    t, info = timed(ms, "CSR, point Jacobi")
+    del ms
     singles = [core.KrylovSolver(pl, v, rtol=args.rtol) for pl, v in systems]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -379,19 +392,26 @@ def multimesh_leg(args, log):
     for inf in infs:
         if inf["status"] != 0:
             raise RuntimeError(f"single solve did not converge: {inf}")
-    log(f"multi-mesh: batch {t*1e3:.1f} ms vs one after another {t_seq*1e3:.1f} ms; single-RHS "
-        f"on element products, four at a time: {t_fast*1e3:.1f} ms")
-    best = "multi_mesh_batch" if t <= t_fast else "single_rhs_four_streams"
-    return {"rhs_per_s": round(len(systems) / min(t, t_fast), 3), "mode": best,
-            "batch_rhs_per_s": round(len(systems) / t, 3),
-            "single_rhs_four_streams_s": round(t_fast, 4), "s_per_batch": round(t, 4),
+    log(f"multi-mesh: batch on element products {t_e*1e3:.1f} ms, on CSR {t*1e3:.1f} ms vs "
+        f"one after another (CSR) {t_seq*1e3:.1f} ms; single-RHS on element products, four at "
+        f"a time: {t_fast*1e3:.1f} ms")
+    best = "multi_mesh_batch" if t_e <= t_fast else "single_rhs_four_streams"
+    return {"rhs_per_s": round(len(systems) / min(t_e, t_fast), 3), "mode": best,
+            "batch_rhs_per_s": round(len(systems) / t_e, 3), "s_per_batch": round(t_e, 4),
+            "batch_iterations_max": int(info_e["iterations"].max()),
+            "batch_graph_launches": info_e["launches"],
+            "speedup_vs_single_rhs_four_streams": round(t_fast / t_e, 3),
+            "single_rhs_four_streams_s": round(t_fast, 4),
+            "csr_batch_s": round(t, 4), "csr_batch_iterations_max": int(info["iterations"].max()),
             "meshes": len(systems), "total_dofs": ndof,
-            "iterations_max": int(info["iterations"].max()), "graph_launches": info["launches"],
-            "sequential_s": round(t_seq, 4), "speedup_vs_sequential": round(t_seq / t, 3),
+            "sequential_csr_s": round(t_seq, 4),
             "workload": ("32 meshes, 8^3..24^3 Kuhn hexes x 6 (p=2), seeded random 3-layer "
                          "conductivities 0.01-3.3 S/m, HED at (0,0,-950) m, 0.1/0.25/0.5/1 Hz "
-                         f"cyclic, rtol {args.rtol}, COCG + Jacobi per mesh; sequential = "
-                         "the same systems through the single-RHS solver (COCG + Jacobi); "
+                         f"cyclic, rtol {args.rtol}; batch = one COCG per mesh (FP64) with all "
+                         "meshes' element products in one launch and entity-block Jacobi, "
+                         "HIP-graph replay of 25 iterations between host checks; csr_batch = "
+                         "the same batch on CSR products and point Jacobi; sequential_csr = its "
+                         "systems one after another through the single-RHS solver; "
                          "single_rhs_four_streams = single-RHS solvers (mixed, entity-block "
                          "Jacobi, element products), four meshes at a time")}
 

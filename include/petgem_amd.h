@@ -313,6 +313,22 @@ int pg_solver_profile(void *solver, int reps, double *ms64, double *ms32, int32_
 int pg_eop_create(int p, int64_t E, const void *ws, const int32_t *dofs, int64_t ndof,
                   const int32_t *block_ptr, int64_t nblocks, const int32_t *order,
                   const uint8_t *ent_bc, double omega, void **op, void *stream);
+/* Multi-mesh operator: the block-diagonal union of nmesh meshes of one order p
+ * (mesh s: n_elems[s] elements, its geometry workspace ws[s], element DOFs
+ * dofs[s], ndof[s] DOFs, entity DOF ranges block_ptr[s] (host), entity
+ * schedule order[s] (host, nullable), PEC entities ent_bc[s] (nullable) and
+ * its own angular frequency omega[s]); the union numbers DOFs and entities
+ * mesh after mesh.  Every wavefront of its element products holds elements
+ * of one mesh, so a multi-mesh batch (pg_multi_set_element_operator) skips
+ * the meshes that have converged.  The coefficients are read once, here:
+ * pg_eop_refresh rejects such an operator.  Arrays of pointers are host
+ * arrays.  No counterpart in the reference (one mesh per run,
+ * petgem/solver.py:191-235). */
+int pg_eop_create_multi(int p, int nmesh, const int64_t *n_elems, const void *const *ws,
+                        const int32_t *const *dofs, const int64_t *ndof,
+                        const int32_t *const *block_ptr, const int64_t *nblocks,
+                        const int32_t *const *order, const uint8_t *const *ent_bc,
+                        const double *omega, void **op, void *stream);
 /* Re-read the element coefficients after pg_element_geometry (new conductivity)
  * or for a new frequency. */
 int pg_eop_refresh(void *op, const void *ws, double omega, void *stream);
@@ -391,6 +407,15 @@ int pg_multi_create(int nmesh, const int64_t *n, const int64_t *nnz,
  * graph launches (or iteration batches without a graph). */
 int pg_multi_solve(void *batch, const double *const *b, double *const *x, int32_t *iterations,
                    double *relres, int32_t *status, int32_t *launches, void *stream);
+/* Optional: iterate with the products of a multi-mesh element operator of the
+ * same meshes in the same order (pg_eop_create_multi, whose DOFs must be the
+ * batch's rows) and entity-block Jacobi (each mesh entity's diagonal block of
+ * the batch's CSR inverted) instead of the CSR products and point Jacobi.
+ * Each mesh keeps its own COCG recurrence and stops when it converges; its
+ * element products are skipped from then on.  True residuals still use the
+ * CSR systems.  The operator must outlive the batch.  Replaces, like the
+ * batch itself, one KSP per problem (petgem/solver.py:584-590). */
+int pg_multi_set_element_operator(void *batch, void *eop, void *stream);
 int pg_multi_destroy(void *batch);
 
 #ifdef __cplusplus

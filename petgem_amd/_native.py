@@ -74,6 +74,8 @@ SIGNATURES = {
                                                    _P, _P, _P]),
     "pg_eop_create": (ctypes.c_int, [ctypes.c_int, _I64, _P, _P, _I64, _P, _I64, _P, _P, _D,
                                      ctypes.POINTER(_P), _P]),
+    "pg_eop_create_multi": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, _P,
+                                           _P, _P, _P, ctypes.POINTER(_P), _P]),
     "pg_eop_refresh": (ctypes.c_int, [_P, _P, _D, _P]),
     "pg_eop_apply_z": (ctypes.c_int, [_P, _P, _P, _P]),
     "pg_eop_apply_c": (ctypes.c_int, [_P, _P, _P, _P]),
@@ -99,6 +101,7 @@ SIGNATURES = {
     "pg_multi_create": (ctypes.c_int, [ctypes.c_int, _P, _P, _P, _P, _P, _D, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P), _P]),
     "pg_multi_solve": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "pg_multi_set_element_operator": (ctypes.c_int, [_P, _P, _P]),
     "pg_multi_destroy": (ctypes.c_int, [_P]),
 }
 

@@ -326,13 +326,20 @@ class BatchSolver:
 
 
 class MultiMeshSolver:
-    """Independent meshes of uneven size solved together (one COCG + Jacobi per
+    """Independent meshes of uneven size solved together (one COCG recurrence per
     mesh, all advancing in the same launches; the iterations between host
-    checks replayed from a captured HIP graph)."""
+    checks replayed from a captured HIP graph).  Point Jacobi and CSR products,
+    or with `element` entity-block Jacobi and the element products of all
+    meshes in one launch."""
 
     def __init__(self, systems, rtol=1e-8, maxit=100000, check_every=25, use_graph=True,
-                 stream=None):
-        """systems: list of (plan, values) with values (nnz, 2) float64 on the device."""
+                 element=None, stream=None):
+        """systems: list of (plan, values) with values (nnz, 2) float64 on the device.
+        element: optional list, one per mesh in the order of `systems`, of
+        (assembler, omega, ent_bc, entity_order) of the same systems (the
+        assembler after its geometry; ent_bc and entity_order may be None)."""
+        self.handle = None
+        self.eop = None
         k = len(systems)
         self.n = np.array([pl.ndof for pl, _ in systems], dtype=np.int64)
         nnz = np.array([pl.nnz for pl, _ in systems], dtype=np.int64)
@@ -347,6 +354,38 @@ class MultiMeshSolver:
                                             nat.stream_handle(stream)))
         self.handle = h
         self.k = k
+        if element is not None:
+            self._set_element(element, stream)
+
+    def _set_element(self, element, stream):
+        if len(element) != self.k:
+            raise ValueError(f"need one (assembler, omega, ent_bc, order) per mesh ({self.k})")
+        meshes = [a.mesh for a, _, _, _ in element]
+        p = meshes[0].p
+        if any(m.p != p for m in meshes):
+            raise ValueError("the meshes of a multi-mesh element operator share one order p")
+        k = self.k
+        bps = [np.ascontiguousarray(m.block_ptr(), np.int32) for m in meshes]
+        ords = [None if o is None else np.ascontiguousarray(o, np.int32)
+                for _, _, _, o in element]
+        dev_arr = lambda ts: (ctypes.c_void_p * k)(*[nat.ptr(t) for t in ts])  # noqa: E731
+        host_arr = lambda xs: (ctypes.c_void_p * k)(  # noqa: E731
+            *[None if x is None else x.ctypes.data for x in xs])
+        ne = np.array([m.n_elems for m in meshes], dtype=np.int64)
+        nd = np.array([m.ndof for m in meshes], dtype=np.int64)
+        nb = np.array([b.size - 1 for b in bps], dtype=np.int64)
+        om = np.array([float(o) for _, o, _, _ in element], dtype=np.float64)
+        if not np.array_equal(nd, self.n):
+            raise ValueError("element operator meshes differ from the systems")
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().pg_eop_create_multi(
+            p, k, ne.ctypes.data, dev_arr([a.workspace for a, _, _, _ in element]),
+            dev_arr([m.dofs for m in meshes]), nd.ctypes.data, host_arr(bps), nb.ctypes.data,
+            host_arr(ords), dev_arr([e for _, _, e, _ in element]), om.ctypes.data,
+            ctypes.byref(h), nat.stream_handle(stream)))
+        self.eop = h
+        nat.check(nat.lib().pg_multi_set_element_operator(self.handle, self.eop,
+                                                          nat.stream_handle(stream)))
 
     def solve(self, bs, stream=None):
         """bs: list of (ndof_i, 2) float64 right-hand sides; returns xs, info."""
@@ -371,6 +410,9 @@ class MultiMeshSolver:
             if self.handle:
                 nat.lib().pg_multi_destroy(self.handle)
                 self.handle = None
+            if self.eop:  # after the batch that uses it
+                nat.lib().pg_eop_destroy(self.eop)
+                self.eop = None
         except Exception:
             pass
 

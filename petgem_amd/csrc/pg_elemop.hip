@@ -172,6 +172,7 @@ __device__ inline void write_partials(double (&v)[NV], double* part) {
 // ---------------------------------------------------------------------------
 template <int P, typename R, bool PAIR>
 __global__ void __launch_bounds__(local_bs<R>(), (min_waves<P, R>())) k_eop_local(int64_t nw, const int32_t* __restrict__ done,
+                                                   const int32_t* __restrict__ wseg, int dstride,
                                                    const int32_t* __restrict__ wcombo,
                                                    const int32_t* __restrict__ edofs,
                                                    const R* __restrict__ coef,
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(local_bs<R>(), (min_waves<P, R>())) k_eop_loca
   constexpr int N = P * (P + 2) * (P + 3) / 2;
   constexpr int NLE = n_local_entities<P>();
   static_assert(N == loc_off<P>(NLE - 1) + loc_nd<P>(NLE - 1), "entity DOFs");
-  if (done && *done) return;
+  if (done && !wseg && *done) return;
   // XCD-contiguous ranges of waves (workgroup b runs on XCD b % 8)
   const int64_t nb = gridDim.x;
   int64_t b = blockIdx.x;
@@ -191,6 +192,8 @@ __global__ void __launch_bounds__(local_bs<R>(), (min_waves<P, R>())) k_eop_loca
   const int64_t w =
       b * (local_bs<R>() / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (w >= nw) return;
+  // segmented operator: the wave's mesh has converged
+  if (wseg && done[(int64_t)__builtin_amdgcn_readfirstlane(wseg[w]) * dstride]) return;
   const int lane = threadIdx.x % kWave;
   const int combo = __builtin_amdgcn_readfirstlane(wcombo[w]);
   constexpr int NTS = table_stride<P>();  // constant-evaluated, not a loop at run time
@@ -307,6 +310,7 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 constexpr int kMfmaXStride = 46;  // complex values per element row in LDS (45 + 1 pad)
 
 __global__ void __launch_bounds__(kWave) k_eop_mfma3(int64_t nw, const int32_t* __restrict__ done,
+                                                     const int32_t* __restrict__ wseg, int dstride,
                                                      const int32_t* __restrict__ wcombo,
                                                      const int32_t* __restrict__ edofs,
                                                      const double* __restrict__ coef,
@@ -317,11 +321,12 @@ __global__ void __launch_bounds__(kWave) k_eop_mfma3(int64_t nw, const int32_t* 
   constexpr int P = 3, N = 45, NLE = n_local_entities<P>();
   __shared__ double2 xs[kWave * kMfmaXStride];
   __shared__ double cfs[kWave * 12];
-  if (done && *done) return;
+  if (done && !wseg && *done) return;
   const int64_t nb = gridDim.x;
   int64_t w = blockIdx.x;
   if (nb % 8 == 0) w = (w % 8) * (nb / 8) + w / 8;  // XCD-contiguous ranges of slot groups
   if (w >= nw) return;
+  if (wseg && done[(int64_t)wseg[w] * dstride]) return;  // segmented: the mesh has converged
   const int lane = threadIdx.x;
   const int combo = __builtin_amdgcn_readfirstlane(wcombo[w]);
   // stage this lane's element: coefficients and x (zero for PEC / empty)
@@ -606,18 +611,21 @@ __global__ void __launch_bounds__(256) k_meop_local(
 }
 
 // coefficients in slot order: mass part times -omega mu0, then curl-curl part
+// (multi-mesh operator: omega mu0 of the wave's segment, womu[wseg[w]])
 template <typename R>
 __global__ void k_eop_coef(int64_t nslot, const int32_t* __restrict__ perm,
                            const double* __restrict__ wscoef, double omega_mu,
+                           const int32_t* __restrict__ wseg, const double* __restrict__ womu,
                            R* __restrict__ coef) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslot) return;
   const int64_t w = s / kWave, lane = s % kWave;
   const int64_t e = perm[s];
+  const double om = womu ? womu[wseg[w]] : omega_mu;
 #pragma unroll
   for (int c = 0; c < 12; ++c) {
     double v = e >= 0 ? wscoef[12 * e + c] : 0.0;
-    if (c < 6) v = -(omega_mu * v);
+    if (c < 6) v = -(om * v);
     coef[(w * 12 + c) * kWave + lane] = (R)v;
   }
 }
@@ -650,6 +658,12 @@ struct ElemOp {
   double* afrag = nullptr;
   int32_t* kcol = nullptr;
   int nks = 0;
+  // multi-mesh operator (pg_eop_create_multi): segment of every wave, omega mu0
+  // of every segment, first schedule position of every segment (nseg + 1)
+  int nseg = 0;
+  int32_t* wseg = nullptr;
+  double* womu = nullptr;
+  std::vector<int64_t> seg_pos, seg_wave;  // + first wave of every segment (nseg + 1)
   std::vector<void*> owned;
 
   template <typename T>
@@ -673,39 +687,58 @@ bool eop_supports(const ElemOp* op, bool complex128) {
 }
 int64_t eop_ndof(const ElemOp* op) { return op->ndof; }
 
+// element products of waves [w0, w0 + nwr) (all waves: 0, op->nw)
+template <typename R>
+static void launch_local(const ElemOp* op, const int32_t* done, const int32_t* wseg, int dstride,
+                         const typename eop::Cx<R>::T* x, typename eop::Cx<R>::T* ye,
+                         const R* coef, const R* tab, hipStream_t st, int64_t w0 = 0,
+                         int64_t nwr = -1) {
+  using C = typename eop::Cx<R>::T;
+  if (nwr < 0) nwr = op->nw;
+  if (nwr <= 0) return;
+  const int nle = op->p == 1 ? 6 : (op->p == 2 ? 10 : 11);
+  const int32_t* wc = op->wcombo + w0;
+  const int32_t* ed = op->edofs + w0 * nle * kWave;
+  const int32_t* ws = wseg ? wseg + w0 : nullptr;
+  coef += w0 * 12 * kWave;
+  ye += w0 * op->N * kWave;
+  const int wpb = eop::local_bs<R>() / kWave;
+  int64_t nbl = (nwr + wpb - 1) / wpb;
+  if (nbl >= 64) nbl = (nbl + 7) / 8 * 8;
+  const dim3 gl((unsigned)nbl), bll(wpb * kWave);
+  const bool pair_ok =
+      op->even_entities && reinterpret_cast<uintptr_t>(x) % (2 * sizeof(C)) == 0;
+  if (op->p == 1) {
+    hipLaunchKernelGGL((eop::k_eop_local<1, R, false>), gl, bll, 0, st, nwr, done, ws, dstride,
+                       wc, ed, coef, tab, x, ye);
+  } else if (op->p == 3) {
+    // complex64: one lane per element; complex128: the matrix-core kernel (the
+    // values of x and of the products do not fit one lane)
+    if constexpr (sizeof(R) == 4) {
+      hipLaunchKernelGGL((eop::k_eop_local<3, R, false>), gl, bll, 0, st, nwr, done, ws,
+                         dstride, wc, ed, coef, tab, x, ye);
+    } else {
+      int64_t nbm = nwr;
+      if (nbm >= 64) nbm = (nbm + 7) / 8 * 8;
+      hipLaunchKernelGGL(eop::k_eop_mfma3, dim3((unsigned)nbm), dim3(kWave), 0, st, nwr, done,
+                         ws, dstride, wc, ed, coef, op->afrag, op->kcol, op->nks, x, ye);
+    }
+  } else if (pair_ok) {
+    hipLaunchKernelGGL((eop::k_eop_local<2, R, true>), gl, bll, 0, st, nwr, done, ws, dstride,
+                       wc, ed, coef, tab, x, ye);
+  } else {
+    hipLaunchKernelGGL((eop::k_eop_local<2, R, false>), gl, bll, 0, st, nwr, done, ws, dstride,
+                       wc, ed, coef, tab, x, ye);
+  }
+}
+
 template <typename R>
 static void launch_impl(const ElemOp* op, int mode, const int32_t* done,
                         const typename eop::Cx<R>::T* x, typename eop::Cx<R>::T* y,
                         const double2* b, double* part, hipStream_t st, hipEvent_t mid,
                         typename eop::Cx<R>::T* ye, const R* coef, const R* tab) {
   using C = typename eop::Cx<R>::T;
-  const int wpb = eop::local_bs<R>() / kWave;
-  int64_t nbl = (op->nw + wpb - 1) / wpb;
-  if (nbl >= 64) nbl = (nbl + 7) / 8 * 8;
-  const dim3 gl((unsigned)nbl), bll(wpb * kWave);
-  const bool pair_ok =
-      op->even_entities && reinterpret_cast<uintptr_t>(x) % (2 * sizeof(C)) == 0;
-  if (op->p == 1) {
-    hipLaunchKernelGGL((eop::k_eop_local<1, R, false>), gl, bll, 0, st, op->nw, done, op->wcombo,
-                       op->edofs, coef, tab, x, ye);
-  } else if (op->p == 3) {
-    // complex64: one lane per element; complex128: the matrix-core kernel (the
-    // values of x and of the products do not fit one lane)
-    if constexpr (sizeof(R) == 4) {
-      hipLaunchKernelGGL((eop::k_eop_local<3, R, false>), gl, bll, 0, st, op->nw, done,
-                         op->wcombo, op->edofs, coef, tab, x, ye);
-    } else {
-      int64_t nbm = op->nw;
-      if (nbm >= 64) nbm = (nbm + 7) / 8 * 8;
-      hipLaunchKernelGGL(eop::k_eop_mfma3, dim3((unsigned)nbm), dim3(kWave), 0, st, op->nw,
-                         done, op->wcombo, op->edofs, coef, op->afrag, op->kcol, op->nks, x, ye);
-    }
-  } else if (pair_ok)
-    hipLaunchKernelGGL((eop::k_eop_local<2, R, true>), gl, bll, 0, st, op->nw, done, op->wcombo,
-                       op->edofs, coef, tab, x, ye);
-  else
-    hipLaunchKernelGGL((eop::k_eop_local<2, R, false>), gl, bll, 0, st, op->nw, done, op->wcombo,
-                       op->edofs, coef, tab, x, ye);
+  launch_local<R>(op, done, nullptr, 0, x, ye, coef, tab, st);
   // a failed record only loses this timing sample (the elapsed-time query fails)
   if (mid) (void)hipEventRecord(mid, st);
   const dim3 g(op->grid), bl(eop::kBS);
@@ -743,18 +776,20 @@ static int eop_refresh(ElemOp* op, const void* ws, double omega, hipStream_t st)
   const double om = omega * eop::kMu0;
   const dim3 g((unsigned)((nslot + 255) / 256)), b(256);
   hipLaunchKernelGGL(eop::k_eop_coef<double>, g, b, 0, st, nslot, op->perm, wscoef, om,
-                     op->coef64);
+                     op->wseg, op->womu, op->coef64);
   hipLaunchKernelGGL(eop::k_eop_coef<float>, g, b, 0, st, nslot, op->perm, wscoef, om,
-                     op->coef32);
+                     op->wseg, op->womu, op->coef32);
   // raw coefficients: -(-1 * v) = v exactly
   if (op->coefm)
     hipLaunchKernelGGL(eop::k_eop_coef<double>, g, b, 0, st, nslot, op->perm, wscoef, -1.0,
-                       op->coefm);
+                       nullptr, nullptr, op->coefm);
   PG_CHECK_LAUNCH();
   return PG_OK;
 }
 
 int eop_multi_prepare(ElemOp* op, const void* ws, hipStream_t st) {
+  if (op->nseg > 0)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "multi-right-hand-side products of a multi-mesh operator");
   if (op->p > 2)
     PG_FAIL(PG_ERR_UNSUPPORTED, "multi-right-hand-side element products: p <= 2 only");
   if (op->coefm) return PG_OK;
@@ -774,13 +809,27 @@ int eop_multi_prepare(ElemOp* op, const void* ws, hipStream_t st) {
   const int64_t nslot = op->nw * kWave;
   const dim3 g((unsigned)((nslot + 255) / 256)), b(256);
   hipLaunchKernelGGL(eop::k_eop_coef<double>, g, b, 0, st, nslot, op->perm,
-                     reinterpret_cast<const double*>(ws), -1.0, op->coefm);
+                     reinterpret_cast<const double*>(ws), -1.0, nullptr, nullptr, op->coefm);
   PG_CHECK_LAUNCH();
   return PG_OK;
 }
 
 EopSchedule eop_schedule(const ElemOp* op) {
   return EopSchedule{op->gm, op->gslot, op->nent, op->nw, op->p, op->N, op->D};
+}
+
+int eop_segments(const ElemOp* op, const int64_t** seg_pos, const int64_t** seg_wave) {
+  if (seg_pos) *seg_pos = op->seg_pos.data();
+  if (seg_wave) *seg_wave = op->seg_wave.data();
+  return op->nseg;
+}
+
+const double2* eop_local_segments(const ElemOp* op, const int32_t* done, int dstride,
+                                  const double2* x, hipStream_t st, int64_t w0, int64_t w1) {
+  if (w1 < 0) w1 = op->nw;
+  launch_local<double>(op, done, op->wseg, dstride, x, op->ye64, op->coef64, op->tab64, st, w0,
+                       w1 - w0);
+  return op->ye64;
 }
 
 template <typename RT>
@@ -808,38 +857,38 @@ void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double*
   multi_local<float>(op, R, done, omu, x, ye, st);
 }
 
-}  // namespace pg
+// Operator set-up from host copies of the element DOFs (hd, E x n), element
+// orientation codes (hor), entity DOF ranges (bp), schedule (order, may be
+// null) and PEC entities (hbc); ws = the device geometry coefficients.  With
+// seg (a multi-mesh operator) the elements are grouped by segment first, so
+// that no wave of 64 elements spans two segments, and each segment gets its
+// own frequency.
+struct SegInfo {
+  int nseg = 0;
+  std::vector<int64_t> pos;  // first schedule position of each segment (nseg + 1)
+  std::vector<double> omega;
+};
 
-using namespace pg;
-
-extern "C" {
-
-int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t ndof,
-                  const int32_t* block_ptr, int64_t nblocks, const int32_t* order,
-                  const uint8_t* ent_bc, double omega, void** out, void* stream) {
-  if (!out || !ws || !dofs || !block_ptr || E <= 0 || ndof <= 0 || nblocks <= 0)
-    PG_FAIL(PG_ERR_ARG, "bad arguments");
-  if (p < 1 || p > 3)
-    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator: order p=%d not supported (1..3)", p);
-  const int N = ndof_element(p);
-  const int nle = p == 1 ? 6 : (p == 2 ? 10 : 11);  // local entities carrying DOFs
+// host checks of the entity DOF ranges (before any device work)
+static int check_block_ptr(int p, const int32_t* bp, int64_t nblocks, int64_t ndof) {
   const int maxd = p == 1 ? 1 : (p == 2 ? 2 : 6);
-  hipStream_t st = as_stream(stream);
-  const int32_t* bp = block_ptr;
   if (bp[0] != 0 || bp[nblocks] != ndof)
     PG_FAIL(PG_ERR_ARG, "block_ptr must start at 0 and end at ndof");
   for (int64_t g = 0; g < nblocks; ++g)
     if (bp[g + 1] - bp[g] < 1 || bp[g + 1] - bp[g] > maxd)
       PG_FAIL(PG_ERR_ARG, "entity sizes must be in 1..%d at p=%d", maxd, p);
-  // host copies of the element DOFs, orientations and PEC entities
-  std::vector<int32_t> hd((size_t)E * N), hor(E);
-  std::vector<uint8_t> hbc(nblocks, 0);
-  const int32_t* dor = reinterpret_cast<const int32_t*>(reinterpret_cast<const double*>(ws) + 12 * E);
-  PG_HIP(hipMemcpyAsync(hd.data(), dofs, hd.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  PG_HIP(hipMemcpyAsync(hor.data(), dor, E * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  if (ent_bc)
-    PG_HIP(hipMemcpyAsync(hbc.data(), ent_bc, nblocks, hipMemcpyDeviceToHost, st));
-  PG_HIP(hipStreamSynchronize(st));
+  return PG_OK;
+}
+
+static int eop_build(int p, int64_t E, const void* ws, int64_t ndof, const int32_t* bp,
+                     int64_t nblocks, const int32_t* order, const std::vector<int32_t>& hd,
+                     const std::vector<int32_t>& hor, const std::vector<uint8_t>& hbc,
+                     double omega, const SegInfo* seg, ElemOp** out, hipStream_t st) {
+  const int N = ndof_element(p);
+  const int nle = p == 1 ? 6 : (p == 2 ? 10 : 11);  // local entities carrying DOFs
+  const int maxd = p == 1 ? 1 : (p == 2 ? 2 : 6);
+  int rc0 = check_block_ptr(p, bp, nblocks, ndof);
+  if (rc0 != PG_OK) return rc0;
   std::vector<int32_t> ent_of(ndof);
   for (int64_t g = 0; g < nblocks; ++g)
     for (int d = bp[g]; d < bp[g + 1]; ++d) ent_of[d] = (int32_t)g;
@@ -888,26 +937,40 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
     }
     ec[e] = it->second;
   }
-  // element order: by combination, then by the earliest schedule position of
-  // the element's entities (spatial order when the schedule is)
+  // element order: by segment (multi-mesh operator), by combination, then by
+  // the earliest schedule position of the element's entities (spatial order
+  // when the schedule is)
   std::vector<int64_t> key(E);
+  std::vector<int32_t> es(E, 0);
   for (int64_t e = 0; e < E; ++e) {
-    int32_t m = pos[eg[e * nle]];
-    for (int la = 1; la < nle; ++la) m = std::min(m, pos[eg[e * nle + la]]);
+    int32_t m = pos[eg[e * nle]], mx = m;
+    for (int la = 1; la < nle; ++la) {
+      m = std::min(m, pos[eg[e * nle + la]]);
+      mx = std::max(mx, pos[eg[e * nle + la]]);
+    }
     key[e] = ((int64_t)ec[e] << 32) | (uint32_t)m;
+    if (seg) {
+      const auto& sp = seg->pos;
+      const int sm = (int)(std::upper_bound(sp.begin(), sp.end(), (int64_t)m) - sp.begin()) - 1;
+      if (sm < 0 || sm >= seg->nseg || mx >= sp[sm + 1])
+        PG_FAIL(PG_ERR_ARG, "element %lld has entities of two meshes", (long long)e);
+      es[e] = sm;
+    }
   }
   std::vector<int64_t> sorted(E);
   std::iota(sorted.begin(), sorted.end(), 0);
-  std::stable_sort(sorted.begin(), sorted.end(),
-                   [&](int64_t a, int64_t b) { return key[a] < key[b]; });
-  // waves of 64 elements of one combination
-  std::vector<int32_t> wcombo, perm;
+  std::stable_sort(sorted.begin(), sorted.end(), [&](int64_t a, int64_t b) {
+    return es[a] != es[b] ? es[a] < es[b] : key[a] < key[b];
+  });
+  // waves of 64 elements of one combination (and one segment)
+  std::vector<int32_t> wcombo, perm, wsg;
   for (int64_t k = 0; k < E;) {
-    const int c = ec[sorted[k]];
+    const int c = ec[sorted[k]], sg = es[sorted[k]];
     int64_t k1 = k;
-    while (k1 < E && ec[sorted[k1]] == c) ++k1;
+    while (k1 < E && ec[sorted[k1]] == c && es[sorted[k1]] == sg) ++k1;
     for (int64_t a = k; a < k1; a += kWave) {
       wcombo.push_back(c);
+      wsg.push_back(sg);
       for (int l = 0; l < kWave; ++l) perm.push_back(a + l < k1 ? (int32_t)sorted[a + l] : -1);
     }
     k = k1;
@@ -1070,6 +1133,18 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
     PG_A(op->kcol, kcol.size());
     op->nks = nks;
   }
+  std::vector<double> womu;
+  if (seg) {
+    op->nseg = seg->nseg;
+    op->seg_pos = seg->pos;
+    op->seg_wave.assign(seg->nseg + 1, nw);
+    for (int64_t w = nw - 1; w >= 0; --w) op->seg_wave[wsg[w]] = w;
+    for (int sg = seg->nseg - 1; sg >= 0; --sg)  // a segment without waves starts where the next does
+      op->seg_wave[sg] = std::min(op->seg_wave[sg], op->seg_wave[sg + 1]);
+    for (double om : seg->omega) womu.push_back(om * eop::kMu0);
+    PG_A(op->wseg, nw);
+    PG_A(op->womu, womu.size());
+  }
 #undef PG_A
   const struct {
     void* dst;
@@ -1083,7 +1158,9 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
             {op->tab64, tab.data(), tab.size() * sizeof(double)},
             {op->tab32, tab32.data(), tab32.size() * sizeof(float)},
             {op->afrag, afrag.data(), afrag.size() * sizeof(double)},
-            {op->kcol, kcol.data(), kcol.size() * sizeof(int32_t)}};
+            {op->kcol, kcol.data(), kcol.size() * sizeof(int32_t)},
+            {op->wseg, wsg.data(), op->wseg ? wsg.size() * sizeof(int32_t) : 0},
+            {op->womu, womu.data(), womu.size() * sizeof(double)}};
   for (const auto& u : up)
     if (u.bytes && hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
       delete op;
@@ -1101,8 +1178,126 @@ int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t
   return PG_OK;
 }
 
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+
+int pg_eop_create(int p, int64_t E, const void* ws, const int32_t* dofs, int64_t ndof,
+                  const int32_t* block_ptr, int64_t nblocks, const int32_t* order,
+                  const uint8_t* ent_bc, double omega, void** out, void* stream) {
+  if (!out || !ws || !dofs || !block_ptr || E <= 0 || ndof <= 0 || nblocks <= 0)
+    PG_FAIL(PG_ERR_ARG, "bad arguments");
+  if (p < 1 || p > 3)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator: order p=%d not supported (1..3)", p);
+  const int N = ndof_element(p);
+  hipStream_t st = as_stream(stream);
+  int rc0 = check_block_ptr(p, block_ptr, nblocks, ndof);
+  if (rc0 != PG_OK) return rc0;
+  // host copies of the element DOFs, orientations and PEC entities
+  std::vector<int32_t> hd((size_t)E * N), hor(E);
+  std::vector<uint8_t> hbc(nblocks, 0);
+  const int32_t* dor = reinterpret_cast<const int32_t*>(reinterpret_cast<const double*>(ws) + 12 * E);
+  PG_HIP(hipMemcpyAsync(hd.data(), dofs, hd.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipMemcpyAsync(hor.data(), dor, E * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  if (ent_bc)
+    PG_HIP(hipMemcpyAsync(hbc.data(), ent_bc, nblocks, hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  ElemOp* op = nullptr;
+  const int rc = eop_build(p, E, ws, ndof, block_ptr, nblocks, order, hd, hor, hbc, omega,
+                           nullptr, &op, st);
+  if (rc != PG_OK) return rc;
+  *out = op;
+  return PG_OK;
+}
+
+int pg_eop_create_multi(int p, int nseg, const int64_t* n_elems, const void* const* ws,
+                        const int32_t* const* dofs, const int64_t* ndof,
+                        const int32_t* const* block_ptr, const int64_t* nblocks,
+                        const int32_t* const* order, const uint8_t* const* ent_bc,
+                        const double* omega, void** out, void* stream) {
+  if (!out || nseg <= 0 || !n_elems || !ws || !dofs || !ndof || !block_ptr || !nblocks || !omega)
+    PG_FAIL(PG_ERR_ARG, "bad arguments");
+  if (p < 1 || p > 3)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "element-by-element operator: order p=%d not supported (1..3)", p);
+  const int N = ndof_element(p);
+  hipStream_t st = as_stream(stream);
+  // the union: mesh s's elements, DOFs and entities follow those of meshes < s
+  int64_t E = 0, nd = 0, nb = 0;
+  SegInfo seg;
+  seg.nseg = nseg;
+  seg.pos.push_back(0);
+  for (int s = 0; s < nseg; ++s) {
+    if (n_elems[s] <= 0 || ndof[s] <= 0 || nblocks[s] <= 0 || !ws[s] || !dofs[s] ||
+        !block_ptr[s])
+      PG_FAIL(PG_ERR_ARG, "mesh %d: bad arguments", s);
+    int rc0 = check_block_ptr(p, block_ptr[s], nblocks[s], ndof[s]);
+    if (rc0 != PG_OK) return rc0;
+    E += n_elems[s];
+    nd += ndof[s];
+    nb += nblocks[s];
+    seg.pos.push_back(nb);
+    seg.omega.push_back(omega[s]);
+  }
+  if (nd >= (1LL << 31) || nb >= (1LL << 31) || E * N >= (1LL << 31))
+    PG_FAIL(PG_ERR_UNSUPPORTED, "the meshes together are too large for 32-bit indices");
+  std::vector<int32_t> hd((size_t)E * N), hor(E), bp(nb + 1), ord(nb);
+  std::vector<uint8_t> hbc(nb, 0);
+  double* wsu = nullptr;
+  if (hipMalloc(&wsu, (size_t)E * 12 * sizeof(double)) != hipSuccess)
+    PG_FAIL(PG_ERR_HIP, "hipMalloc(%lld) failed", (long long)(E * 12 * sizeof(double)));
+  int64_t e0 = 0, d0 = 0, b0 = 0;
+  int rc = PG_OK;
+  for (int s = 0; s < nseg && rc == PG_OK; ++s) {
+    const int64_t Es = n_elems[s];
+    const double* wss = reinterpret_cast<const double*>(ws[s]);
+    if (hipMemcpyAsync(wsu + 12 * e0, wss, Es * 12 * sizeof(double), hipMemcpyDeviceToDevice,
+                       st) != hipSuccess ||
+        hipMemcpyAsync(hd.data() + e0 * N, dofs[s], Es * N * sizeof(int32_t),
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(hor.data() + e0, reinterpret_cast<const int32_t*>(wss + 12 * Es),
+                       Es * sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (ent_bc && ent_bc[s] &&
+         hipMemcpyAsync(hbc.data() + b0, ent_bc[s], nblocks[s], hipMemcpyDeviceToHost, st) !=
+             hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = PG_ERR_HIP;
+      break;
+    }
+    for (int64_t k = e0 * N; k < (e0 + Es) * N; ++k) hd[k] += (int32_t)d0;
+    for (int64_t g = 0; g < nblocks[s]; ++g) {
+      bp[b0 + g] = (int32_t)(block_ptr[s][g] + d0);
+      const int32_t o = order && order[s] ? order[s][g] : (int32_t)g;
+      if (o < 0 || o >= nblocks[s]) {
+        rc = PG_ERR_ARG;
+        break;
+      }
+      ord[b0 + g] = (int32_t)(o + b0);
+    }
+    e0 += Es;
+    d0 += ndof[s];
+    b0 += nblocks[s];
+  }
+  bp[nb] = (int32_t)nd;
+  ElemOp* op = nullptr;
+  if (rc == PG_OK)
+    rc = eop_build(p, E, wsu, nd, bp.data(), nb, ord.data(), hd, hor, hbc, 0.0, &seg, &op, st);
+  else
+    set_error("multi-mesh element operator: copies failed or an entity order out of range");
+  if (rc != PG_OK) {
+    (void)hipFree(wsu);
+    return rc;
+  }
+  op->owned.push_back(wsu);  // the union's coefficients, read by no refresh
+  *out = op;
+  return PG_OK;
+}
+
 int pg_eop_refresh(void* op, const void* ws, double omega, void* stream) {
   if (!op || !ws) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  if (static_cast<ElemOp*>(op)->nseg > 0)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "a multi-mesh element operator keeps its creation coefficients");
   return eop_refresh(static_cast<ElemOp*>(op), ws, omega, as_stream(stream));
 }
 

@@ -43,6 +43,7 @@
 #include <cmath>
 #include <vector>
 
+#include "pg_blockinv.h"
 #include "pg_common.h"
 #include "pg_eop.h"
 
@@ -776,7 +777,7 @@ __global__ void k_demote(int64_t n, const double* __restrict__ a, float* __restr
 }
 
 // entity-block Jacobi set-up: one thread per block gathers A_BB from the CSR and
-// inverts it (Gauss-Jordan with partial pivoting); row-major at boff[k]
+// inverts it; row-major at boff[k]
 __global__ void k_bjacobi_setup(int64_t nb, const int32_t* __restrict__ bptr,
                                 const int32_t* __restrict__ boff,
                                 const int32_t* __restrict__ rowptr,
@@ -786,72 +787,7 @@ __global__ void k_bjacobi_setup(int64_t nb, const int32_t* __restrict__ bptr,
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nb) return;
   const int i0 = bptr[k], d = bptr[k + 1] - i0;
-  if (d <= 0 || d > kMaxBlock) {
-    bad[0] = 1;
-    return;
-  }
-  double2 a[kMaxBlock][kMaxBlock], inv[kMaxBlock][kMaxBlock];
-  for (int t = 0; t < d; ++t) {
-    const int row = i0 + t;
-    int lo = rowptr[row], hi = rowptr[row + 1] - 1, pos = -1;
-    while (lo <= hi) {
-      const int mid = (lo + hi) >> 1;
-      const int c = cols[mid];
-      if (c == i0) {
-        pos = mid;
-        break;
-      }
-      if (c < i0) lo = mid + 1;
-      else hi = mid - 1;
-    }
-    if (pos < 0 || pos + d > rowptr[row + 1] || cols[pos + d - 1] != i0 + d - 1) {
-      bad[0] = 1;
-      return;
-    }
-    for (int u = 0; u < d; ++u) {
-      a[t][u] = vals[pos + u];
-      inv[t][u] = make_double2(t == u ? 1.0 : 0.0, 0.0);
-    }
-  }
-  for (int c = 0; c < d; ++c) {
-    int piv = c;
-    double best = hypot(a[c][c].x, a[c][c].y);
-    for (int t = c + 1; t < d; ++t) {
-      const double v = hypot(a[t][c].x, a[t][c].y);
-      if (v > best) {
-        best = v;
-        piv = t;
-      }
-    }
-    if (!(best > 0.0)) {
-      bad[0] = 1;
-      return;
-    }
-    if (piv != c)
-      for (int u = 0; u < d; ++u) {
-        const double2 t1 = a[c][u], t2 = inv[c][u];
-        a[c][u] = a[piv][u];
-        inv[c][u] = inv[piv][u];
-        a[piv][u] = t1;
-        inv[piv][u] = t2;
-      }
-    const double2 ip = zdiv(make_double2(1.0, 0.0), a[c][c]);
-    for (int u = 0; u < d; ++u) {
-      a[c][u] = zmul(a[c][u], ip);
-      inv[c][u] = zmul(inv[c][u], ip);
-    }
-    for (int t = 0; t < d; ++t) {
-      if (t == c) continue;
-      const double2 f = a[t][c];
-      for (int u = 0; u < d; ++u) {
-        a[t][u] = zsub(a[t][u], zmul(f, a[c][u]));
-        inv[t][u] = zsub(inv[t][u], zmul(f, inv[c][u]));
-      }
-    }
-  }
-  double2* out = binv + boff[k];
-  for (int t = 0; t < d; ++t)
-    for (int u = 0; u < d; ++u) out[t * d + u] = inv[t][u];
+  if (d > kMaxBlock || !invert_block(rowptr, cols, vals, i0, d, binv + boff[k])) bad[0] = 1;
 }
 
 // z = P v (block), one thread per block

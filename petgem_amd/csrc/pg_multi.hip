@@ -20,9 +20,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <vector>
 
+#include "pg_blockinv.h"
 #include "pg_common.h"
+#include "pg_eop.h"
 
 namespace pg {
 namespace multi {
@@ -381,6 +384,246 @@ __global__ void k_mjacobi(int64_t n, const int32_t* __restrict__ rowptr,
   dinv[r] = zdiv(make_double2(1.0, 0.0), d);
 }
 
+// ---------------------------------------------------------------------------
+// Element products and entity-block Jacobi (pg_multi_set_element_operator).
+// The meshes' entities are numbered mesh after mesh, so segment s owns the
+// entities [epos0[s], epos0[s+1]) both in index order (the DOF ranges bptr,
+// the block inverses at boff) and in the operator's schedule (gm, gslot).
+// Workgroups of kBS entities are dealt to the segments, ewseg / ewg0 as
+// wseg / wg0 for the rows.
+// ---------------------------------------------------------------------------
+__device__ inline Span espan_of(const int32_t* __restrict__ ewseg,
+                                const int32_t* __restrict__ ewg0,
+                                const int64_t* __restrict__ epos0) {
+  const int w = blockIdx.x;
+  const int s = ewseg[w];
+  const int64_t lo = epos0[s] + (int64_t)(w - ewg0[s]) * kBS;
+  const int64_t hi = lo + kBS < epos0[s + 1] ? lo + kBS : epos0[s + 1];
+  return {s, lo, hi};
+}
+
+// q = A p from the local results: per scheduled entity the sum of its
+// incidences in ascending slot order (PEC entity: identity row); partials of p^T q
+template <int D>
+__global__ void __launch_bounds__(kBS) k_esum(const Ctl* ctl, const Seg* seg,
+                                              const int32_t* __restrict__ ewseg,
+                                              const int32_t* __restrict__ ewg0,
+                                              const int64_t* __restrict__ epos0,
+                                              const int4* __restrict__ gm,
+                                              const int32_t* __restrict__ gslot,
+                                              const double2* __restrict__ ye,
+                                              const double2* __restrict__ p,
+                                              double2* __restrict__ q, double* part) {
+  if (ctl->all_done) return;
+  const Span sp = espan_of(ewseg, ewg0, epos0);
+  if (seg[sp.s].done) return;
+  double red[2] = {0.0, 0.0};
+  const int64_t i = sp.lo + threadIdx.x;
+  if (i < sp.hi) {
+    const int4 m = gm[i];
+    const int r0 = m.x, d = m.y, s0 = m.z, s1 = m.w;
+    double2 pr[D], acc[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) {
+      pr[t] = t < d ? p[r0 + t] : make_double2(0.0, 0.0);
+      acc[t] = s0 == s1 ? pr[t] : make_double2(0.0, 0.0);
+    }
+    int k = s0;
+    for (; k + 2 <= s1; k += 2) {  // two incidences' loads in flight together
+      const int b0 = gslot[k], b1 = gslot[k + 1];
+      double2 v0[D], v1[D];
+#pragma unroll
+      for (int t = 0; t < D; ++t)
+        if (t < d) {
+          v0[t] = ye[b0 + t * kWave];
+          v1[t] = ye[b1 + t * kWave];
+        }
+#pragma unroll
+      for (int t = 0; t < D; ++t)
+        if (t < d) {
+          acc[t].x += v0[t].x;
+          acc[t].y += v0[t].y;
+          acc[t].x += v1[t].x;
+          acc[t].y += v1[t].y;
+        }
+    }
+    if (k < s1) {
+      const int b0 = gslot[k];
+#pragma unroll
+      for (int t = 0; t < D; ++t)
+        if (t < d) {
+          const double2 v = ye[b0 + t * kWave];
+          acc[t].x += v.x;
+          acc[t].y += v.y;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      if (t < d) {
+        q[r0 + t] = acc[t];
+        red[0] += pr[t].x * acc[t].x - pr[t].y * acc[t].y;
+        red[1] += pr[t].x * acc[t].y + pr[t].y * acc[t].x;
+      }
+  }
+  write_partials<2>(red, part);
+}
+
+// x += alpha p; r -= alpha q; z = B r (entity blocks); partials of r^T z, |r|^2
+template <int D>
+__global__ void __launch_bounds__(kBS) k_eupdate(const Ctl* ctl, const Seg* seg,
+                                                 const int32_t* __restrict__ ewseg,
+                                                 const int32_t* __restrict__ ewg0,
+                                                 const int64_t* __restrict__ epos0,
+                                                 const int32_t* __restrict__ bptr,
+                                                 const int32_t* __restrict__ boff,
+                                                 const double2* __restrict__ binv,
+                                                 double2* __restrict__ x, double2* __restrict__ r,
+                                                 const double2* __restrict__ p,
+                                                 const double2* __restrict__ q,
+                                                 double2* __restrict__ z, double* part) {
+  if (ctl->all_done) return;
+  const Span sp = espan_of(ewseg, ewg0, epos0);
+  if (seg[sp.s].done) return;
+  const double2 a = seg[sp.s].alpha;
+  const double2 na = make_double2(-a.x, -a.y);
+  double red[3] = {0.0, 0.0, 0.0};
+  const int64_t k = sp.lo + threadIdx.x;
+  if (k < sp.hi) {
+    const int i0 = bptr[k], d = bptr[k + 1] - i0;
+    const double2* B = binv + boff[k];
+    double2 rl[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      if (t < d) {
+        x[i0 + t] = cfma(a, p[i0 + t], x[i0 + t]);
+        rl[t] = cfma(na, q[i0 + t], r[i0 + t]);
+        r[i0 + t] = rl[t];
+      }
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      if (t < d) {
+        double2 zt = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+          if (u < d) zt = cfma(B[t * d + u], rl[u], zt);
+        z[i0 + t] = zt;
+        red[0] += rl[t].x * zt.x - rl[t].y * zt.y;
+        red[1] += rl[t].x * zt.y + rl[t].y * zt.x;
+        red[2] += rl[t].x * rl[t].x + rl[t].y * rl[t].y;
+      }
+  }
+  write_partials<3>(red, part);
+}
+
+// the same when every entity has two DOFs 2k, 2k + 1 (p = 2; inverse at 4k):
+// one 32-byte access per vector and entity, no index reads
+__global__ void __launch_bounds__(kBS) k_eupdate2(const Ctl* ctl, const Seg* seg,
+                                                  const int32_t* __restrict__ ewseg,
+                                                  const int32_t* __restrict__ ewg0,
+                                                  const int64_t* __restrict__ epos0,
+                                                  const double2* __restrict__ binv,
+                                                  double2* __restrict__ x, double2* __restrict__ r,
+                                                  const double2* __restrict__ p,
+                                                  const double2* __restrict__ q,
+                                                  double2* __restrict__ z, double* part) {
+  if (ctl->all_done) return;
+  const Span sp = espan_of(ewseg, ewg0, epos0);
+  if (seg[sp.s].done) return;
+  const double2 a = seg[sp.s].alpha;
+  const double2 na = make_double2(-a.x, -a.y);
+  double red[3] = {0.0, 0.0, 0.0};
+  const int64_t k = sp.lo + threadIdx.x;
+  if (k < sp.hi) {
+    double4* x2 = reinterpret_cast<double4*>(x);
+    double4* r2 = reinterpret_cast<double4*>(r);
+    double4* z2 = reinterpret_cast<double4*>(z);
+    const double4 xv = x2[k], rv = r2[k];
+    const double4 pv = reinterpret_cast<const double4*>(p)[k];
+    const double4 qv = reinterpret_cast<const double4*>(q)[k];
+    const double4 B0 = reinterpret_cast<const double4*>(binv)[2 * k];
+    const double4 B1 = reinterpret_cast<const double4*>(binv)[2 * k + 1];
+    const double2 xa = cfma(a, make_double2(pv.x, pv.y), make_double2(xv.x, xv.y));
+    const double2 xb = cfma(a, make_double2(pv.z, pv.w), make_double2(xv.z, xv.w));
+    const double2 ra = cfma(na, make_double2(qv.x, qv.y), make_double2(rv.x, rv.y));
+    const double2 rb = cfma(na, make_double2(qv.z, qv.w), make_double2(rv.z, rv.w));
+    x2[k] = make_double4(xa.x, xa.y, xb.x, xb.y);
+    r2[k] = make_double4(ra.x, ra.y, rb.x, rb.y);
+    // z = B r, B row-major 2 x 2: B0 = (B00, B01), B1 = (B10, B11)
+    const double2 za = cfma(make_double2(B0.z, B0.w), rb,
+                            cfma(make_double2(B0.x, B0.y), ra, make_double2(0.0, 0.0)));
+    const double2 zb = cfma(make_double2(B1.z, B1.w), rb,
+                            cfma(make_double2(B1.x, B1.y), ra, make_double2(0.0, 0.0)));
+    z2[k] = make_double4(za.x, za.y, zb.x, zb.y);
+    red[0] += ra.x * za.x - ra.y * za.y;
+    red[1] += ra.x * za.y + ra.y * za.x;
+    red[2] += ra.x * ra.x + ra.y * ra.y;
+    red[0] += rb.x * zb.x - rb.y * zb.y;
+    red[1] += rb.x * zb.y + rb.y * zb.x;
+    red[2] += rb.x * rb.x + rb.y * rb.y;
+  }
+  write_partials<3>(red, part);
+}
+
+// (re)start after r = b - A x: z = B r, p = 0; partials of r^T z, |r|^2
+template <int D>
+__global__ void __launch_bounds__(kBS) k_erestart(const int32_t* __restrict__ ewseg,
+                                                  const int32_t* __restrict__ ewg0,
+                                                  const int64_t* __restrict__ epos0,
+                                                  const int32_t* __restrict__ bptr,
+                                                  const int32_t* __restrict__ boff,
+                                                  const double2* __restrict__ binv,
+                                                  const double2* __restrict__ r,
+                                                  double2* __restrict__ z, double2* __restrict__ p,
+                                                  double* part) {
+  const Span sp = espan_of(ewseg, ewg0, epos0);
+  double red[3] = {0.0, 0.0, 0.0};
+  const int64_t k = sp.lo + threadIdx.x;
+  if (k < sp.hi) {
+    const int i0 = bptr[k], d = bptr[k + 1] - i0;
+    const double2* B = binv + boff[k];
+    double2 rl[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) rl[t] = t < d ? r[i0 + t] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      if (t < d) {
+        double2 zt = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+          if (u < d) zt = cfma(B[t * d + u], rl[u], zt);
+        z[i0 + t] = zt;
+        p[i0 + t] = make_double2(0.0, 0.0);
+        red[0] += rl[t].x * zt.x - rl[t].y * zt.y;
+        red[1] += rl[t].x * zt.y + rl[t].y * zt.x;
+        red[2] += rl[t].x * rl[t].x + rl[t].y * rl[t].y;
+      }
+  }
+  write_partials<3>(red, part);
+}
+
+// block inverses of the concatenated CSR's entity blocks (index order)
+__global__ void k_ebinv(int64_t nent, const int32_t* __restrict__ bptr,
+                        const int32_t* __restrict__ boff, const int32_t* __restrict__ rowptr,
+                        const int32_t* __restrict__ cols, const double2* __restrict__ vals,
+                        double2* __restrict__ binv, int32_t* __restrict__ bad) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nent) return;
+  const int i0 = bptr[k];
+  if (!invert_block(rowptr, cols, vals, i0, bptr[k + 1] - i0, binv + boff[k])) bad[0] = 1;
+}
+
+// Work tables of one set of launches (element mode): the workgroups of the row
+// kernels (wseg, wg0, W) and of the entity kernels (ewseg, ewg0, We) dealt to
+// a set of meshes (the others get none, an empty range in wg0 / ewg0), and the
+// wave ranges [w0, w1) of their element products.  The full set covers every
+// mesh; as meshes converge the solve launches a compacted set.
+struct Tabs {
+  int W = 0, We = 0;
+  int32_t *wseg = nullptr, *wg0 = nullptr, *ewseg = nullptr, *ewg0 = nullptr;
+  std::vector<std::pair<int64_t, int64_t>> runs;
+  int64_t work = 0;  // entities covered
+};
+
 struct Multi {
   int nseg = 0, W = 0, check = 25, maxit = 100000;
   int64_t n = 0, nnz = 0;
@@ -398,8 +641,24 @@ struct Multi {
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool use_graph = false;
+  // element products + entity-block Jacobi (pg_multi_set_element_operator)
+  const ElemOp* eop = nullptr;
+  EopSchedule es{};
+  int D = 1;
+  bool pairs = false;  // every entity two DOFs 2k, 2k + 1 (p = 2)
+  int32_t *ebptr = nullptr, *eboff = nullptr;
+  int64_t* epos0 = nullptr;
+  double2* binv = nullptr;
+  std::vector<int64_t> spos, swave;  // per mesh: first entity, first wave (nseg + 1)
+  Tabs full, cur;
+  const Tabs* act = nullptr;  // the set the iterations launch
+  hipGraph_t cgraph = nullptr;  // iterations on the compacted set
+  hipGraphExec_t cexec = nullptr;
   std::vector<void*> owned;
   ~Multi() {  // release errors cannot be reported from here
+    if (cexec) (void)hipGraphExecDestroy(cexec);
+    if (cgraph) (void)hipGraphDestroy(cgraph);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (st) (void)hipStreamDestroy(st);
@@ -419,8 +678,55 @@ struct Multi {
   }
 };
 
-// one COCG iteration of every mesh (six launches on the batch's stream)
+static const int32_t* seg_done(const Multi* B) {
+  return reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(B->seg) +
+                                          offsetof(Seg, done));
+}
+static_assert(sizeof(Seg) % sizeof(int32_t) == 0, "Seg stride in int32 units");
+constexpr int kSegStride = (int)(sizeof(Seg) / sizeof(int32_t));
+
+template <int D>
+static void enqueue_iteration_e(Multi* B, const Tabs& T) {
+  const dim3 g(T.W), ge(T.We), b(kBS), gs(B->nseg);
+  hipLaunchKernelGGL(k_mpupdate, g, b, 0, B->st, B->ctl, B->seg, T.wseg, T.wg0, B->row0, B->z,
+                     B->p);
+  const double2* ye = nullptr;
+  for (const auto& run : T.runs)
+    ye = eop_local_segments(B->eop, seg_done(B), kSegStride, B->p, B->st, run.first, run.second);
+  hipLaunchKernelGGL(k_esum<D>, ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0, B->epos0,
+                     B->es.gm, B->es.gslot, ye, B->p, B->q, B->part);
+  hipLaunchKernelGGL(k_mfin_mu, gs, b, 0, B->st, B->ctl, B->seg, T.ewg0, B->part, T.We);
+  if (D == 2 && B->pairs)
+    hipLaunchKernelGGL(k_eupdate2, ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0, B->epos0,
+                       B->binv, B->x, B->r, B->p, B->q, B->z, B->part);
+  else
+    hipLaunchKernelGGL(k_eupdate<D>, ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0, B->epos0,
+                       B->ebptr, B->eboff, B->binv, B->x, B->r, B->p, B->q, B->z, B->part);
+  hipLaunchKernelGGL(k_mfin_update, gs, b, 0, B->st, B->ctl, B->seg, T.ewg0, B->part, T.We);
+  hipLaunchKernelGGL(k_mfin_all, dim3(1), b, 0, B->st, B->ctl, B->seg, B->nseg);
+}
+
+template <int D>
+static void enqueue_restart_e(Multi* B, int first) {
+  const Tabs& T = B->full;
+  hipLaunchKernelGGL(k_mtrue, dim3(T.W), dim3(kBS), 0, B->st, T.wseg, T.wg0, B->row0, B->rowptr,
+                     B->cols, B->vals, B->x, B->b, B->r, B->part);
+  hipLaunchKernelGGL(k_erestart<D>, dim3(T.We), dim3(kBS), 0, B->st, T.ewseg, T.ewg0, B->epos0,
+                     B->ebptr, B->eboff, B->binv, B->r, B->z, B->p, B->part);
+  hipLaunchKernelGGL(k_mfin_restart, dim3(B->nseg), dim3(kBS), 0, B->st, B->ctl, B->seg, T.ewg0,
+                     B->part, T.We, first);
+  hipLaunchKernelGGL(k_mfin_all, dim3(1), dim3(kBS), 0, B->st, B->ctl, B->seg, B->nseg);
+}
+
+// one COCG iteration of every mesh (six launches on the batch's stream; with
+// the element operator seven, plus one per further run of waves)
 static void enqueue_iteration(Multi* B) {
+  if (B->eop) {
+    if (B->D == 1) enqueue_iteration_e<1>(B, *B->act);
+    else if (B->D == 2) enqueue_iteration_e<2>(B, *B->act);
+    else enqueue_iteration_e<6>(B, *B->act);
+    return;
+  }
   const dim3 g(B->W), b(kBS), gs(B->nseg);
   hipLaunchKernelGGL(k_mpupdate, g, b, 0, B->st, B->ctl, B->seg, B->wseg, B->wg0, B->row0, B->z,
                      B->p);
@@ -434,11 +740,87 @@ static void enqueue_iteration(Multi* B) {
 }
 
 static void enqueue_restart(Multi* B, int first) {
+  if (B->eop) {
+    if (B->D == 1) enqueue_restart_e<1>(B, first);
+    else if (B->D == 2) enqueue_restart_e<2>(B, first);
+    else enqueue_restart_e<6>(B, first);
+    return;
+  }
   hipLaunchKernelGGL(k_mrestart, dim3(B->W), dim3(kBS), 0, B->st, B->wseg, B->wg0, B->row0,
                      B->rowptr, B->cols, B->vals, B->x, B->b, B->r, B->dinv, B->z, B->p, B->part);
   hipLaunchKernelGGL(k_mfin_restart, dim3(B->nseg), dim3(kBS), 0, B->st, B->ctl, B->seg, B->wg0,
                      B->part, B->W, first);
   hipLaunchKernelGGL(k_mfin_all, dim3(1), dim3(kBS), 0, B->st, B->ctl, B->seg, B->nseg);
+}
+
+// capture `check` iterations of the current work set (when the batch uses a
+// graph) into *g / *ge, replacing what they held
+static int capture(Multi* B, hipGraph_t* g, hipGraphExec_t* ge) {
+  if (!B->use_graph) return PG_OK;
+  if (*ge) (void)hipGraphExecDestroy(*ge);
+  if (*g) (void)hipGraphDestroy(*g);
+  *ge = nullptr;
+  *g = nullptr;
+  if (hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    PG_FAIL(PG_ERR_HIP, "graph capture failed to start");
+  for (int k = 0; k < B->check; ++k) enqueue_iteration(B);
+  if (hipStreamEndCapture(B->st, g) != hipSuccess ||
+      hipGraphInstantiate(ge, *g, nullptr, nullptr, 0) != hipSuccess)
+    PG_FAIL(PG_ERR_HIP, "graph capture failed");
+  return PG_OK;
+}
+
+// host tables of the meshes with active[s] set (element mode)
+static void build_tabs(const Multi* B, const std::vector<char>& active, std::vector<int32_t>& wseg,
+                       std::vector<int32_t>& wg0, std::vector<int32_t>& ewseg,
+                       std::vector<int32_t>& ewg0, Tabs* T) {
+  const int ns = B->nseg;
+  wseg.clear();
+  ewseg.clear();
+  wg0.assign(ns + 1, 0);
+  ewg0.assign(ns + 1, 0);
+  T->runs.clear();
+  T->work = 0;
+  for (int s = 0; s < ns; ++s) {
+    wg0[s] = (int32_t)wseg.size();
+    ewg0[s] = (int32_t)ewseg.size();
+    if (!active[s]) continue;
+    const int64_t nr = B->hrow0[s + 1] - B->hrow0[s], ne = B->spos[s + 1] - B->spos[s];
+    for (int64_t w = 0; w < std::max<int64_t>(1, (nr + kRW - 1) / kRW); ++w) wseg.push_back(s);
+    for (int64_t w = 0; w < std::max<int64_t>(1, (ne + kBS - 1) / kBS); ++w) ewseg.push_back(s);
+    T->work += ne;
+    const int64_t w0 = B->swave[s], w1 = B->swave[s + 1];
+    if (w1 > w0) {
+      if (!T->runs.empty() && T->runs.back().second == w0) T->runs.back().second = w1;
+      else T->runs.emplace_back(w0, w1);
+    }
+  }
+  wg0[ns] = (int32_t)wseg.size();
+  ewg0[ns] = (int32_t)ewseg.size();
+  T->W = (int)wseg.size();
+  T->We = (int)ewseg.size();
+}
+
+// the compacted set of the meshes not yet done (hseg read back), uploaded to
+// the `cur` tables (no launch in flight) and captured
+static int compact(Multi* B) {
+  std::vector<char> active(B->nseg);
+  for (int s = 0; s < B->nseg; ++s) active[s] = B->hseg[s].done ? 0 : 1;
+  std::vector<int32_t> wseg, wg0, ewseg, ewg0;
+  build_tabs(B, active, wseg, wg0, ewseg, ewg0, &B->cur);
+  Tabs& T = B->cur;
+  PG_HIP(hipMemcpyAsync(T.wseg, wseg.data(), std::max<size_t>(1, wseg.size()) * sizeof(int32_t),
+                        hipMemcpyHostToDevice, B->st));
+  PG_HIP(hipMemcpyAsync(T.wg0, wg0.data(), wg0.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                        B->st));
+  PG_HIP(hipMemcpyAsync(T.ewseg, ewseg.data(),
+                        std::max<size_t>(1, ewseg.size()) * sizeof(int32_t),
+                        hipMemcpyHostToDevice, B->st));
+  PG_HIP(hipMemcpyAsync(T.ewg0, ewg0.data(), ewg0.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice, B->st));
+  PG_HIP(hipStreamSynchronize(B->st));
+  B->act = &B->cur;
+  return capture(B, &B->cgraph, &B->cexec);
 }
 
 }  // namespace multi
@@ -561,21 +943,102 @@ int pg_multi_create(int nseg, const int64_t* n, const int64_t* nnz, const int32_
     delete B;
     PG_FAIL(PG_ERR_ARG, "zero diagonal entry: Jacobi preconditioner undefined");
   }
-  if (use_graph) {
-    // capture `check` iterations once; replayed by every solve
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      delete B;
-      PG_FAIL(PG_ERR_HIP, "graph capture failed to start");
-    }
-    for (int k = 0; k < B->check; ++k) enqueue_iteration(B);
-    if (hipStreamEndCapture(st, &B->graph) != hipSuccess ||
-        hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) != hipSuccess) {
-      delete B;
-      PG_FAIL(PG_ERR_HIP, "graph capture failed");
-    }
+  B->use_graph = use_graph != 0;
+  if ((rc = capture(B, &B->graph, &B->exec)) != PG_OK) {
+    delete B;
+    return rc;
   }
   *out = B;
   return PG_OK;
+}
+
+int pg_multi_set_element_operator(void* batch, void* op, void* stream) {
+  if (!batch || !op) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  Multi* B = static_cast<Multi*>(batch);
+  const ElemOp* E = static_cast<const ElemOp*>(op);
+  if (B->eop) PG_FAIL(PG_ERR_ARG, "the batch already has an element operator");
+  const int64_t* spos = nullptr;
+  const int nseg = eop_segments(E, &spos, nullptr);
+  if (nseg != B->nseg)
+    PG_FAIL(PG_ERR_ARG, "element operator of %d meshes for a batch of %d", nseg, B->nseg);
+  if (eop_ndof(E) != B->n) PG_FAIL(PG_ERR_ARG, "element operator and batch differ in size");
+  if (!eop_supports(E, true))
+    PG_FAIL(PG_ERR_UNSUPPORTED, "element operator without a complex128 product");
+  const EopSchedule es = eop_schedule(E);
+  const int64_t nent = es.nent;
+  hipStream_t cst = as_stream(stream);
+  PG_HIP(hipStreamSynchronize(cst));
+  // entity DOF ranges in index order (the schedule's ranges sorted by first
+  // DOF; they must tile 0 .. n) and each mesh's entities within its rows
+  std::vector<int4> gm(nent);
+  PG_HIP(hipMemcpy(gm.data(), es.gm, nent * sizeof(int4), hipMemcpyDeviceToHost));
+  for (int s = 0; s < nseg; ++s)
+    for (int64_t i = spos[s]; i < spos[s + 1]; ++i)
+      if (gm[i].x < B->hrow0[s] || gm[i].x + gm[i].y > B->hrow0[s + 1])
+        PG_FAIL(PG_ERR_ARG, "mesh %d: scheduled entity outside its rows", s);
+  std::vector<int2> rg(nent);
+  for (int64_t i = 0; i < nent; ++i) rg[i] = make_int2(gm[i].x, gm[i].y);
+  std::sort(rg.begin(), rg.end(), [](int2 a, int2 b) { return a.x < b.x; });
+  std::vector<int32_t> bp(nent + 1), boff(nent + 1);
+  boff[0] = 0;
+  for (int64_t k = 0; k < nent; ++k) {
+    if (rg[k].x != (k ? bp[k - 1] + rg[k - 1].y : 0) || rg[k].y < 1 || rg[k].y > kMaxInvBlock)
+      PG_FAIL(PG_ERR_ARG, "element operator entities do not tile the rows");
+    bp[k] = rg[k].x;
+    boff[k + 1] = boff[k] + rg[k].y * rg[k].y;
+  }
+  bp[nent] = (int32_t)B->n;
+  if (rg[nent - 1].x + rg[nent - 1].y != B->n)
+    PG_FAIL(PG_ERR_ARG, "element operator entities do not tile the rows");
+  const int64_t* swave = nullptr;
+  eop_segments(E, nullptr, &swave);
+  B->spos.assign(spos, spos + nseg + 1);
+  B->swave.assign(swave, swave + nseg + 1);
+  bool pairs = es.D == 2;
+  for (int64_t k = 0; k < nent && pairs; ++k) pairs = bp[k] == 2 * k && rg[k].y == 2;
+  // the full work set (every mesh) and room for compacted ones
+  std::vector<int32_t> wseg, wg0, ewseg, ewg0;
+  Tabs full;
+  build_tabs(B, std::vector<char>(nseg, 1), wseg, wg0, ewseg, ewg0, &full);
+  int rc;
+  if ((rc = B->alloc(&full.ewseg, full.We)) != PG_OK ||
+      (rc = B->alloc(&full.ewg0, nseg + 1)) != PG_OK ||
+      (rc = B->alloc(&B->cur.wseg, full.W)) != PG_OK ||
+      (rc = B->alloc(&B->cur.wg0, nseg + 1)) != PG_OK ||
+      (rc = B->alloc(&B->cur.ewseg, full.We)) != PG_OK ||
+      (rc = B->alloc(&B->cur.ewg0, nseg + 1)) != PG_OK ||
+      (rc = B->alloc(&B->epos0, nseg + 1)) != PG_OK ||
+      (rc = B->alloc(&B->ebptr, nent + 1)) != PG_OK ||
+      (rc = B->alloc(&B->eboff, nent + 1)) != PG_OK ||
+      (rc = B->alloc(&B->binv, boff[nent])) != PG_OK)
+    return rc;
+  if (full.W != B->W)
+    PG_FAIL(PG_ERR_ARG, "row workgroups of the element mode differ from the batch's");
+  full.wseg = B->wseg;  // the batch's own row tables cover every mesh
+  full.wg0 = B->wg0;
+  if (full.We > B->W && (rc = B->alloc(&B->part, 3 * (size_t)full.We)) != PG_OK) return rc;
+  hipStream_t st = B->st;
+  PG_HIP(hipMemcpy(full.ewseg, ewseg.data(), full.We * sizeof(int32_t), hipMemcpyHostToDevice));
+  PG_HIP(hipMemcpy(full.ewg0, ewg0.data(), (nseg + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  PG_HIP(hipMemcpy(B->epos0, spos, (nseg + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  PG_HIP(hipMemcpy(B->ebptr, bp.data(), (nent + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  PG_HIP(hipMemcpy(B->eboff, boff.data(), (nent + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  PG_HIP(hipMemsetAsync(B->bad, 0, sizeof(int32_t), st));
+  hipLaunchKernelGGL(k_ebinv, dim3((unsigned)((nent + kBS - 1) / kBS)), dim3(kBS), 0, st, nent,
+                     B->ebptr, B->eboff, B->rowptr, B->cols, B->vals, B->binv, B->bad);
+  PG_CHECK_LAUNCH();
+  PG_HIP(hipMemcpyAsync(&B->hctl->maxit, B->bad, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  if (B->hctl->maxit)
+    PG_FAIL(PG_ERR_ARG, "singular or non-contiguous entity block: block Jacobi undefined");
+  B->es = es;
+  B->D = es.D;
+  B->pairs = pairs;
+  B->full = full;
+  B->act = &B->full;
+  B->eop = E;
+  // the captured iterations now run the element products
+  return capture(B, &B->graph, &B->exec);
 }
 
 int pg_multi_solve(void* batch, const double* const* b, double* const* x, int32_t* iterations,
@@ -598,17 +1061,36 @@ int pg_multi_solve(void* batch, const double* const* b, double* const* x, int32_
   PG_CHECK_LAUNCH();
   int nlaunch = 0;
   for (int restart = 0;; ++restart) {
+    // element mode: every mesh after a (re)start; once the meshes still
+    // iterating hold under 70 % of the entities the launches cover, the
+    // launches are cut to them (tables compacted, iterations captured again)
+    B->act = &B->full;
+    hipGraphExec_t ex = B->exec;
+    int64_t covered = B->full.work;
     while (true) {
-      if (B->exec) {
-        PG_HIP(hipGraphLaunch(B->exec, st));
+      if (ex) {
+        PG_HIP(hipGraphLaunch(ex, st));
       } else {
         for (int k = 0; k < B->check; ++k) enqueue_iteration(B);
         PG_CHECK_LAUNCH();
       }
       ++nlaunch;
       PG_HIP(hipMemcpyAsync(B->hctl, B->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+      if (B->eop)
+        PG_HIP(hipMemcpyAsync(B->hseg, B->seg, B->nseg * sizeof(Seg), hipMemcpyDeviceToHost, st));
       PG_HIP(hipStreamSynchronize(st));
       if (B->hctl->all_done) break;
+      if (B->eop) {
+        int64_t work = 0;
+        for (int s = 0; s < B->nseg; ++s)
+          if (!B->hseg[s].done) work += B->spos[s + 1] - B->spos[s];
+        if (10 * work < 7 * covered) {
+          int rc = compact(B);
+          if (rc != PG_OK) return rc;
+          ex = B->cexec;
+          covered = work;
+        }
+      }
     }
     // true residuals; meshes above the tolerance restart from their iterate
     hipLaunchKernelGGL(k_mtrue, dim3(B->W), dim3(kBS), 0, st, B->wseg, B->wg0, B->row0, B->rowptr,

@@ -118,3 +118,89 @@ def test_solve_each_on_streams_is_bitwise_per_mesh():
         assert info["status"] == 0 and info["relres"] <= 1e-8
         x1, _ = S.solve(b)
         assert torch.equal(x, x1)
+
+
+# ---------------------------------------------------------------------------
+# element products + entity-block Jacobi (pg_eop_create_multi)
+# ---------------------------------------------------------------------------
+def _system_e(n, p, freq, seed, n_rx=0):
+    w = Workload(n, p, noise=0.2, seed=seed, n_rx=max(n_rx, 1), frequency=freq)
+    T = w.topo
+    mesh = core.DeviceMesh.from_topology(T, "cuda")
+    asm = core.Assembler(mesh)
+    asm.geometry(w.sigma)
+    ent_bc, dof_bc = mesh.entity_mask(T.boundary_dofs)
+    om = core.omega_of(freq)
+    vals = asm.assemble(om, ent_bc=ent_bc)
+    el = int(core.locate_points(mesh, np.array([[0.0, 0.0, -950.0]])).cpu()[0])
+    b = core.source_vector(mesh, el, [0.0, 0.0, -950.0], (1.0, 0.0, 0.0), om)
+    core.apply_dirichlet(asm.plan, None, dof_bc, b)
+    return dict(w=w, mesh=mesh, asm=asm, plan=asm.plan, vals=vals, b=b, om=om, ent_bc=ent_bc,
+                order=mesh.entity_order())
+
+
+E_CASES = {1: [(4, 0.5), (6, 2.0), (3, 1.0)], 2: [(4, 1.0), (2, 0.25), (5, 0.5)],
+           3: [(3, 1.0), (2, 0.5)]}
+
+
+def _element_batch(syss, rtol, check_every=10, use_graph=True):
+    return core.MultiMeshSolver([(s["plan"], s["vals"]) for s in syss], rtol=rtol,
+                                check_every=check_every, use_graph=use_graph,
+                                element=[(s["asm"], s["om"], s["ent_bc"], s["order"])
+                                         for s in syss])
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_multi_mesh_element_matches_oracle(p):
+    """Meshes of one order p (sizes and frequencies differ) in one batch on the
+    multi-mesh element operator with entity-block Jacobi: every mesh's solution
+    and receiver fields within 1e-6 of the oracle's SuperLU solution of its
+    system."""
+    syss = [_system_e(n, p, f, seed=20 + i, n_rx=21) for i, (n, f) in enumerate(E_CASES[p])]
+    print(f"multi-mesh element p={p}: meshes {[s['mesh'].n_elems for s in syss]}", flush=True)
+    ms = _element_batch(syss, rtol=1e-10)
+    xs, info = ms.solve([s["b"] for s in syss])
+    print(f"  iterations {info['iterations']} relres max {info['relres'].max():.2e}", flush=True)
+    assert (info["status"] == 0).all() and info["relres"].max() <= 1e-10
+    for s, x in zip(syss, xs):
+        w = s["w"]
+        T = w.topo
+        eo, fo = otopo.orientations(T.elems_nodes, T.elems_edges, T.edges_nodes,
+                                    T.elem_faces_edges)
+        ref = fem.forward(dict(edge_orient=eo, face_orient=fo, dofs=T.dofs, ndof=T.ndof,
+                               boundary_dofs=T.boundary_dofs), T.coords, w.sigma, T.p,
+                          dict(w.source, position=[0.0, 0.0, -950.0]), w.receivers)
+        rx_el = core.locate_points(s["mesh"], w.receivers)
+        fields = core.as_complex(core.receiver_fields(s["mesh"], w.receivers, rx_el, x, s["om"]))
+        xerr = np.abs(core.as_complex(x) - ref["x"]).max() / np.abs(ref["x"]).max()
+        ferr = np.abs(fields - ref["fields"]).max() / np.abs(ref["fields"]).max()
+        print(f"  n={w.n} {w.frequency} Hz: x err {xerr:.2e}, field err {ferr:.2e}", flush=True)
+        assert xerr <= 1e-6 and ferr <= 1e-6
+
+
+def test_multi_mesh_element_matches_single_solver_and_replays():
+    """The element-operator batch agrees with each mesh's single-RHS solver on
+    the same method (FP64, entity-block Jacobi, element products) to 1e-7, its
+    graph replays are bitwise its eager launches, and a second solve with other
+    right-hand sides starts afresh."""
+    syss = [_system_e(n, 2, f, seed=40 + i) for i, (n, f) in
+            enumerate([(5, 1.0), (3, 0.25), (6, 0.5), (4, 2.0)])]
+    graph = _element_batch(syss, rtol=1e-9, check_every=7)
+    eager = _element_batch(syss, rtol=1e-9, check_every=7, use_graph=False)
+    for rep in range(2):
+        bs = [(1.0 + rep) * s["b"] for s in syss]
+        xg, ig = graph.solve(bs)
+        torch.cuda.synchronize()
+        xe, ie = eager.solve(bs)
+        assert (ig["status"] == 0).all() and ig["relres"].max() <= 1e-9
+        assert np.array_equal(ie["iterations"], ig["iterations"])
+        for a, c in zip(xe, xg):
+            assert torch.equal(a, c), f"solve {rep}: graph and eager results differ"
+    for s, x in zip(syss, xg):
+        eop = core.ElementOperator(s["asm"], s["om"], s["ent_bc"], s["order"])
+        S = core.KrylovSolver(s["plan"], s["vals"], precision="fp64", pc="bjacobi", rtol=1e-9,
+                              block_ptr=s["mesh"].block_ptr(), entity_order=s["order"], eop=eop,
+                              spmv="element")
+        x1, inf1 = S.solve(2.0 * s["b"])
+        xh, x1h = core.as_complex(x), core.as_complex(x1)
+        assert np.abs(xh - x1h).max() <= 1e-7 * np.abs(x1h).max()

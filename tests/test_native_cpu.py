@@ -197,3 +197,27 @@ def test_element_operator_rejects_bad_arguments_before_device_work():
     rc = L.pg_eop_create(2, 0, fake, fake, 4, nat.ptr(bp), 2, None, None, 1.0, ctypes.byref(h),
                          None)
     assert rc == nat.PG_ERR_ARG
+
+
+def test_multi_mesh_element_operator_rejects_bad_arguments_before_device_work():
+    """pg_eop_create_multi checks the order and every mesh's entity ranges on the
+    host before it touches a device pointer."""
+    L = nat.lib()
+    h = ctypes.c_void_p()
+    fake = ctypes.c_void_p(256)  # stands for a device pointer; not read
+    bp = np.array([0, 2, 4], np.int32)
+    bad = np.array([0, 2, 5], np.int32)
+    P2 = ctypes.c_void_p * 2
+    ne = np.array([1, 1], np.int64)
+    nd = np.array([4, 4], np.int64)
+    nb = np.array([2, 2], np.int64)
+    om = np.array([1.0, 2.0])
+    args = lambda bps: (ne.ctypes.data, P2(fake, fake), P2(fake, fake), nd.ctypes.data,  # noqa
+                        P2(*[b.ctypes.data for b in bps]), nb.ctypes.data, None, None,
+                        om.ctypes.data, ctypes.byref(h), None)
+    rc = L.pg_eop_create_multi(4, 2, *args([bp, bp]))
+    assert rc == nat.PG_ERR_UNSUPPORTED and b"p=4" in L.pg_last_error()
+    rc = L.pg_eop_create_multi(2, 2, *args([bp, bad]))
+    assert rc == nat.PG_ERR_ARG and b"block_ptr" in L.pg_last_error()
+    rc = L.pg_eop_create_multi(2, 0, *args([bp, bp]))
+    assert rc == nat.PG_ERR_ARG
