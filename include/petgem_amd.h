@@ -416,6 +416,17 @@ int pg_multi_solve(void *batch, const double *const *b, double *const *x, int32_
  * CSR systems.  The operator must outlive the batch.  Replaces, like the
  * batch itself, one KSP per problem (petgem/solver.py:584-590). */
 int pg_multi_set_element_operator(void *batch, void *eop, void *stream);
+/* Optional, after pg_multi_set_element_operator: mixed precision.  Each mesh
+ * iterates complex64 COCG rounds (complex64 element products, vectors and
+ * block inverses) on a correction of its FP64 iterate; a round ends when its
+ * residual has dropped by inner_rtol or has not improved by 10 % for
+ * stall_iters iterations (0: 400), and the mesh is then refreshed in FP64
+ * (x += correction, r = b - A x with the CSR system): converged, or a new
+ * round from that residual, or, when the round gained less than a factor 2
+ * in |r|, the mesh finishes in FP64 iterations.  The FP64 parts follow the
+ * reference's solve (petgem/solver.py:584-590); the rounds have no
+ * counterpart there. */
+int pg_multi_set_mixed(void *batch, double inner_rtol, int stall_iters, void *stream);
 int pg_multi_destroy(void *batch);
 
 #ifdef __cplusplus

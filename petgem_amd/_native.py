@@ -102,6 +102,7 @@ SIGNATURES = {
                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P), _P]),
     "pg_multi_solve": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "pg_multi_set_element_operator": (ctypes.c_int, [_P, _P, _P]),
+    "pg_multi_set_mixed": (ctypes.c_int, [_P, _D, ctypes.c_int, _P]),
     "pg_multi_destroy": (ctypes.c_int, [_P]),
 }
 

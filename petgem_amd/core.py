@@ -333,11 +333,17 @@ class MultiMeshSolver:
     meshes in one launch."""
 
     def __init__(self, systems, rtol=1e-8, maxit=100000, check_every=25, use_graph=True,
-                 element=None, stream=None):
+                 element=None, precision="fp64", inner_rtol=1e-4, stall_iters=0, stream=None):
         """systems: list of (plan, values) with values (nnz, 2) float64 on the device.
         element: optional list, one per mesh in the order of `systems`, of
         (assembler, omega, ent_bc, entity_order) of the same systems (the
-        assembler after its geometry; ent_bc and entity_order may be None)."""
+        assembler after its geometry; ent_bc and entity_order may be None).
+        precision "mixed" (with `element`): complex64 rounds under FP64
+        refreshes per mesh (inner_rtol, stall_iters as in KrylovSolver)."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+        if precision == "mixed" and element is None:
+            raise ValueError("precision='mixed' needs the element operator (element=...)")
         self.handle = None
         self.eop = None
         k = len(systems)
@@ -356,6 +362,9 @@ class MultiMeshSolver:
         self.k = k
         if element is not None:
             self._set_element(element, stream)
+        if precision == "mixed":
+            nat.check(nat.lib().pg_multi_set_mixed(self.handle, float(inner_rtol),
+                                                   int(stall_iters), nat.stream_handle(stream)))
 
     def _set_element(self, element, stream):
         if len(element) != self.k:

@@ -831,6 +831,13 @@ const double2* eop_local_segments(const ElemOp* op, const int32_t* done, int dst
                        w1 - w0);
   return op->ye64;
 }
+const float2* eop_local_segments(const ElemOp* op, const int32_t* done, int dstride,
+                                 const float2* x, hipStream_t st, int64_t w0, int64_t w1) {
+  if (w1 < 0) w1 = op->nw;
+  launch_local<float>(op, done, op->wseg, dstride, x, op->ye32, op->coef32, op->tab32, st, w0,
+                      w1 - w0);
+  return op->ye32;
+}
 
 template <typename RT>
 static void multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,

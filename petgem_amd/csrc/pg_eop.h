@@ -53,13 +53,16 @@ void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double*
 // eop_segments returns the number of segments (0 for a one-mesh operator) and
 // points seg_pos at the first schedule position of each and seg_wave at the
 // first wave of each (nseg + 1 values each; schedule and waves are segment
-// after segment).  eop_local_segments writes the complex128 local results of
-// the waves [w0, w1) (w1 < 0: to the last) whose segment s is not done
-// (done[s * dstride] == 0) into the operator's buffer and returns it; the
-// caller sums them per entity with eop_schedule.
+// after segment).  eop_local_segments writes the local results (complex128
+// or complex64, as x) of the waves [w0, w1) (w1 < 0: to the last) whose
+// segment s is not done (done[s * dstride] == 0) into the operator's buffer
+// and returns it; the caller sums them per entity with eop_schedule.
 int eop_segments(const ElemOp* op, const int64_t** seg_pos, const int64_t** seg_wave);
 const double2* eop_local_segments(const ElemOp* op, const int32_t* done, int dstride,
                                   const double2* x, hipStream_t st, int64_t w0 = 0,
                                   int64_t w1 = -1);
+const float2* eop_local_segments(const ElemOp* op, const int32_t* done, int dstride,
+                                 const float2* x, hipStream_t st, int64_t w0 = 0,
+                                 int64_t w1 = -1);
 
 }  // namespace pg

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
+#include <type_traits>
 #include <vector>
 
 #include "pg_blockinv.h"
@@ -38,6 +39,12 @@ struct Seg {  // per-mesh scalars
   double2 rho, mu, alpha, beta;
   double res2, bnorm2;
   int32_t iter, done, status, pad;
+  // mixed precision: FP64 |b - A x|^2 at the last refresh, the complex64
+  // round's target |r|^2 and its stagnation watch; conv = converged (or
+  // stopped), fp64 = its complex64 rounds stopped gaining (the mesh finishes in
+  // FP64), refreshed = the FP64 residual has been formed once
+  double res64, target2, best2;
+  int32_t last_imp, conv, fp64, refreshed;
 };
 
 struct Ctl {
@@ -108,6 +115,27 @@ __device__ inline void seg_totals(const double* part, int W, int w0, int w1, dou
 __device__ inline double2 cfma(double2 a, double2 b, double2 c) {
   return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
 }
+__device__ inline float2 cfma(float2 a, float2 b, float2 c) {
+  return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, c.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, c.y)));
+}
+__device__ inline double2 to_d(double2 v) { return v; }
+__device__ inline double2 to_d(float2 v) { return make_double2(v.x, v.y); }
+template <typename T>
+__device__ inline T from_d(double2 v);
+template <>
+__device__ inline double2 from_d<double2>(double2 v) { return v; }
+template <>
+__device__ inline float2 from_d<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+template <typename T>
+struct PairOf;  // two consecutive complex values
+template <>
+struct PairOf<double2> {
+  using type = double4;
+};
+template <>
+struct PairOf<float2> {
+  using type = float4;
+};
 
 // y = A x over this workgroup's rows (kG lanes per row); MODE 0: y = A x and
 // partials of x^T y; MODE 1: y = b - A x and partials of |y|^2
@@ -147,16 +175,16 @@ __device__ inline void spmv_rows(const Span& sp, const int32_t* __restrict__ row
 }
 
 // p = z + beta_s p
+template <typename VT>
 __global__ void __launch_bounds__(kBS) k_mpupdate(const Ctl* ctl, const Seg* seg,
                                                   const int32_t* __restrict__ wseg,
                                                   const int32_t* __restrict__ wg0,
                                                   const int64_t* __restrict__ row0,
-                                                  const double2* __restrict__ z,
-                                                  double2* __restrict__ p) {
+                                                  const VT* __restrict__ z, VT* __restrict__ p) {
   if (ctl->all_done) return;
   const Span sp = span_of(wseg, wg0, row0);
   if (seg[sp.s].done) return;
-  const double2 be = seg[sp.s].beta;
+  const VT be = from_d<VT>(seg[sp.s].beta);
   for (int64_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBS) p[i] = cfma(be, p[i], z[i]);
 }
 
@@ -231,7 +259,7 @@ __global__ void __launch_bounds__(kBS) k_mupdate(const Ctl* ctl, const Seg* seg,
 
 __global__ void __launch_bounds__(kBS) k_mfin_update(const Ctl* ctl, Seg* seg,
                                                      const int32_t* __restrict__ wg0,
-                                                     const double* part, int W) {
+                                                     const double* part, int W, int stall_limit) {
   if (ctl->all_done) return;
   const int s = blockIdx.x;
   if (seg[s].done) return;
@@ -248,6 +276,16 @@ __global__ void __launch_bounds__(kBS) k_mfin_update(const Ctl* ctl, Seg* seg,
   if (!isfinite(tot[2]) || !isfinite(beta.x) || !isfinite(beta.y)) {
     c.status = PG_ERR_BREAKDOWN;
     c.done = 1;
+    c.conv = 1;
+  } else if (stall_limit > 0) {  // a complex64 round: ends at its target or stagnated
+    if (tot[2] <= c.target2 || c.iter >= ctl->maxit) {
+      c.done = 1;
+    } else if (tot[2] < 0.81 * c.best2) {
+      c.best2 = tot[2];
+      c.last_imp = c.iter;
+    } else if (c.iter - c.last_imp >= stall_limit) {
+      c.done = 1;
+    }
   } else if (tot[2] <= ctl->tol2 * c.bnorm2 || c.iter >= ctl->maxit) {
     c.done = 1;
   }
@@ -404,16 +442,15 @@ __device__ inline Span espan_of(const int32_t* __restrict__ ewseg,
 
 // q = A p from the local results: per scheduled entity the sum of its
 // incidences in ascending slot order (PEC entity: identity row); partials of p^T q
-template <int D>
+template <int D, typename VT>
 __global__ void __launch_bounds__(kBS) k_esum(const Ctl* ctl, const Seg* seg,
                                               const int32_t* __restrict__ ewseg,
                                               const int32_t* __restrict__ ewg0,
                                               const int64_t* __restrict__ epos0,
                                               const int4* __restrict__ gm,
                                               const int32_t* __restrict__ gslot,
-                                              const double2* __restrict__ ye,
-                                              const double2* __restrict__ p,
-                                              double2* __restrict__ q, double* part) {
+                                              const VT* __restrict__ ye, const VT* __restrict__ p,
+                                              VT* __restrict__ q, double* part) {
   if (ctl->all_done) return;
   const Span sp = espan_of(ewseg, ewg0, epos0);
   if (seg[sp.s].done) return;
@@ -422,16 +459,17 @@ __global__ void __launch_bounds__(kBS) k_esum(const Ctl* ctl, const Seg* seg,
   if (i < sp.hi) {
     const int4 m = gm[i];
     const int r0 = m.x, d = m.y, s0 = m.z, s1 = m.w;
-    double2 pr[D], acc[D];
+    const VT zero = from_d<VT>(make_double2(0.0, 0.0));
+    VT pr[D], acc[D];
 #pragma unroll
     for (int t = 0; t < D; ++t) {
-      pr[t] = t < d ? p[r0 + t] : make_double2(0.0, 0.0);
-      acc[t] = s0 == s1 ? pr[t] : make_double2(0.0, 0.0);
+      pr[t] = t < d ? p[r0 + t] : zero;
+      acc[t] = s0 == s1 ? pr[t] : zero;
     }
     int k = s0;
     for (; k + 2 <= s1; k += 2) {  // two incidences' loads in flight together
       const int b0 = gslot[k], b1 = gslot[k + 1];
-      double2 v0[D], v1[D];
+      VT v0[D], v1[D];
 #pragma unroll
       for (int t = 0; t < D; ++t)
         if (t < d) {
@@ -452,7 +490,7 @@ __global__ void __launch_bounds__(kBS) k_esum(const Ctl* ctl, const Seg* seg,
 #pragma unroll
       for (int t = 0; t < D; ++t)
         if (t < d) {
-          const double2 v = ye[b0 + t * kWave];
+          const VT v = ye[b0 + t * kWave];
           acc[t].x += v.x;
           acc[t].y += v.y;
         }
@@ -461,37 +499,37 @@ __global__ void __launch_bounds__(kBS) k_esum(const Ctl* ctl, const Seg* seg,
     for (int t = 0; t < D; ++t)
       if (t < d) {
         q[r0 + t] = acc[t];
-        red[0] += pr[t].x * acc[t].x - pr[t].y * acc[t].y;
-        red[1] += pr[t].x * acc[t].y + pr[t].y * acc[t].x;
+        const double2 pd = to_d(pr[t]), qd = to_d(acc[t]);
+        red[0] += pd.x * qd.x - pd.y * qd.y;
+        red[1] += pd.x * qd.y + pd.y * qd.x;
       }
   }
   write_partials<2>(red, part);
 }
 
 // x += alpha p; r -= alpha q; z = B r (entity blocks); partials of r^T z, |r|^2
-template <int D>
+template <int D, typename VT>
 __global__ void __launch_bounds__(kBS) k_eupdate(const Ctl* ctl, const Seg* seg,
                                                  const int32_t* __restrict__ ewseg,
                                                  const int32_t* __restrict__ ewg0,
                                                  const int64_t* __restrict__ epos0,
                                                  const int32_t* __restrict__ bptr,
                                                  const int32_t* __restrict__ boff,
-                                                 const double2* __restrict__ binv,
-                                                 double2* __restrict__ x, double2* __restrict__ r,
-                                                 const double2* __restrict__ p,
-                                                 const double2* __restrict__ q,
-                                                 double2* __restrict__ z, double* part) {
+                                                 const VT* __restrict__ binv, VT* __restrict__ x,
+                                                 VT* __restrict__ r, const VT* __restrict__ p,
+                                                 const VT* __restrict__ q, VT* __restrict__ z,
+                                                 double* part) {
   if (ctl->all_done) return;
   const Span sp = espan_of(ewseg, ewg0, epos0);
   if (seg[sp.s].done) return;
-  const double2 a = seg[sp.s].alpha;
-  const double2 na = make_double2(-a.x, -a.y);
+  const double2 ad = seg[sp.s].alpha;
+  const VT a = from_d<VT>(ad), na = from_d<VT>(make_double2(-ad.x, -ad.y));
   double red[3] = {0.0, 0.0, 0.0};
   const int64_t k = sp.lo + threadIdx.x;
   if (k < sp.hi) {
     const int i0 = bptr[k], d = bptr[k + 1] - i0;
-    const double2* B = binv + boff[k];
-    double2 rl[D];
+    const VT* B = binv + boff[k];
+    VT rl[D];
 #pragma unroll
     for (int t = 0; t < D; ++t)
       if (t < d) {
@@ -502,66 +540,183 @@ __global__ void __launch_bounds__(kBS) k_eupdate(const Ctl* ctl, const Seg* seg,
 #pragma unroll
     for (int t = 0; t < D; ++t)
       if (t < d) {
-        double2 zt = make_double2(0.0, 0.0);
+        VT zt = from_d<VT>(make_double2(0.0, 0.0));
 #pragma unroll
         for (int u = 0; u < D; ++u)
           if (u < d) zt = cfma(B[t * d + u], rl[u], zt);
         z[i0 + t] = zt;
-        red[0] += rl[t].x * zt.x - rl[t].y * zt.y;
-        red[1] += rl[t].x * zt.y + rl[t].y * zt.x;
-        red[2] += rl[t].x * rl[t].x + rl[t].y * rl[t].y;
+        const double2 rd = to_d(rl[t]), zd = to_d(zt);
+        red[0] += rd.x * zd.x - rd.y * zd.y;
+        red[1] += rd.x * zd.y + rd.y * zd.x;
+        red[2] += rd.x * rd.x + rd.y * rd.y;
       }
   }
   write_partials<3>(red, part);
 }
 
 // the same when every entity has two DOFs 2k, 2k + 1 (p = 2; inverse at 4k):
-// one 32-byte access per vector and entity, no index reads
+// one 32-byte (complex128) or 16-byte (complex64) access per vector and
+// entity, no index reads
+template <typename VT>
 __global__ void __launch_bounds__(kBS) k_eupdate2(const Ctl* ctl, const Seg* seg,
                                                   const int32_t* __restrict__ ewseg,
                                                   const int32_t* __restrict__ ewg0,
                                                   const int64_t* __restrict__ epos0,
-                                                  const double2* __restrict__ binv,
-                                                  double2* __restrict__ x, double2* __restrict__ r,
-                                                  const double2* __restrict__ p,
-                                                  const double2* __restrict__ q,
-                                                  double2* __restrict__ z, double* part) {
+                                                  const VT* __restrict__ binv,
+                                                  VT* __restrict__ x, VT* __restrict__ r,
+                                                  const VT* __restrict__ p,
+                                                  const VT* __restrict__ q, VT* __restrict__ z,
+                                                  double* part) {
+  using V = typename PairOf<VT>::type;
   if (ctl->all_done) return;
   const Span sp = espan_of(ewseg, ewg0, epos0);
   if (seg[sp.s].done) return;
-  const double2 a = seg[sp.s].alpha;
-  const double2 na = make_double2(-a.x, -a.y);
+  const double2 ad = seg[sp.s].alpha;
+  const VT a = from_d<VT>(ad), na = from_d<VT>(make_double2(-ad.x, -ad.y));
+  const VT zero = from_d<VT>(make_double2(0.0, 0.0));
   double red[3] = {0.0, 0.0, 0.0};
   const int64_t k = sp.lo + threadIdx.x;
   if (k < sp.hi) {
-    double4* x2 = reinterpret_cast<double4*>(x);
-    double4* r2 = reinterpret_cast<double4*>(r);
-    double4* z2 = reinterpret_cast<double4*>(z);
-    const double4 xv = x2[k], rv = r2[k];
-    const double4 pv = reinterpret_cast<const double4*>(p)[k];
-    const double4 qv = reinterpret_cast<const double4*>(q)[k];
-    const double4 B0 = reinterpret_cast<const double4*>(binv)[2 * k];
-    const double4 B1 = reinterpret_cast<const double4*>(binv)[2 * k + 1];
-    const double2 xa = cfma(a, make_double2(pv.x, pv.y), make_double2(xv.x, xv.y));
-    const double2 xb = cfma(a, make_double2(pv.z, pv.w), make_double2(xv.z, xv.w));
-    const double2 ra = cfma(na, make_double2(qv.x, qv.y), make_double2(rv.x, rv.y));
-    const double2 rb = cfma(na, make_double2(qv.z, qv.w), make_double2(rv.z, rv.w));
-    x2[k] = make_double4(xa.x, xa.y, xb.x, xb.y);
-    r2[k] = make_double4(ra.x, ra.y, rb.x, rb.y);
+    V* x2 = reinterpret_cast<V*>(x);
+    V* r2 = reinterpret_cast<V*>(r);
+    V* z2 = reinterpret_cast<V*>(z);
+    const V xv = x2[k], rv = r2[k];
+    const V pv = reinterpret_cast<const V*>(p)[k];
+    const V qv = reinterpret_cast<const V*>(q)[k];
+    const V B0 = reinterpret_cast<const V*>(binv)[2 * k];
+    const V B1 = reinterpret_cast<const V*>(binv)[2 * k + 1];
+    const VT xa = cfma(a, VT{pv.x, pv.y}, VT{xv.x, xv.y});
+    const VT xb = cfma(a, VT{pv.z, pv.w}, VT{xv.z, xv.w});
+    const VT ra = cfma(na, VT{qv.x, qv.y}, VT{rv.x, rv.y});
+    const VT rb = cfma(na, VT{qv.z, qv.w}, VT{rv.z, rv.w});
+    x2[k] = V{xa.x, xa.y, xb.x, xb.y};
+    r2[k] = V{ra.x, ra.y, rb.x, rb.y};
     // z = B r, B row-major 2 x 2: B0 = (B00, B01), B1 = (B10, B11)
-    const double2 za = cfma(make_double2(B0.z, B0.w), rb,
-                            cfma(make_double2(B0.x, B0.y), ra, make_double2(0.0, 0.0)));
-    const double2 zb = cfma(make_double2(B1.z, B1.w), rb,
-                            cfma(make_double2(B1.x, B1.y), ra, make_double2(0.0, 0.0)));
-    z2[k] = make_double4(za.x, za.y, zb.x, zb.y);
-    red[0] += ra.x * za.x - ra.y * za.y;
-    red[1] += ra.x * za.y + ra.y * za.x;
-    red[2] += ra.x * ra.x + ra.y * ra.y;
-    red[0] += rb.x * zb.x - rb.y * zb.y;
-    red[1] += rb.x * zb.y + rb.y * zb.x;
-    red[2] += rb.x * rb.x + rb.y * rb.y;
+    const VT za = cfma(VT{B0.z, B0.w}, rb, cfma(VT{B0.x, B0.y}, ra, zero));
+    const VT zb = cfma(VT{B1.z, B1.w}, rb, cfma(VT{B1.x, B1.y}, ra, zero));
+    z2[k] = V{za.x, za.y, zb.x, zb.y};
+    const double2 r0 = to_d(ra), r1 = to_d(rb), z0 = to_d(za), z1 = to_d(zb);
+    red[0] += r0.x * z0.x - r0.y * z0.y;
+    red[1] += r0.x * z0.y + r0.y * z0.x;
+    red[2] += r0.x * r0.x + r0.y * r0.y;
+    red[0] += r1.x * z1.x - r1.y * z1.y;
+    red[1] += r1.x * z1.y + r1.y * z1.x;
+    red[2] += r1.x * r1.x + r1.y * r1.y;
   }
   write_partials<3>(red, part);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision (pg_multi_set_mixed): each mesh iterates complex64 COCG
+// rounds on a correction x32; a round stops at its target |r|^2 (inner
+// tolerance times the round's first) or when it stagnates.  The host then
+// refreshes the meshes whose round stopped: x += x32, r64 = b - A x in FP64
+// (the CSR), and either the mesh has converged, or its round gained less than
+// a factor 2 in |r| (it finishes in FP64 afterwards), or a new round starts
+// from r64 rounded to complex64.
+// ---------------------------------------------------------------------------
+// x += x32, x32 = 0 over the rows of the meshes being refreshed
+__global__ void __launch_bounds__(kBS) k_saccum(const int32_t* __restrict__ wseg,
+                                                const int32_t* __restrict__ wg0,
+                                                const int64_t* __restrict__ row0,
+                                                double2* __restrict__ x, float2* __restrict__ x32) {
+  const Span sp = span_of(wseg, wg0, row0);
+  for (int64_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBS) {
+    const float2 c = x32[i];
+    x[i] = make_double2(x[i].x + (double)c.x, x[i].y + (double)c.y);
+    x32[i] = make_float2(0.f, 0.f);
+  }
+}
+
+// the refreshed |b - A x|^2 of each mesh being refreshed (row partials of k_mtrue)
+__global__ void __launch_bounds__(kBS) k_fin_srefresh(Ctl* ctl, Seg* seg,
+                                                      const int32_t* __restrict__ wg0,
+                                                      const double* part, int W) {
+  const int s = blockIdx.x;
+  if (wg0[s] == wg0[s + 1]) return;  // not being refreshed
+  double tot[1];
+  seg_totals<1>(part, W, wg0[s], wg0[s + 1], tot);
+  if (threadIdx.x != 0) return;
+  Seg& c = seg[s];
+  const double r64 = tot[0];
+  if (!c.refreshed) c.bnorm2 = r64;  // x = 0 at the first refresh
+  const double prev = c.res64;
+  c.res64 = r64;
+  c.res2 = r64;
+  c.done = 1;
+  if (!isfinite(r64)) {
+    c.status = PG_ERR_BREAKDOWN;
+    c.conv = 1;
+  } else if (r64 <= ctl->tol2 * c.bnorm2 || c.iter >= ctl->maxit) {
+    c.conv = 1;
+  } else if (c.refreshed && r64 > 0.25 * prev) {
+    c.fp64 = 1;
+  } else {
+    c.done = 0;  // a new complex64 round
+  }
+  c.refreshed = 1;
+}
+
+// new round of the meshes being refreshed that go on: r32 = r64 rounded,
+// z32 = B32 r32, p32 = 0; partials of r^T z and |r|^2
+template <int D>
+__global__ void __launch_bounds__(kBS) k_sround(const Seg* seg, const int32_t* __restrict__ ewseg,
+                                                const int32_t* __restrict__ ewg0,
+                                                const int64_t* __restrict__ epos0,
+                                                const int32_t* __restrict__ bptr,
+                                                const int32_t* __restrict__ boff,
+                                                const float2* __restrict__ binv,
+                                                const double2* __restrict__ r64,
+                                                float2* __restrict__ r, float2* __restrict__ z,
+                                                float2* __restrict__ p, double* part) {
+  const Span sp = espan_of(ewseg, ewg0, epos0);
+  if (seg[sp.s].done) return;
+  double red[3] = {0.0, 0.0, 0.0};
+  const int64_t k = sp.lo + threadIdx.x;
+  if (k < sp.hi) {
+    const int i0 = bptr[k], d = bptr[k + 1] - i0;
+    const float2* B = binv + boff[k];
+    float2 rl[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      if (t < d) {
+        rl[t] = from_d<float2>(r64[i0 + t]);
+        r[i0 + t] = rl[t];
+        p[i0 + t] = make_float2(0.f, 0.f);
+      }
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+      if (t < d) {
+        float2 zt = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+          if (u < d) zt = cfma(B[t * d + u], rl[u], zt);
+        z[i0 + t] = zt;
+        const double2 rd = to_d(rl[t]), zd = to_d(zt);
+        red[0] += rd.x * zd.x - rd.y * zd.y;
+        red[1] += rd.x * zd.y + rd.y * zd.x;
+        red[2] += rd.x * rd.x + rd.y * rd.y;
+      }
+  }
+  write_partials<3>(red, part);
+}
+
+__global__ void __launch_bounds__(kBS) k_fin_sround(Ctl* ctl, Seg* seg,
+                                                    const int32_t* __restrict__ ewg0,
+                                                    const double* part, int We, double inner2) {
+  const int s = blockIdx.x;
+  if (ewg0[s] == ewg0[s + 1] || seg[s].done) return;
+  double tot[3];
+  seg_totals<3>(part, We, ewg0[s], ewg0[s + 1], tot);
+  if (threadIdx.x != 0) return;
+  Seg& c = seg[s];
+  c.rho = make_double2(tot[0], tot[1]);
+  c.beta = make_double2(0.0, 0.0);
+  c.alpha = make_double2(0.0, 0.0);
+  c.target2 = inner2 * tot[2];
+  c.best2 = tot[2];
+  c.last_imp = c.iter;
+  ctl->all_done = 0;  // every resuming mesh stores the same value
 }
 
 // (re)start after r = b - A x: z = B r, p = 0; partials of r^T z, |r|^2
@@ -650,13 +805,25 @@ struct Multi {
   int32_t *ebptr = nullptr, *eboff = nullptr;
   int64_t* epos0 = nullptr;
   double2* binv = nullptr;
+  int64_t nbinv = 0;
   std::vector<int64_t> spos, swave;  // per mesh: first entity, first wave (nseg + 1)
-  Tabs full, cur;
+  Tabs full, cur, ref;  // every mesh; a compacted set; the meshes being refreshed
   const Tabs* act = nullptr;  // the set the iterations launch
   hipGraph_t cgraph = nullptr;  // iterations on the compacted set
   hipGraphExec_t cexec = nullptr;
+  // mixed precision (pg_multi_set_mixed): complex64 iteration vectors and
+  // block inverses; phase32 = the iterations being launched are complex64
+  bool mixed = false, phase32 = false;
+  double inner2 = 1e-8;
+  int stall = 400;
+  float2 *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *z32 = nullptr,
+         *binv32 = nullptr;
+  hipGraph_t mgraph = nullptr;  // complex64 iterations on every mesh
+  hipGraphExec_t mexec = nullptr;
   std::vector<void*> owned;
   ~Multi() {  // release errors cannot be reported from here
+    if (mexec) (void)hipGraphExecDestroy(mexec);
+    if (mgraph) (void)hipGraphDestroy(mgraph);
     if (cexec) (void)hipGraphExecDestroy(cexec);
     if (cgraph) (void)hipGraphDestroy(cgraph);
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -685,25 +852,53 @@ static const int32_t* seg_done(const Multi* B) {
 static_assert(sizeof(Seg) % sizeof(int32_t) == 0, "Seg stride in int32 units");
 constexpr int kSegStride = (int)(sizeof(Seg) / sizeof(int32_t));
 
-template <int D>
-static void enqueue_iteration_e(Multi* B, const Tabs& T) {
+template <int D, typename VT>
+static void enqueue_iteration_e(Multi* B, const Tabs& T, VT* x, VT* r, VT* p, VT* q, VT* z,
+                                const VT* binv) {
   const dim3 g(T.W), ge(T.We), b(kBS), gs(B->nseg);
-  hipLaunchKernelGGL(k_mpupdate, g, b, 0, B->st, B->ctl, B->seg, T.wseg, T.wg0, B->row0, B->z,
-                     B->p);
-  const double2* ye = nullptr;
+  hipLaunchKernelGGL(k_mpupdate<VT>, g, b, 0, B->st, B->ctl, B->seg, T.wseg, T.wg0, B->row0, z,
+                     p);
+  const VT* ye = nullptr;
   for (const auto& run : T.runs)
-    ye = eop_local_segments(B->eop, seg_done(B), kSegStride, B->p, B->st, run.first, run.second);
-  hipLaunchKernelGGL(k_esum<D>, ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0, B->epos0,
-                     B->es.gm, B->es.gslot, ye, B->p, B->q, B->part);
+    ye = eop_local_segments(B->eop, seg_done(B), kSegStride, p, B->st, run.first, run.second);
+  hipLaunchKernelGGL((k_esum<D, VT>), ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0, B->epos0,
+                     B->es.gm, B->es.gslot, ye, p, q, B->part);
   hipLaunchKernelGGL(k_mfin_mu, gs, b, 0, B->st, B->ctl, B->seg, T.ewg0, B->part, T.We);
   if (D == 2 && B->pairs)
-    hipLaunchKernelGGL(k_eupdate2, ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0, B->epos0,
-                       B->binv, B->x, B->r, B->p, B->q, B->z, B->part);
+    hipLaunchKernelGGL(k_eupdate2<VT>, ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0,
+                       B->epos0, binv, x, r, p, q, z, B->part);
   else
-    hipLaunchKernelGGL(k_eupdate<D>, ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0, B->epos0,
-                       B->ebptr, B->eboff, B->binv, B->x, B->r, B->p, B->q, B->z, B->part);
-  hipLaunchKernelGGL(k_mfin_update, gs, b, 0, B->st, B->ctl, B->seg, T.ewg0, B->part, T.We);
+    hipLaunchKernelGGL((k_eupdate<D, VT>), ge, b, 0, B->st, B->ctl, B->seg, T.ewseg, T.ewg0,
+                       B->epos0, B->ebptr, B->eboff, binv, x, r, p, q, z, B->part);
+  hipLaunchKernelGGL(k_mfin_update, gs, b, 0, B->st, B->ctl, B->seg, T.ewg0, B->part, T.We,
+                     std::is_same<VT, float2>::value ? B->stall : 0);
   hipLaunchKernelGGL(k_mfin_all, dim3(1), b, 0, B->st, B->ctl, B->seg, B->nseg);
+}
+
+template <int D>
+static void enqueue_iteration_e(Multi* B, const Tabs& T) {
+  if (B->phase32)
+    enqueue_iteration_e<D, float2>(B, T, B->x32, B->r32, B->p32, B->q32, B->z32, B->binv32);
+  else
+    enqueue_iteration_e<D, double2>(B, T, B->x, B->r, B->p, B->q, B->z, B->binv);
+}
+
+// refresh of the meshes in `ref` (complex64 rounds): x += x32, r64 = b - A x,
+// then a new round for those that go on (rows of r64 in B->r)
+template <int D>
+static void enqueue_refresh(Multi* B) {
+  const Tabs& T = B->ref;
+  hipLaunchKernelGGL(k_saccum, dim3(T.W), dim3(kBS), 0, B->st, T.wseg, T.wg0, B->row0, B->x,
+                     B->x32);
+  hipLaunchKernelGGL(k_mtrue, dim3(T.W), dim3(kBS), 0, B->st, T.wseg, T.wg0, B->row0, B->rowptr,
+                     B->cols, B->vals, B->x, B->b, B->r, B->part);
+  hipLaunchKernelGGL(k_fin_srefresh, dim3(B->nseg), dim3(kBS), 0, B->st, B->ctl, B->seg, T.wg0,
+                     B->part, T.W);
+  hipLaunchKernelGGL(k_sround<D>, dim3(T.We), dim3(kBS), 0, B->st, B->seg, T.ewseg, T.ewg0,
+                     B->epos0, B->ebptr, B->eboff, B->binv32, B->r, B->r32, B->z32, B->p32,
+                     B->part);
+  hipLaunchKernelGGL(k_fin_sround, dim3(B->nseg), dim3(kBS), 0, B->st, B->ctl, B->seg, T.ewg0,
+                     B->part, T.We, B->inner2);
 }
 
 template <int D>
@@ -728,14 +923,15 @@ static void enqueue_iteration(Multi* B) {
     return;
   }
   const dim3 g(B->W), b(kBS), gs(B->nseg);
-  hipLaunchKernelGGL(k_mpupdate, g, b, 0, B->st, B->ctl, B->seg, B->wseg, B->wg0, B->row0, B->z,
-                     B->p);
+  hipLaunchKernelGGL(k_mpupdate<double2>, g, b, 0, B->st, B->ctl, B->seg, B->wseg, B->wg0,
+                     B->row0, B->z, B->p);
   hipLaunchKernelGGL(k_mspmv, g, b, 0, B->st, B->ctl, B->seg, B->wseg, B->wg0, B->row0, B->rowptr,
                      B->cols, B->vals, B->p, B->q, B->part);
   hipLaunchKernelGGL(k_mfin_mu, gs, b, 0, B->st, B->ctl, B->seg, B->wg0, B->part, B->W);
   hipLaunchKernelGGL(k_mupdate, g, b, 0, B->st, B->ctl, B->seg, B->wseg, B->wg0, B->row0, B->x,
                      B->r, B->p, B->q, B->dinv, B->z, B->part);
-  hipLaunchKernelGGL(k_mfin_update, gs, b, 0, B->st, B->ctl, B->seg, B->wg0, B->part, B->W);
+  hipLaunchKernelGGL(k_mfin_update, gs, b, 0, B->st, B->ctl, B->seg, B->wg0, B->part, B->W,
+                     0);
   hipLaunchKernelGGL(k_mfin_all, dim3(1), b, 0, B->st, B->ctl, B->seg, B->nseg);
 }
 
@@ -801,26 +997,103 @@ static void build_tabs(const Multi* B, const std::vector<char>& active, std::vec
   T->We = (int)ewseg.size();
 }
 
-// the compacted set of the meshes not yet done (hseg read back), uploaded to
-// the `cur` tables (no launch in flight) and captured
-static int compact(Multi* B) {
-  std::vector<char> active(B->nseg);
-  for (int s = 0; s < B->nseg; ++s) active[s] = B->hseg[s].done ? 0 : 1;
+// tables of the meshes with active[s] set, uploaded to T's device buffers (no
+// launch in flight that reads them)
+static int upload_tabs(Multi* B, const std::vector<char>& active, Tabs* T) {
   std::vector<int32_t> wseg, wg0, ewseg, ewg0;
-  build_tabs(B, active, wseg, wg0, ewseg, ewg0, &B->cur);
-  Tabs& T = B->cur;
-  PG_HIP(hipMemcpyAsync(T.wseg, wseg.data(), std::max<size_t>(1, wseg.size()) * sizeof(int32_t),
+  build_tabs(B, active, wseg, wg0, ewseg, ewg0, T);
+  PG_HIP(hipMemcpyAsync(T->wseg, wseg.data(), std::max<size_t>(1, wseg.size()) * sizeof(int32_t),
                         hipMemcpyHostToDevice, B->st));
-  PG_HIP(hipMemcpyAsync(T.wg0, wg0.data(), wg0.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+  PG_HIP(hipMemcpyAsync(T->wg0, wg0.data(), wg0.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                         B->st));
-  PG_HIP(hipMemcpyAsync(T.ewseg, ewseg.data(),
+  PG_HIP(hipMemcpyAsync(T->ewseg, ewseg.data(),
                         std::max<size_t>(1, ewseg.size()) * sizeof(int32_t),
                         hipMemcpyHostToDevice, B->st));
-  PG_HIP(hipMemcpyAsync(T.ewg0, ewg0.data(), ewg0.size() * sizeof(int32_t),
+  PG_HIP(hipMemcpyAsync(T->ewg0, ewg0.data(), ewg0.size() * sizeof(int32_t),
                         hipMemcpyHostToDevice, B->st));
   PG_HIP(hipStreamSynchronize(B->st));
+  return PG_OK;
+}
+
+// launch only the meshes with active[s] from now on (compacted tables,
+// iterations captured again)
+static int compact(Multi* B, const std::vector<char>& active) {
+  int rc = upload_tabs(B, active, &B->cur);
+  if (rc != PG_OK) return rc;
   B->act = &B->cur;
   return capture(B, &B->cgraph, &B->cexec);
+}
+
+// refresh the meshes with mask[s] (complex64 rounds); hseg read back after
+static int refresh(Multi* B, const std::vector<char>& mask) {
+  int rc = upload_tabs(B, mask, &B->ref);
+  if (rc != PG_OK) return rc;
+  if (B->D == 1) enqueue_refresh<1>(B);
+  else if (B->D == 2) enqueue_refresh<2>(B);
+  else enqueue_refresh<6>(B);
+  PG_CHECK_LAUNCH();
+  PG_HIP(hipMemcpyAsync(B->hseg, B->seg, B->nseg * sizeof(Seg), hipMemcpyDeviceToHost, B->st));
+  PG_HIP(hipStreamSynchronize(B->st));
+  return PG_OK;
+}
+
+__global__ void k_seg_reset(Seg* seg, int nseg) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg) seg[s] = Seg{};
+}
+__global__ void k_demote(int64_t n, const double2* __restrict__ a, float2* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+// the complex64 rounds of a mixed solve; returns with every mesh converged
+// (conv) or handed to FP64 (fp64), x accumulated
+static int mixed_rounds(Multi* B, int* nlaunch) {
+  hipStream_t st = B->st;
+  hipLaunchKernelGGL(k_seg_reset, dim3((B->nseg + kBS - 1) / kBS), dim3(kBS), 0, st, B->seg,
+                     B->nseg);
+  PG_HIP(hipMemsetAsync(B->x32, 0, (size_t)B->n * sizeof(float2), st));
+  PG_CHECK_LAUNCH();
+  std::vector<char> mask(B->nseg, 1), live(B->nseg, 1);
+  int rc = refresh(B, mask);  // the first refresh: r64 = b, |b|^2, the first rounds
+  if (rc != PG_OK) return rc;
+  B->phase32 = true;
+  B->act = &B->full;
+  hipGraphExec_t ex = B->mexec;
+  int64_t covered = B->full.work;
+  while (true) {
+    bool any = false, paused = false;
+    int64_t work = 0;
+    for (int s = 0; s < B->nseg; ++s) {
+      const Seg& c = B->hseg[s];
+      live[s] = !c.conv && !c.fp64;
+      mask[s] = live[s] && c.done;
+      any = any || live[s];
+      paused = paused || mask[s];
+      if (live[s]) work += B->spos[s + 1] - B->spos[s];
+    }
+    if (!any) break;
+    if (paused) {  // rounds that stopped since the last check
+      if ((rc = refresh(B, mask)) != PG_OK) return rc;
+      continue;
+    }
+    if (10 * work < 7 * covered) {  // launch only the meshes still iterating
+      if ((rc = compact(B, live)) != PG_OK) return rc;
+      ex = B->cexec;
+      covered = work;
+    }
+    if (ex) {
+      PG_HIP(hipGraphLaunch(ex, st));
+    } else {
+      for (int k = 0; k < B->check; ++k) enqueue_iteration(B);
+      PG_CHECK_LAUNCH();
+    }
+    ++*nlaunch;
+    PG_HIP(hipMemcpyAsync(B->hseg, B->seg, B->nseg * sizeof(Seg), hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+  }
+  B->phase32 = false;
+  return PG_OK;
 }
 
 }  // namespace multi
@@ -1007,6 +1280,10 @@ int pg_multi_set_element_operator(void* batch, void* op, void* stream) {
       (rc = B->alloc(&B->cur.wg0, nseg + 1)) != PG_OK ||
       (rc = B->alloc(&B->cur.ewseg, full.We)) != PG_OK ||
       (rc = B->alloc(&B->cur.ewg0, nseg + 1)) != PG_OK ||
+      (rc = B->alloc(&B->ref.wseg, full.W)) != PG_OK ||
+      (rc = B->alloc(&B->ref.wg0, nseg + 1)) != PG_OK ||
+      (rc = B->alloc(&B->ref.ewseg, full.We)) != PG_OK ||
+      (rc = B->alloc(&B->ref.ewg0, nseg + 1)) != PG_OK ||
       (rc = B->alloc(&B->epos0, nseg + 1)) != PG_OK ||
       (rc = B->alloc(&B->ebptr, nent + 1)) != PG_OK ||
       (rc = B->alloc(&B->eboff, nent + 1)) != PG_OK ||
@@ -1034,6 +1311,7 @@ int pg_multi_set_element_operator(void* batch, void* op, void* stream) {
   B->es = es;
   B->D = es.D;
   B->pairs = pairs;
+  B->nbinv = boff[nent];
   B->full = full;
   B->act = &B->full;
   B->eop = E;
@@ -1057,9 +1335,18 @@ int pg_multi_solve(void* batch, const double* const* b, double* const* x, int32_
   PG_HIP(hipMemsetAsync(B->x, 0, B->n * sizeof(double2), st));
   hipLaunchKernelGGL(k_mctl_init, dim3(1), dim3(64), 0, st, B->ctl, B->maxit, B->rtol * B->rtol,
                      B->nseg);
-  enqueue_restart(B, 1);
+  int nlaunch = 0, rc;
+  bool iterate64 = true;
+  if (B->mixed) {
+    // complex64 rounds; the meshes handed to FP64 then go on from their iterate
+    if ((rc = mixed_rounds(B, &nlaunch)) != PG_OK) return rc;
+    iterate64 = false;
+    for (int s = 0; s < B->nseg; ++s) iterate64 = iterate64 || B->hseg[s].fp64;
+    if (iterate64) enqueue_restart(B, 0);
+  } else {
+    enqueue_restart(B, 1);
+  }
   PG_CHECK_LAUNCH();
-  int nlaunch = 0;
   for (int restart = 0;; ++restart) {
     // element mode: every mesh after a (re)start; once the meshes still
     // iterating hold under 70 % of the entities the launches cover, the
@@ -1067,7 +1354,24 @@ int pg_multi_solve(void* batch, const double* const* b, double* const* x, int32_
     B->act = &B->full;
     hipGraphExec_t ex = B->exec;
     int64_t covered = B->full.work;
-    while (true) {
+    if (iterate64 && B->eop) {
+      PG_HIP(hipMemcpyAsync(B->hseg, B->seg, B->nseg * sizeof(Seg), hipMemcpyDeviceToHost, st));
+      PG_HIP(hipStreamSynchronize(st));
+    }
+    while (iterate64) {
+      if (B->eop) {
+        std::vector<char> live(B->nseg);
+        int64_t work = 0;
+        for (int s = 0; s < B->nseg; ++s) {
+          live[s] = !B->hseg[s].done;
+          if (live[s]) work += B->spos[s + 1] - B->spos[s];
+        }
+        if (10 * work < 7 * covered && work > 0) {
+          if ((rc = compact(B, live)) != PG_OK) return rc;
+          ex = B->cexec;
+          covered = work;
+        }
+      }
       if (ex) {
         PG_HIP(hipGraphLaunch(ex, st));
       } else {
@@ -1080,18 +1384,8 @@ int pg_multi_solve(void* batch, const double* const* b, double* const* x, int32_
         PG_HIP(hipMemcpyAsync(B->hseg, B->seg, B->nseg * sizeof(Seg), hipMemcpyDeviceToHost, st));
       PG_HIP(hipStreamSynchronize(st));
       if (B->hctl->all_done) break;
-      if (B->eop) {
-        int64_t work = 0;
-        for (int s = 0; s < B->nseg; ++s)
-          if (!B->hseg[s].done) work += B->spos[s + 1] - B->spos[s];
-        if (10 * work < 7 * covered) {
-          int rc = compact(B);
-          if (rc != PG_OK) return rc;
-          ex = B->cexec;
-          covered = work;
-        }
-      }
     }
+    iterate64 = true;
     // true residuals; meshes above the tolerance restart from their iterate
     hipLaunchKernelGGL(k_mtrue, dim3(B->W), dim3(kBS), 0, st, B->wseg, B->wg0, B->row0, B->rowptr,
                        B->cols, B->vals, B->x, B->b, B->r, B->part);
@@ -1121,6 +1415,38 @@ int pg_multi_solve(void* batch, const double* const* b, double* const* x, int32_
   }
   PG_HIP(hipStreamSynchronize(st));
   if (launches) *launches = nlaunch;
+  return PG_OK;
+}
+
+int pg_multi_set_mixed(void* batch, double inner_rtol, int stall_iters, void* stream) {
+  if (!batch) PG_FAIL(PG_ERR_ARG, "bad arguments");
+  Multi* B = static_cast<Multi*>(batch);
+  if (!B->eop)
+    PG_FAIL(PG_ERR_UNSUPPORTED, "mixed precision needs the element operator "
+                                "(pg_multi_set_element_operator)");
+  if (!(inner_rtol > 0.0 && inner_rtol < 1.0)) PG_FAIL(PG_ERR_ARG, "inner_rtol must be in (0, 1)");
+  PG_HIP(hipStreamSynchronize(as_stream(stream)));
+  int rc;
+  if (!B->x32) {
+    if ((rc = B->alloc(&B->x32, B->n)) != PG_OK || (rc = B->alloc(&B->r32, B->n)) != PG_OK ||
+        (rc = B->alloc(&B->p32, B->n)) != PG_OK || (rc = B->alloc(&B->q32, B->n)) != PG_OK ||
+        (rc = B->alloc(&B->z32, B->n)) != PG_OK || (rc = B->alloc(&B->binv32, B->nbinv)) != PG_OK)
+      return rc;
+    hipLaunchKernelGGL(k_demote, dim3((unsigned)((B->nbinv + kBS - 1) / kBS)), dim3(kBS), 0, B->st,
+                       B->nbinv, B->binv, B->binv32);
+    PG_CHECK_LAUNCH();
+    // the complex64 products of every mesh start from zero local results
+    PG_HIP(hipMemsetAsync(B->p32, 0, (size_t)B->n * sizeof(float2), B->st));
+    PG_HIP(hipStreamSynchronize(B->st));
+  }
+  B->inner2 = inner_rtol * inner_rtol;
+  B->stall = stall_iters > 0 ? stall_iters : 400;
+  B->phase32 = true;
+  B->act = &B->full;
+  rc = capture(B, &B->mgraph, &B->mexec);
+  B->phase32 = false;
+  if (rc != PG_OK) return rc;
+  B->mixed = true;
   return PG_OK;
 }
 

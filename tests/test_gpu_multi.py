@@ -143,22 +143,24 @@ E_CASES = {1: [(4, 0.5), (6, 2.0), (3, 1.0)], 2: [(4, 1.0), (2, 0.25), (5, 0.5)]
            3: [(3, 1.0), (2, 0.5)]}
 
 
-def _element_batch(syss, rtol, check_every=10, use_graph=True):
+def _element_batch(syss, rtol, check_every=10, use_graph=True, precision="fp64", **kw):
     return core.MultiMeshSolver([(s["plan"], s["vals"]) for s in syss], rtol=rtol,
                                 check_every=check_every, use_graph=use_graph,
                                 element=[(s["asm"], s["om"], s["ent_bc"], s["order"])
-                                         for s in syss])
+                                         for s in syss], precision=precision, **kw)
 
 
-@pytest.mark.parametrize("p", [1, 2, 3])
-def test_multi_mesh_element_matches_oracle(p):
+@pytest.mark.parametrize("p,precision", [(1, "fp64"), (2, "fp64"), (3, "fp64"), (1, "mixed"),
+                                         (2, "mixed"), (3, "mixed")])
+def test_multi_mesh_element_matches_oracle(p, precision):
     """Meshes of one order p (sizes and frequencies differ) in one batch on the
-    multi-mesh element operator with entity-block Jacobi: every mesh's solution
-    and receiver fields within 1e-6 of the oracle's SuperLU solution of its
-    system."""
+    multi-mesh element operator with entity-block Jacobi, in FP64 and in
+    complex64 rounds under FP64 refreshes: every mesh's solution and receiver
+    fields within 1e-6 of the oracle's SuperLU solution of its system."""
     syss = [_system_e(n, p, f, seed=20 + i, n_rx=21) for i, (n, f) in enumerate(E_CASES[p])]
-    print(f"multi-mesh element p={p}: meshes {[s['mesh'].n_elems for s in syss]}", flush=True)
-    ms = _element_batch(syss, rtol=1e-10)
+    print(f"multi-mesh element p={p} {precision}: meshes {[s['mesh'].n_elems for s in syss]}",
+          flush=True)
+    ms = _element_batch(syss, rtol=1e-10, precision=precision)
     xs, info = ms.solve([s["b"] for s in syss])
     print(f"  iterations {info['iterations']} relres max {info['relres'].max():.2e}", flush=True)
     assert (info["status"] == 0).all() and info["relres"].max() <= 1e-10
@@ -204,3 +206,33 @@ def test_multi_mesh_element_matches_single_solver_and_replays():
         x1, inf1 = S.solve(2.0 * s["b"])
         xh, x1h = core.as_complex(x), core.as_complex(x1)
         assert np.abs(xh - x1h).max() <= 1e-7 * np.abs(x1h).max()
+
+
+def test_multi_mesh_mixed_rounds_and_fp64_hand_over():
+    """Mixed precision on the multi-mesh element operator: the solve is
+    repeatable bitwise (graph replays of the complex64 rounds included), a
+    stall limit small enough to stop every complex64 round early still
+    converges (meshes refreshed or handed to FP64), and both agree with the
+    FP64 batch to the tolerance."""
+    syss = [_system_e(n, 2, f, seed=60 + i) for i, (n, f) in
+            enumerate([(5, 0.1), (3, 1.0), (6, 0.25), (4, 2.0)])]
+    bs = [s["b"] for s in syss]
+    ref, iref = _element_batch(syss, rtol=1e-10).solve(bs)
+    mx = _element_batch(syss, rtol=1e-10, check_every=7, precision="mixed")
+    x1, i1 = mx.solve(bs)
+    torch.cuda.synchronize()
+    x2, i2 = mx.solve(bs)
+    assert (i1["status"] == 0).all() and i1["relres"].max() <= 1e-10
+    assert np.array_equal(i1["iterations"], i2["iterations"])
+    for a, c in zip(x1, x2):
+        assert torch.equal(a, c)
+    tight = _element_batch(syss, rtol=1e-10, check_every=5, precision="mixed", stall_iters=3,
+                           use_graph=False)
+    x3, i3 = tight.solve(bs)
+    print(f"mixed iterations {i1['iterations']}, with stall limit 3 {i3['iterations']}, "
+          f"FP64 {iref['iterations']}", flush=True)
+    assert (i3["status"] == 0).all() and i3["relres"].max() <= 1e-10
+    for xs in (x1, x3):
+        for a, r in zip(xs, ref):
+            ah, rh = core.as_complex(a), core.as_complex(r)
+            assert np.abs(ah - rh).max() <= 1e-7 * np.abs(rh).max()
