@@ -310,6 +310,71 @@ def batched_leg(args, log):
                          "once")}
 
 
+def c3_leg(args, log):
+    """Mixed precision at p = 3: 24^3 Kuhn hexes x 6 = 82,944 tets (p = 3), the
+    layered model with the resistive block, HED at (0, 0, -950) m, 1 Hz and
+    0.25 Hz.  At each frequency the FP64 solve (COCG + entity-block Jacobi,
+    products on the FP64 matrix cores or the CSR, as the autotuner picks) and
+    the mixed solve (complex64 iterations under FP64 refinement) of the same
+    system, one warm-up and one timed solve each; receiver fields of the two
+    compared."""
+    import torch
+    from petgem_amd import core
+    from petgem_amd.synthetic import CONFIGS, Workload
+
+    w = Workload(**CONFIGS["c3"], seed=2026)
+    T = w.topo
+    dev = torch.device("cuda")
+    mesh = core.DeviceMesh.from_topology(T, dev)
+    asm = core.Assembler(mesh)
+    plan = asm.plan
+    ent_bc, dof_bc = mesh.entity_mask(T.boundary_dofs)
+    asm.geometry(torch.from_numpy(w.sigma).to(dev))
+    bptr, order = mesh.block_ptr(), mesh.entity_order()
+    src_el = int(core.locate_points(mesh, np.array(w.source["position"])[None]).cpu()[0])
+    rx_el = core.locate_points(mesh, w.receivers)
+    moment = core.dipole_moment(w.source)
+    out = {"ndof": int(plan.ndof), "n_elems": int(T.n_elems)}
+    for f in (1.0, 0.25):
+        om = core.omega_of(f)
+        vals = asm.assemble(om, ent_bc=ent_bc)
+        b = torch.zeros((plan.ndof, 2), dtype=torch.float64, device=dev)
+        core.source_vector(mesh, src_el, w.source["position"], moment, om, b=b)
+        core.apply_dirichlet(plan, None, dof_bc, b)
+        eop = core.ElementOperator(asm, om, ent_bc, order)
+        res = {}
+        for prec in ("fp64", "mixed"):
+            S = core.KrylovSolver(plan, vals, precision=prec, rtol=args.rtol, pc=args.pc,
+                                  block_ptr=bptr, entity_order=order, eop=eop)
+            S.solve(b)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, info = S.solve(b)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if info["status"] != 0 or info["relres"] > args.rtol:
+                raise RuntimeError(f"C3 {prec} solve at {f} Hz did not converge: {info}")
+            fields = core.as_complex(core.receiver_fields(mesh, w.receivers, rx_el, x, om))
+            res[prec] = (dt, info, fields)
+            log(f"c3 {f} Hz {prec}: {dt*1e3:.1f} ms, {info['iterations']} iterations, "
+                f"relres {info['relres']:.2e}")
+            del S
+        f64, fmx = res["fp64"][2], res["mixed"][2]
+        key = f"{f:g}Hz"
+        out[key] = {"fp64_s": round(res["fp64"][0], 4), "mixed_s": round(res["mixed"][0], 4),
+                    "mixed_speedup": round(res["fp64"][0] / res["mixed"][0], 3),
+                    "fp64_iterations": int(res["fp64"][1]["iterations"]),
+                    "mixed_iterations": int(res["mixed"][1]["iterations"]),
+                    "mixed_refinements": int(res["mixed"][1]["refinements"]),
+                    "fields_mixed_vs_fp64": float(np.abs(fmx - f64).max() / np.abs(f64).max())}
+        del eop, vals
+        torch.cuda.empty_cache()
+    out["workload"] = (f"c3: p=3, 24^3 Kuhn x 6 = {T.n_elems} tets, {plan.ndof} DOFs, layered "
+                       "model with the resistive block, HED at (0,0,-950) m, 1 and 0.25 Hz, rtol "
+                       f"{args.rtol}; FP64 vs mixed solve of each, {args.pc}")
+    return out
+
+
 def multimesh_leg(args, log):
     """32 independent meshes, 8^3 .. 24^3 Kuhn hexes (p=2), seeded random layered
     conductivities (0.01 - 3.3 S/m), one HED each, solved in one multi-mesh batch
@@ -499,9 +564,10 @@ def main():
     ap.add_argument("--frequency", type=float, default=None,
                     help="source frequency of the solve leg (default: the workload's, 1 Hz)")
     ap.add_argument("--pc", default="bjacobi", choices=["jacobi", "bjacobi"])
-    ap.add_argument("--legs", default="solve,dropin,batch,multimesh,adjoint",
+    ap.add_argument("--legs", default="solve,dropin,c3,batch,multimesh,adjoint",
                     help="comma list of: solve (headline), dropin (the reference's petsc.opts "
-                         "solver), batch (RHS/s), multimesh (RHS/s), adjoint (grad/s)")
+                         "solver), c3 (p = 3 FP64 vs mixed), batch (RHS/s), multimesh "
+                         "(RHS/s), adjoint (grad/s)")
     ap.add_argument("--batch-width", type=int, default=64,
                     help="columns per batched solve (chunks of one frequency's positions)")
     ap.add_argument("--batch-precision", default="mixed", choices=["fp64", "mixed"],
@@ -655,6 +721,7 @@ def main():
         del Sg
     del solvers, values
     torch.cuda.empty_cache()
+    c3 = c3_leg(args, log) if "c3" in legs else None
     batched = batched_leg(args, log) if "batch" in legs else None
     multimesh = multimesh_leg(args, log) if "multimesh" in legs else None
     adjoint = adjoint_leg(args, log) if "adjoint" in legs else None
@@ -699,6 +766,7 @@ def main():
         "assembly_ms": round(t_asm_ms, 3),
         "roofline": roof64,
         "dropin_petsc_opts": dropin,
+        "c3_mixed_precision": c3,
         "batched": batched,
         "multimesh": multimesh,
         "adjoint": adjoint,

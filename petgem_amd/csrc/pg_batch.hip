@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(kBS) k_fin_update(const Ctl* ctl, Col* col, in
       cc.best2 = tot[2];
       cc.last_imp = cc.iter;
     } else if (cc.iter - cc.last_imp >= ctl->stall_limit) {
-      cc.stalled = 1;  // complex64 round stagnated: the column finishes in FP64
+      cc.stalled = 1;  // complex64 round stagnated (see k_fin_refine)
       cc.done = 1;
     }
   }
@@ -818,6 +818,10 @@ __global__ void __launch_bounds__(kBS) k_fin_refine(const Ctl* ctl, Col* col, in
     cc = Col{};
     cc.bnorm2 = tot[2];
   }
+  // a round that stagnated after cutting the true residual at least tenfold
+  // is followed by another complex64 round (iterative refinement limited by
+  // complex64 accuracy); without that progress the column finishes in FP64
+  if (!first && cc.stalled && tot[2] <= 1e-2 * 1e-2 * cc.res64) cc.stalled = 0;
   cc.res64 = tot[2];
   cc.res2 = tot[2];
   cc.rho = make_double2(tot[0], tot[1]);
