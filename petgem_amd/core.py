@@ -330,10 +330,14 @@ class MultiMeshSolver:
     mesh, all advancing in the same launches; the iterations between host
     checks replayed from a captured HIP graph).  Point Jacobi and CSR products,
     or with `element` entity-block Jacobi and the element products of all
-    meshes in one launch."""
+    meshes in one launch.  With lanes > 1 the meshes are dealt to that many
+    such batches (largest first, each to the lane with the fewest DOFs so
+    far), which run concurrently on their own streams; a mesh's result does
+    not depend on the other meshes of its batch, so it is bitwise the same."""
 
     def __init__(self, systems, rtol=1e-8, maxit=100000, check_every=25, use_graph=True,
-                 element=None, precision="fp64", inner_rtol=1e-4, stall_iters=0, stream=None):
+                 element=None, precision="fp64", inner_rtol=1e-4, stall_iters=0, lanes=1,
+                 stream=None):
         """systems: list of (plan, values) with values (nnz, 2) float64 on the device.
         element: optional list, one per mesh in the order of `systems`, of
         (assembler, omega, ent_bc, entity_order) of the same systems (the
@@ -346,7 +350,24 @@ class MultiMeshSolver:
             raise ValueError("precision='mixed' needs the element operator (element=...)")
         self.handle = None
         self.eop = None
+        self.lanes = None
         k = len(systems)
+        lanes = max(1, min(int(lanes), k))
+        if lanes > 1:
+            nd = [pl.ndof for pl, _ in systems]
+            load, groups = [0] * lanes, [[] for _ in range(lanes)]
+            for i in sorted(range(k), key=lambda i: -nd[i]):
+                j = int(np.argmin(load))
+                groups[j].append(i)
+                load[j] += nd[i]
+            self.groups = [sorted(g) for g in groups]
+            self.lanes = [MultiMeshSolver([systems[i] for i in g], rtol, maxit, check_every,
+                                          use_graph, None if element is None else
+                                          [element[i] for i in g], precision, inner_rtol,
+                                          stall_iters, 1, stream) for g in self.groups]
+            self.k = k
+            self.n = np.array([pl.ndof for pl, _ in systems], dtype=np.int64)
+            return
         self.n = np.array([pl.ndof for pl, _ in systems], dtype=np.int64)
         nnz = np.array([pl.nnz for pl, _ in systems], dtype=np.int64)
         arr = lambda ts: (ctypes.c_void_p * k)(*[nat.ptr(t) for t in ts])  # noqa: E731
@@ -396,10 +417,42 @@ class MultiMeshSolver:
         nat.check(nat.lib().pg_multi_set_element_operator(self.handle, self.eop,
                                                           nat.stream_handle(stream)))
 
+    def _solve_lanes(self, bs):
+        import threading
+        torch.cuda.synchronize()  # right-hand sides complete
+        res, errors = [None] * len(self.lanes), []
+
+        def work(j):
+            try:
+                res[j] = self.lanes[j].solve([bs[i] for i in self.groups[j]])
+            except Exception as e:  # reported after every lane has stopped
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(j,)) for j in range(len(self.lanes))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        xs = [None] * self.k
+        it = np.zeros(self.k, dtype=np.int32)
+        rel = np.zeros(self.k, dtype=np.float64)
+        st = np.zeros(self.k, dtype=np.int32)
+        launches = 0
+        for g, (xg, ig) in zip(self.groups, res):
+            for a, i in enumerate(g):
+                xs[i] = xg[a]
+                it[i], rel[i], st[i] = ig["iterations"][a], ig["relres"][a], ig["status"][a]
+            launches = max(launches, ig["launches"])
+        return xs, dict(iterations=it, relres=rel, status=st, launches=launches)
+
     def solve(self, bs, stream=None):
         """bs: list of (ndof_i, 2) float64 right-hand sides; returns xs, info."""
         if len(bs) != self.k:
             raise ValueError(f"need {self.k} right-hand sides")
+        if self.lanes is not None:
+            return self._solve_lanes(bs)
         for b, n in zip(bs, self.n):
             if tuple(b.shape) != (int(n), 2) or b.dtype != torch.float64 or not b.is_contiguous():
                 raise ValueError("each b must be a contiguous (ndof, 2) float64 tensor")

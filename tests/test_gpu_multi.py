@@ -236,3 +236,24 @@ def test_multi_mesh_mixed_rounds_and_fp64_hand_over():
         for a, r in zip(xs, ref):
             ah, rh = core.as_complex(a), core.as_complex(r)
             assert np.abs(ah - rh).max() <= 1e-7 * np.abs(rh).max()
+
+
+def test_multi_mesh_lanes_are_bitwise_the_single_batch():
+    """Meshes dealt to several concurrent batches (lanes, own streams) give
+    bitwise the results of one batch over all of them: element products in
+    FP64 and mixed, and the CSR batch."""
+    syss = [_system_e(n, 2, f, seed=80 + i) for i, (n, f) in
+            enumerate([(5, 1.0), (3, 0.25), (6, 0.5), (4, 2.0), (2, 1.0)])]
+    bs = [s["b"] for s in syss]
+    for prec in ("fp64", "mixed"):
+        x1, i1 = _element_batch(syss, rtol=1e-9, precision=prec).solve(bs)
+        x3, i3 = _element_batch(syss, rtol=1e-9, precision=prec, lanes=3).solve(bs)
+        assert (i3["status"] == 0).all() and np.array_equal(i1["iterations"], i3["iterations"])
+        for a, c in zip(x1, x3):
+            assert torch.equal(a, c), f"{prec}: lanes change a mesh's result"
+    pairs = [(s["plan"], s["vals"]) for s in syss]
+    x1, i1 = core.MultiMeshSolver(pairs, rtol=1e-9).solve(bs)
+    x2, i2 = core.MultiMeshSolver(pairs, rtol=1e-9, lanes=2).solve(bs)
+    assert np.array_equal(i1["iterations"], i2["iterations"])
+    for a, c in zip(x1, x2):
+        assert torch.equal(a, c)
