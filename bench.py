@@ -377,43 +377,27 @@ def c3_leg(args, log):
 
 def multimesh_leg(args, log):
     """32 independent meshes, 8^3 .. 24^3 Kuhn hexes (p=2), seeded random layered
-    conductivities (0.01 - 3.3 S/m), one HED each, solved in one multi-mesh batch
-    (HIP-graph replay; COCG per mesh, FP64) with the element products of all
-    meshes in one launch and entity-block Jacobi; beside it the same batch on
-    CSR products and point Jacobi, the latter's systems one after another,
-    and single-RHS solvers (mixed, entity-block Jacobi, element products)
-    four meshes at a time on four streams.  `rhs_per_s` is the faster of the
-    element-operator batch and the four-stream single solves."""
+    conductivities (0.01 - 3.3 S/m), one HED each (synthetic.multimesh_systems),
+    solved as multi-mesh batches (HIP-graph replay; one COCG per mesh) on the
+    meshes' element products with entity-block Jacobi: in four concurrent lanes
+    in FP64 and in complex64 rounds, and as one FP64 batch; beside them the
+    batch on CSR products and point Jacobi, the latter's systems one after
+    another, and single-RHS solvers (mixed, entity-block Jacobi, element
+    products) four meshes at a time on four streams.  `rhs_per_s` is the
+    faster of the fastest batch variant and the four-stream single solves."""
     import torch
     from petgem_amd import core
-    from petgem_amd.synthetic import Workload
+    from petgem_amd.synthetic import multimesh_systems
 
-    sizes = np.round(np.linspace(8, 24, 32)).astype(int)
-    freqs = [0.1, 0.25, 0.5, 1.0]
     systems, bs, fast, elem = [], [], [], []
-    for i, n in enumerate(sizes):
-        w = Workload(int(n), 2, block=False, seed=i)
-        rng = np.random.default_rng(1000 + i)
-        layers = np.exp(rng.uniform(np.log(0.01), np.log(3.3), 3))
-        z = w.topo.coords.mean(axis=1)[:, 2]
-        s = np.where(z > -1000.0, layers[0], np.where(z > -3000.0, layers[1], layers[2]))
-        mesh = core.DeviceMesh.from_topology(w.topo, torch.device("cuda"))
-        asm = core.Assembler(mesh)
-        asm.geometry(np.stack([s, s], axis=1))
-        ent_bc, dof_bc = mesh.entity_mask(w.topo.boundary_dofs)
-        om = core.omega_of(freqs[i % 4])
-        vals = asm.assemble(om, ent_bc=ent_bc)
-        el = int(core.locate_points(mesh, np.array([[0.0, 0.0, -950.0]])).cpu()[0])
-        b = core.source_vector(mesh, el, [0.0, 0.0, -950.0], (1.0, 0.0, 0.0), om)
-        core.apply_dirichlet(asm.plan, None, dof_bc, b)
-        systems.append((asm.plan, vals))
-        bs.append(b)
-        order = mesh.entity_order()
-        elem.append((asm, om, ent_bc, order))
-        eop = core.ElementOperator(asm, om, ent_bc, order)
-        fast.append(core.KrylovSolver(asm.plan, vals, precision="mixed", pc="bjacobi",
-                                      rtol=args.rtol, block_ptr=mesh.block_ptr(),
-                                      entity_order=order, eop=eop))
+    for m in multimesh_systems("cuda"):
+        systems.append((m["plan"], m["vals"]))
+        bs.append(m["b"])
+        elem.append((m["asm"], m["omega"], m["ent_bc"], m["order"]))
+        eop = core.ElementOperator(m["asm"], m["omega"], m["ent_bc"], m["order"])
+        fast.append(core.KrylovSolver(m["plan"], m["vals"], precision="mixed", pc="bjacobi",
+                                      rtol=args.rtol, block_ptr=m["mesh"].block_ptr(),
+                                      entity_order=m["order"], eop=eop))
     ndof = int(sum(pl.ndof for pl, _ in systems))
 
     def timed(ms, name):

@@ -11,9 +11,14 @@ kernel-trace or hardware-counter profiler sees a short, regular program:
   counter readings calibrate the HBM byte counters (bytes read and written
   by a copy are known exactly).
 
+With ``--multimesh`` it instead runs the multi-mesh workload (32 meshes,
+``synthetic.multimesh_systems``) as one FP64 batch on the element products
+with entity-block Jacobi, ``--reps`` solves from one host thread.
+
 Usage::
 
     python -m petgem_amd.kernel_bench --config c2 --reps 20 [--pc jacobi] [--copy-mib 1024]
+    python -m petgem_amd.kernel_bench --multimesh --reps 2
 
 Prints one JSON line with the HIP-event average of each kernel.
 """
@@ -40,7 +45,12 @@ def main(argv=None):
                          "the element operator at 64, 128, 192 or 256 columns")
     ap.add_argument("--batch-iters", type=int, default=50)
     ap.add_argument("--batch-precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--multimesh", action="store_true",
+                    help="only the multi-mesh batch (element products, FP64, one batch)")
     args = ap.parse_args(argv)
+    if args.multimesh:
+        multimesh_run(args.reps)
+        return 0
 
     import torch
     from . import core
@@ -105,6 +115,29 @@ def main(argv=None):
         out.update(batch_run(args.batch_width, args.batch_iters, args.batch_precision))
     print(json.dumps(out), flush=True)
     return 0
+
+
+def multimesh_run(reps):
+    import time
+    import torch
+    from . import core
+    from .synthetic import multimesh_systems
+    ms = multimesh_systems("cuda")
+    batch = core.MultiMeshSolver([(m["plan"], m["vals"]) for m in ms], check_every=25,
+                                 element=[(m["asm"], m["omega"], m["ent_bc"], m["order"])
+                                          for m in ms])
+    out = dict(meshes=len(ms), dofs=int(sum(m["plan"].ndof for m in ms)), solves_ms=[])
+    for _ in range(max(1, reps)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs, info = batch.solve([m["b"] for m in ms])
+        torch.cuda.synchronize()
+        out["solves_ms"].append(round(1e3 * (time.perf_counter() - t0), 1))
+        if (info["status"] != 0).any():
+            raise RuntimeError(f"multi-mesh solve did not converge: {info}")
+    out["iterations"] = info["iterations"].tolist()
+    print(json.dumps(out), flush=True)
+    return out
 
 
 def batch_run(width, iters, precision="fp64"):

@@ -103,3 +103,34 @@ CONFIGS = {
     "c3": dict(n=24, p=3, block=True, noise=0.0, n_rx=101),
     "c4": dict(n=30, p=2, block=True, noise=0.0, n_rx=101),
 }
+
+
+def multimesh_systems(device, rtol=1e-8):
+    """The multi-mesh workload: 32 meshes of 8^3 .. 24^3 Kuhn hexes x 6 (p = 2),
+    seeded random 3-layer conductivities (0.01 - 3.3 S/m, layer tops at -1 and
+    -3 km), an x-directed HED at (0, 0, -950) m in each, 0.1 / 0.25 / 0.5 / 1 Hz
+    cyclic.  Returns per mesh a dict: mesh, assembler (after its geometry),
+    plan, vals (assembled, PEC applied), b, omega, ent_bc, order."""
+    import torch
+    from . import core
+    sizes = np.round(np.linspace(8, 24, 32)).astype(int)
+    freqs = [0.1, 0.25, 0.5, 1.0]
+    out = []
+    for i, n in enumerate(sizes):
+        w = Workload(int(n), 2, block=False, seed=i)
+        rng = np.random.default_rng(1000 + i)
+        layers = np.exp(rng.uniform(np.log(0.01), np.log(3.3), 3))
+        z = w.topo.coords.mean(axis=1)[:, 2]
+        s = np.where(z > -1000.0, layers[0], np.where(z > -3000.0, layers[1], layers[2]))
+        mesh = core.DeviceMesh.from_topology(w.topo, torch.device(device))
+        asm = core.Assembler(mesh)
+        asm.geometry(np.stack([s, s], axis=1))
+        ent_bc, dof_bc = mesh.entity_mask(w.topo.boundary_dofs)
+        om = core.omega_of(freqs[i % 4])
+        vals = asm.assemble(om, ent_bc=ent_bc)
+        el = int(core.locate_points(mesh, np.array([[0.0, 0.0, -950.0]])).cpu()[0])
+        b = core.source_vector(mesh, el, [0.0, 0.0, -950.0], (1.0, 0.0, 0.0), om)
+        core.apply_dirichlet(asm.plan, None, dof_bc, b)
+        out.append(dict(mesh=mesh, asm=asm, plan=asm.plan, vals=vals, b=b, omega=om,
+                        ent_bc=ent_bc, order=mesh.entity_order()))
+    return out
