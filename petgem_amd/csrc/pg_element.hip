@@ -246,7 +246,7 @@ __global__ void k_element_matrices(int64_t E, const double* __restrict__ T,
 }
 
 // -------------------------------------------------------------------------
-// fused assembly: one thread per (entity pair, row k of the pair's block)
+// assembly
 // -------------------------------------------------------------------------
 template <int P>
 __device__ inline int entity_ndof_global(int64_t g, int64_t nE, int64_t nF) {
@@ -255,16 +255,92 @@ __device__ inline int entity_ndof_global(int64_t g, int64_t nE, int64_t nF) {
   return P * (P - 1) * (P - 2) / 2;
 }
 
+// -------------------------------------------------------------------------
+// Two-phase assembly, atomic-free, every CSR slot the sum of its contributors
+// in the plan's ascending element order (the order of the reference's
+// single-rank ADD_VALUES):
+//  k_asm_blocks  one thread per (local entity la, local entity lb, element),
+//                the element fastest: the lanes of a wave take consecutive
+//                elements at the same local pair, so their coefficient loads
+//                are contiguous and their table rows repeat with the few
+//                orientation patterns of neighbouring elements (cache hits
+//                instead of a gather per lane); each writes its d(la) x d(lb)
+//                block of (K_e, -omega mu0 M_e) entries (or (K_e, M_e)):
+//                blk[t][k][l], t = (la nloc + lb) E + e, MAXD x MAXD per t;
+//  k_asm_sums    one thread per (entity pair, row k of its block): sums its
+//                contributors' rows k of those blocks in the plan's ascending
+//                element order and stores the pair block's row.
+// Traffic: the blocks are written and read once (2.46 GB at C2) besides the
+// CSR values, but every load is either a broadcast (an element's coefficients)
+// or contiguous, where one thread per slot row gathering coefficients and
+// table rows per contributor was bound by divergent gathers.
+// -------------------------------------------------------------------------
 template <int P>
-__global__ void __launch_bounds__(256) k_assemble(
-    int64_t npairs, int64_t nE, int64_t nF, const double* __restrict__ T,
-    const int32_t* __restrict__ pair_slot0, const int32_t* __restrict__ pair_rowlen,
-    const int32_t* __restrict__ pair_cbeg, const int32_t* __restrict__ pair_ab,
-    const uint32_t* __restrict__ contrib, const double* __restrict__ coef,
-    const int32_t* __restrict__ orient, double omega_mu, const uint8_t* __restrict__ ent_bc,
-    double2* __restrict__ values, double* __restrict__ Kout, double* __restrict__ Mout) {
-  constexpr int MAXD = P == 1 ? 1 : P * (P - 1);
+constexpr int asm_nloc() {
+  return P == 1 ? 6 : (P == 2 ? 10 : 11);
+}
+template <int P>
+constexpr int asm_maxd() {
+  return P == 1 ? 1 : P * (P - 1);
+}
+template <int P>
+__device__ inline int local_entity_ndof(int la) {
+  return la < 6 ? P : (la < 10 ? P * (P - 1) : P * (P - 1) * (P - 2) / 2);
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_asm_blocks(int64_t E, const double* __restrict__ T,
+                                                    const double* __restrict__ coef,
+                                                    const int32_t* __restrict__ orient,
+                                                    double omega_mu, int km,
+                                                    double2* __restrict__ blk) {
+  constexpr int NL = asm_nloc<P>(), MAXD = asm_maxd<P>();
   constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
+  // p = 2: the workgroup's blocks (4 values each) are staged in LDS and written
+  // as one contiguous run, instead of four stores 64 bytes apart per lane
+  constexpr bool kStage = P == 2;
+  __shared__ double2 stage[kStage ? 256 * MAXD * MAXD : 1];
+  // grid: x over elements, y = the local pair ll = la nloc + lb (no divisions)
+  const int ll = blockIdx.y, la = ll / NL, lb = ll % NL;
+  const int64_t e0 = (int64_t)blockIdx.x * blockDim.x, e = e0 + threadIdx.x;
+  const int64_t t = (int64_t)ll * E + e;
+  if (!kStage && e >= E) return;
+  if (e < E) {
+    const int dA = local_entity_ndof<P>(la), dB = local_entity_ndof<P>(lb);
+    const int pk = orient[e];
+    const double* cf = coef + 12 * e;
+    const int va0 = entity_variant_base(la, orient_of(pk, la), P);
+    const int vb = entity_variant_base(lb, orient_of(pk, lb), P);
+    double2* out = kStage ? stage + threadIdx.x * (MAXD * MAXD) : blk + t * (MAXD * MAXD);
+    for (int k = 0; k < dA; ++k) {
+      const double* trow = T + ((size_t)(va0 + k) * V + vb) * 12;
+#pragma unroll
+      for (int l = 0; l < MAXD; ++l)
+        if (l < dB) {
+          double m, kk;
+          contract(trow + 12 * l, cf, &m, &kk);
+          // separately rounded product, as the reference forms i*w*mu0*M_e before adding
+          out[k * MAXD + l] = make_double2(kk, km ? m : -__dmul_rn(omega_mu, m));
+        }
+    }
+  }
+  if constexpr (kStage) {
+    __syncthreads();
+    const int64_t t0 = (int64_t)ll * E + e0;
+    const int64_t n = (E - e0 < (int64_t)blockDim.x ? E - e0 : (int64_t)blockDim.x) *
+                      (MAXD * MAXD);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) blk[t0 * (MAXD * MAXD) + i] = stage[i];
+  }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_asm_sums(
+    int64_t npairs, int64_t nE, int64_t nF, int64_t E, const int32_t* __restrict__ pair_slot0,
+    const int32_t* __restrict__ pair_rowlen, const int32_t* __restrict__ pair_cbeg,
+    const int32_t* __restrict__ pair_ab, const uint32_t* __restrict__ contrib,
+    const double2* __restrict__ blk, const uint8_t* __restrict__ ent_bc,
+    double2* __restrict__ values, double* __restrict__ Kout, double* __restrict__ Mout) {
+  constexpr int NL = asm_nloc<P>(), MAXD = asm_maxd<P>();
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t pair = t / MAXD;
   const int k = (int)(t - pair * MAXD);
@@ -281,21 +357,14 @@ __global__ void __launch_bounds__(256) k_assemble(
     const uint32_t u = contrib[c];
     const int64_t e = u >> 8;
     const int la = (u >> 4) & 15, lb = u & 15;
-    const int pk = orient[e];
-    const double* cf = coef + 12 * e;
-    const int va = entity_variant_base(la, orient_of(pk, la), P) + k;
-    const int vb = entity_variant_base(lb, orient_of(pk, lb), P);
-    const double* trow = T + ((size_t)va * V + vb) * 12;
+    const double2* row = blk + ((int64_t)(la * NL + lb) * E + e) * (MAXD * MAXD) + k * MAXD;
 #pragma unroll
-    for (int l = 0; l < MAXD; ++l) {
+    for (int l = 0; l < MAXD; ++l)
       if (l < dB) {
-        double m, kk;
-        contract(trow + 12 * l, cf, &m, &kk);
-        re[l] += kk;
-        // separately rounded product, as the reference forms i*w*mu0*M_e before adding
-        im[l] += Kout ? m : -__dmul_rn(omega_mu, m);
+        const double2 v = row[l];
+        re[l] += v.x;
+        im[l] += v.y;
       }
-    }
   }
   if (ent_bc != nullptr && (ent_bc[A] | ent_bc[B])) {
 #pragma unroll
@@ -730,16 +799,26 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
   if (pl->p != t->p) PG_FAIL(PG_ERR_ARG, "plan order %d != tables order %d", pl->p, t->p);
   if (pl->n_elems >= (1LL << 24)) PG_FAIL(PG_ERR_UNSUPPORTED, "more than 2^24 elements");
   const int maxd = t->p == 1 ? 1 : t->p * (t->p - 1);
+  const int nloc = t->p == 1 ? 6 : (t->p == 2 ? 10 : 11);
   const int64_t threads = pl->npairs * maxd;
+  const int64_t nblk = pl->n_elems * nloc * nloc;
   void* w = const_cast<void*>(ws);
   const double om = omega * kMu0;
-  const dim3 g(nblocks(threads, 256)), b(256);
+  const dim3 g(nblocks(threads, 256)), gb(nblocks(pl->n_elems, 256), nloc * nloc), b(256);
   hipStream_t s = as_stream(stream);
   double2* v = reinterpret_cast<double2*>(values);
-#define PG_ASM(P)                                                                          \
-  hipLaunchKernelGGL(k_assemble<P>, g, b, 0, s, pl->npairs, pl->n_edges, pl->n_faces, t->d_T, \
+  // the element blocks: stream-ordered scratch (the pool keeps it between calls)
+  double2* blk = nullptr;
+  PG_HIP(hipMallocAsync(reinterpret_cast<void**>(&blk),
+                        (size_t)nblk * maxd * maxd * sizeof(double2), s));
+  const int km = Kout ? 1 : 0;
+#define PG_ASM(P)                                                                             \
+  hipLaunchKernelGGL(k_asm_blocks<P>, gb, b, 0, s, pl->n_elems, t->d_T, ws_coef(w),             \
+                     ws_orient(w, pl->n_elems), om, km, blk);                                  \
+  hipLaunchKernelGGL(k_asm_sums<P>, g, b, 0, s, pl->npairs, pl->n_edges, pl->n_faces,           \
+                     pl->n_elems,                                                              \
                      pl->pair_slot0, pl->pair_rowlen, pl->pair_cbeg, pl->pair_ab, pl->contrib, \
-                     ws_coef(w), ws_orient(w, pl->n_elems), om, ent_bc, v, Kout, Mout)
+                     blk, ent_bc, v, Kout, Mout)
   switch (t->p) {
     case 1: PG_ASM(1); break;
     case 2: PG_ASM(2); break;
@@ -747,6 +826,7 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
   }
 #undef PG_ASM
   PG_CHECK_LAUNCH();
+  PG_HIP(hipFreeAsync(blk, s));
   return PG_OK;
 }
 
