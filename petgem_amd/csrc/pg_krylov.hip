@@ -847,42 +847,69 @@ __global__ void k_set_stall(Scal* s, int limit) {
 // GMRES(m) kernels (complex128, Hermitian inner products)
 // ---------------------------------------------------------------------------
 // partials of V_i^H w for i < k: part[(2i+c) * grid + block]
+// dots of w with V_0 .. V_{k-1} (bilinear: conj(V_i)^T w), eight vectors per
+// sweep over w; with `norm`, also the partials of |w|^2 (slot 2k).  A set
+// `skip` flag (the second Gram-Schmidt pass found unnecessary) returns at once.
+constexpr int kDotGroup = 8;
 __global__ void __launch_bounds__(kBS) k_mdot(int64_t n, const double2* __restrict__ V,
                                               int64_t ldv, int k, const double2* __restrict__ w,
-                                              double* part) {
+                                              double* part, int norm,
+                                              const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
   int64_t lo, hi;
   chunk(n, kBS, &lo, &hi);
   const int64_t G = gridDim.x;
-  for (int i = 0; i < k; ++i) {
-    double red[2] = {0.0, 0.0};
-    const double2* Vi = V + (int64_t)i * ldv;
+  for (int g0 = 0; g0 < k; g0 += kDotGroup) {
+    const int gk = k - g0 < kDotGroup ? k - g0 : kDotGroup;
+    double red[2 * kDotGroup + 1];
+#pragma unroll
+    for (int q = 0; q < 2 * kDotGroup + 1; ++q) red[q] = 0.0;
     for (int64_t j = lo + threadIdx.x; j < hi; j += kBS) {
-      const double2 v = Vi[j], ww = w[j];
-      red[0] += v.x * ww.x + v.y * ww.y;
-      red[1] += v.x * ww.y - v.y * ww.x;
+      const double2 ww = w[j];
+#pragma unroll
+      for (int q = 0; q < kDotGroup; ++q)
+        if (q < gk) {
+          const double2 v = V[(int64_t)(g0 + q) * ldv + j];
+          red[2 * q] += v.x * ww.x + v.y * ww.y;
+          red[2 * q + 1] += v.x * ww.y - v.y * ww.x;
+        }
+      if (norm && g0 == 0) red[2 * kDotGroup] += ww.x * ww.x + ww.y * ww.y;
     }
-    block_reduce<2>(red);
+    block_reduce<2 * kDotGroup + 1>(red);
     if (threadIdx.x == 0) {
-      part[(2 * i) * G + blockIdx.x] = red[0];
-      part[(2 * i + 1) * G + blockIdx.x] = red[1];
+#pragma unroll
+      for (int q = 0; q < kDotGroup; ++q)  // unrolled: red stays in registers
+        if (q < gk) {
+          part[(int64_t)(2 * (g0 + q)) * G + blockIdx.x] = red[2 * q];
+          part[(int64_t)(2 * (g0 + q) + 1) * G + blockIdx.x] = red[2 * q + 1];
+        }
+      if (norm && g0 == 0) part[(int64_t)(2 * k) * G + blockIdx.x] = red[2 * kDotGroup];
     }
   }
 }
 
-// one workgroup per vector i: h[i] = sum of its partials (fixed order)
+// one workgroup per vector i: h[i] = sum of its partials (fixed order); zero
+// when the pass is skipped
 __global__ void __launch_bounds__(kBS) k_fin_mdot(const double* part, int np,
-                                                  double2* __restrict__ h) {
+                                                  double2* __restrict__ h,
+                                                  const int32_t* __restrict__ skip) {
   const int i = blockIdx.x;
+  if (skip && *skip) {
+    if (threadIdx.x == 0) h[i] = make_double2(0.0, 0.0);
+    return;
+  }
   double tot[2];
   sum_partials<2>(part + (int64_t)(2 * i) * np, np, tot);
   if (threadIdx.x == 0) h[i] = make_double2(tot[0], tot[1]);
 }
 
-// w -= sum_i coef[i] V_i ; partials of |w|^2
+// w -= sum_i coef[i] V_i ; partials of |w|^2 (returns at once when skipped)
 __global__ void __launch_bounds__(kBS) k_maxpy_norm(int64_t n, const double2* __restrict__ V,
                                                     int64_t ldv, int k,
                                                     const double2* __restrict__ coef,
-                                                    double2* __restrict__ w, double* part) {
+                                                    double2* __restrict__ w, double* part,
+                                                    const int32_t* __restrict__ skip) {
+  if (skip && *skip) return;
   int64_t lo, hi;
   chunk(n, kBS, &lo, &hi);
   double red[1] = {0.0};
@@ -897,6 +924,16 @@ __global__ void __launch_bounds__(kBS) k_maxpy_norm(int64_t n, const double2* __
     red[0] += acc.x * acc.x + acc.y * acc.y;
   }
   write_partials<1>(red, part);
+}
+
+// the second Gram-Schmidt pass is needed when the first left less than
+// 1/sqrt(2) of |w| (the Daniel-Gragg-Kaufman-Stewart test): skip otherwise
+__global__ void __launch_bounds__(kBS) k_reorth_test(const double* before, const double* after,
+                                                     int np, int32_t* skip) {
+  double b[1], a[1];
+  sum_partials<1>(before, np, b);
+  sum_partials<1>(after, np, a);
+  if (threadIdx.x == 0) *skip = (a[0] > 0.5 * b[0]) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(kBS) k_fin_sum1(const double* part, int np, double* out) {
@@ -1061,6 +1098,7 @@ struct Solver {
   double2* hhost = nullptr;  // pinned
   double* partg = nullptr;
   double* dnorm = nullptr;
+  int32_t* skip = nullptr;  // GMRES: second Gram-Schmidt pass not needed
   int restart = 30;
   // SpMV sampling inside the solve: before, between the element operator's two
   // kernels, after
@@ -1363,7 +1401,7 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
   // |b|^2: w0 = b, no projection
   PG_HIP(hipMemcpyAsync(S->V, b, n * sizeof(double2), hipMemcpyDeviceToDevice, st));
   hipLaunchKernelGGL(k_maxpy_norm, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, 0, S->hdev, S->V,
-                     S->part);
+                     S->part, nullptr);
   hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(kBS), 0, st, S->part, S->gv, &S->s->bnorm2);
   PG_TRY(true_residual(S, st, b, x));
   PG_TRY(read_scalars(S, st));
@@ -1394,12 +1432,19 @@ static int gmres_solve(Solver* S, const double2* b, double2* x, pg_solve_info* i
       double2* w = S->V + (int64_t)(j + 1) * n;
       apply_pc(S, st, Vj, S->z);
       spmv<0, double2, double2>(S, S->c64, st, S->vals, S->z, w, nullptr);
+      // classical Gram-Schmidt; the second pass only when the first lost more
+      // than half of |w|^2 (decided on the device, no host round trip)
       for (int pass = 0; pass < 2; ++pass) {
         double2* hcol = S->hdev + pass * (m + 1);
-        hipLaunchKernelGGL(k_mdot, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, j + 1, w, S->partg);
-        hipLaunchKernelGGL(k_fin_mdot, dim3(j + 1), dim3(kBS), 0, st, S->partg, S->gv, hcol);
+        const int32_t* skip = pass ? S->skip : nullptr;
+        hipLaunchKernelGGL(k_mdot, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, j + 1, w, S->partg,
+                           pass == 0 ? 1 : 0, skip);
+        hipLaunchKernelGGL(k_fin_mdot, dim3(j + 1), dim3(kBS), 0, st, S->partg, S->gv, hcol, skip);
         hipLaunchKernelGGL(k_maxpy_norm, dim3(S->gv), dim3(kBS), 0, st, n, S->V, n, j + 1, hcol,
-                           w, S->part);
+                           w, S->part, skip);
+        if (pass == 0)
+          hipLaunchKernelGGL(k_reorth_test, dim3(1), dim3(kBS), 0, st,
+                             S->partg + (int64_t)(2 * (j + 1)) * S->gv, S->part, S->gv, S->skip);
       }
       hipLaunchKernelGGL(k_fin_sum1, dim3(1), dim3(kBS), 0, st, S->part, S->gv, S->dnorm);
       PG_CHECK_LAUNCH();
@@ -1791,6 +1836,7 @@ int pg_solver_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_
   PG_A(S->q, n);
   PG_A(S->bad, 1);
   PG_A(S->dnorm, 1);
+  PG_A(S->skip, 1);
   // FP64 recursion vectors (also in mixed precision: the fallback when a
   // complex64 round stagnates)
   PG_A(S->r, n);
