@@ -259,12 +259,14 @@ __device__ inline int entity_ndof_global(int64_t g, int64_t nE, int64_t nF) {
 // Two-phase assembly, atomic-free, every CSR slot the sum of its contributors
 // in the plan's ascending element order (the order of the reference's
 // single-rank ADD_VALUES):
-//  k_asm_blocks  one thread per (local entity la, local entity lb, element),
-//                the element fastest: the lanes of a wave take consecutive
-//                elements at the same local pair, so their coefficient loads
-//                are contiguous and their table rows repeat with the few
-//                orientation patterns of neighbouring elements (cache hits
-//                instead of a gather per lane); each writes its d(la) x d(lb)
+//  k_asm_blocks  one thread per (local entity la, local entity lb, element):
+//                the lanes of a wave take consecutive elements at the same
+//                local pair, so their coefficient loads are contiguous and
+//                their table rows repeat with the few orientation patterns of
+//                neighbouring elements (cache hits instead of a gather per
+//                lane), and the nloc^2 workgroups of one block of elements run
+//                one after another (its coefficients read once from L2); each
+//                writes its d(la) x d(lb)
 //                block of (K_e, -omega mu0 M_e) entries (or (K_e, M_e)):
 //                blk[t][k][l], t = (la nloc + lb) E + e, MAXD x MAXD per t;
 //  k_asm_sums    one thread per (entity pair, row k of its block): sums its
@@ -275,6 +277,13 @@ __device__ inline int entity_ndof_global(int64_t g, int64_t nE, int64_t nF) {
 // or contiguous, where one thread per slot row gathering coefficients and
 // table rows per contributor was bound by divergent gathers.
 // -------------------------------------------------------------------------
+// streaming (non-temporal) stores of the block scratch: written once, read
+// once by the sums, so it should not evict the tables and coefficients from L2
+typedef double asm_d2 __attribute__((ext_vector_type(2)));
+__device__ inline void nt_store(double2* p, double2 v) {
+  __builtin_nontemporal_store(asm_d2{v.x, v.y}, reinterpret_cast<asm_d2*>(p));
+}
+
 template <int P>
 constexpr int asm_nloc() {
   return P == 1 ? 6 : (P == 2 ? 10 : 11);
@@ -300,9 +309,12 @@ __global__ void __launch_bounds__(256) k_asm_blocks(int64_t E, const double* __r
   // as one contiguous run, instead of four stores 64 bytes apart per lane
   constexpr bool kStage = P == 2;
   __shared__ double2 stage[kStage ? 256 * MAXD * MAXD : 1];
-  // grid: x over elements, y = the local pair ll = la nloc + lb (no divisions)
-  const int ll = blockIdx.y, la = ll / NL, lb = ll % NL;
-  const int64_t e0 = (int64_t)blockIdx.x * blockDim.x, e = e0 + threadIdx.x;
+  // workgroup b: local pair ll = b mod nloc^2 of the element block b / nloc^2,
+  // so that the nloc^2 workgroups of one block of 256 elements run together
+  // and read its coefficients from L2 (the pair as the slow index re-read all
+  // coefficients once per pair)
+  const int ll = (int)(blockIdx.x % (NL * NL)), la = ll / NL, lb = ll % NL;
+  const int64_t e0 = (int64_t)(blockIdx.x / (NL * NL)) * blockDim.x, e = e0 + threadIdx.x;
   const int64_t t = (int64_t)ll * E + e;
   if (!kStage && e >= E) return;
   if (e < E) {
@@ -320,7 +332,9 @@ __global__ void __launch_bounds__(256) k_asm_blocks(int64_t E, const double* __r
           double m, kk;
           contract(trow + 12 * l, cf, &m, &kk);
           // separately rounded product, as the reference forms i*w*mu0*M_e before adding
-          out[k * MAXD + l] = make_double2(kk, km ? m : -__dmul_rn(omega_mu, m));
+          const double2 v = make_double2(kk, km ? m : -__dmul_rn(omega_mu, m));
+          if constexpr (kStage) out[k * MAXD + l] = v;
+          else nt_store(out + k * MAXD + l, v);
         }
     }
   }
@@ -329,7 +343,7 @@ __global__ void __launch_bounds__(256) k_asm_blocks(int64_t E, const double* __r
     const int64_t t0 = (int64_t)ll * E + e0;
     const int64_t n = (E - e0 < (int64_t)blockDim.x ? E - e0 : (int64_t)blockDim.x) *
                       (MAXD * MAXD);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) blk[t0 * (MAXD * MAXD) + i] = stage[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) nt_store(blk + t0 * (MAXD * MAXD) + i, stage[i]);
   }
 }
 
@@ -804,7 +818,7 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
   const int64_t nblk = pl->n_elems * nloc * nloc;
   void* w = const_cast<void*>(ws);
   const double om = omega * kMu0;
-  const dim3 g(nblocks(threads, 256)), gb(nblocks(pl->n_elems, 256), nloc * nloc), b(256);
+  const dim3 g(nblocks(threads, 256)), gb(nblocks(pl->n_elems, 256) * nloc * nloc), b(256);
   hipStream_t s = as_stream(stream);
   double2* v = reinterpret_cast<double2*>(values);
   // the element blocks: stream-ordered scratch (the pool keeps it between calls)
