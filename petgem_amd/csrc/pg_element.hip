@@ -256,9 +256,80 @@ __device__ inline int entity_ndof_global(int64_t g, int64_t nE, int64_t nF) {
 }
 
 // -------------------------------------------------------------------------
-// Two-phase assembly, atomic-free, every CSR slot the sum of its contributors
-// in the plan's ascending element order (the order of the reference's
-// single-rank ADD_VALUES):
+// One-pass assembly (used at p = 3): one thread per (entity pair, block row)
+// loops over the pair's contributors in ascending element order, contracting
+// each entry from the tables and the element's coefficients.  At p = 3 the
+// two-phase form below would pad every local block to 6 x 6 (a 5.8 GB scratch
+// at C3) and loses to this form.
+// -------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(256) k_assemble(
+    int64_t npairs, int64_t nE, int64_t nF, const double* __restrict__ T,
+    const int32_t* __restrict__ pair_slot0, const int32_t* __restrict__ pair_rowlen,
+    const int32_t* __restrict__ pair_cbeg, const int32_t* __restrict__ pair_ab,
+    const uint32_t* __restrict__ contrib, const double* __restrict__ coef,
+    const int32_t* __restrict__ orient, double omega_mu, const uint8_t* __restrict__ ent_bc,
+    double2* __restrict__ values, double* __restrict__ Kout, double* __restrict__ Mout) {
+  constexpr int MAXD = P == 1 ? 1 : P * (P - 1);
+  constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t pair = t / MAXD;
+  const int k = (int)(t - pair * MAXD);
+  if (pair >= npairs) return;
+  const int64_t A = pair_ab[2 * pair], B = pair_ab[2 * pair + 1];
+  const int dA = entity_ndof_global<P>(A, nE, nF);
+  const int dB = entity_ndof_global<P>(B, nE, nF);
+  if (k >= dA) return;
+  double re[MAXD], im[MAXD];
+#pragma unroll
+  for (int l = 0; l < MAXD; ++l) re[l] = im[l] = 0.0;
+  const int c0 = pair_cbeg[pair], c1 = pair_cbeg[pair + 1];
+  for (int c = c0; c < c1; ++c) {
+    const uint32_t u = contrib[c];
+    const int64_t e = u >> 8;
+    const int la = (u >> 4) & 15, lb = u & 15;
+    const int pk = orient[e];
+    const double* cf = coef + 12 * e;
+    const int va = entity_variant_base(la, orient_of(pk, la), P) + k;
+    const int vb = entity_variant_base(lb, orient_of(pk, lb), P);
+    const double* trow = T + ((size_t)va * V + vb) * 12;
+#pragma unroll
+    for (int l = 0; l < MAXD; ++l) {
+      if (l < dB) {
+        double m, kk;
+        contract(trow + 12 * l, cf, &m, &kk);
+        re[l] += kk;
+        // separately rounded product, as the reference forms i*w*mu0*M_e before adding
+        im[l] += Kout ? m : -__dmul_rn(omega_mu, m);
+      }
+    }
+  }
+  if (ent_bc != nullptr && (ent_bc[A] | ent_bc[B])) {
+#pragma unroll
+    for (int l = 0; l < MAXD; ++l) {
+      re[l] = (A == B && k == l && ent_bc[A]) ? 1.0 : 0.0;
+      im[l] = 0.0;
+    }
+  }
+  const int64_t s0 = (int64_t)pair_slot0[pair] + (int64_t)k * pair_rowlen[pair];
+  if (Kout) {
+#pragma unroll
+    for (int l = 0; l < MAXD; ++l)
+      if (l < dB) {
+        Kout[s0 + l] = re[l];
+        Mout[s0 + l] = im[l];
+      }
+  } else {
+#pragma unroll
+    for (int l = 0; l < MAXD; ++l)
+      if (l < dB) values[s0 + l] = make_double2(re[l], im[l]);
+  }
+}
+
+// -------------------------------------------------------------------------
+// Two-phase assembly (p = 1, 2), atomic-free, every CSR slot the sum of its
+// contributors in the plan's ascending element order (the order of the
+// reference's single-rank ADD_VALUES), bitwise the one-pass kernel's result:
 //  k_asm_blocks  one thread per (local entity la, local entity lb, element):
 //                the lanes of a wave take consecutive elements at the same
 //                local pair, so their coefficient loads are contiguous and
@@ -821,6 +892,13 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
   const dim3 g(nblocks(threads, 256)), gb(nblocks(pl->n_elems, 256) * nloc * nloc), b(256);
   hipStream_t s = as_stream(stream);
   double2* v = reinterpret_cast<double2*>(values);
+  if (t->p == 3) {
+    hipLaunchKernelGGL(k_assemble<3>, g, b, 0, s, pl->npairs, pl->n_edges, pl->n_faces, t->d_T,
+                       pl->pair_slot0, pl->pair_rowlen, pl->pair_cbeg, pl->pair_ab, pl->contrib,
+                       ws_coef(w), ws_orient(w, pl->n_elems), om, ent_bc, v, Kout, Mout);
+    PG_CHECK_LAUNCH();
+    return PG_OK;
+  }
   // the element blocks: stream-ordered scratch (the pool keeps it between calls)
   double2* blk = nullptr;
   PG_HIP(hipMallocAsync(reinterpret_cast<void**>(&blk),
@@ -833,10 +911,10 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
                      pl->n_elems,                                                              \
                      pl->pair_slot0, pl->pair_rowlen, pl->pair_cbeg, pl->pair_ab, pl->contrib, \
                      blk, ent_bc, v, Kout, Mout)
-  switch (t->p) {
-    case 1: PG_ASM(1); break;
-    case 2: PG_ASM(2); break;
-    case 3: PG_ASM(3); break;
+  if (t->p == 1) {
+    PG_ASM(1);
+  } else {
+    PG_ASM(2);
   }
 #undef PG_ASM
   PG_CHECK_LAUNCH();
