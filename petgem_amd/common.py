@@ -125,7 +125,7 @@ class SolverOptions:
         self.max_it = 100000
         self.restart = 30
         self.precision = "fp64"
-        self.inner_rtol = 1e-4
+        self.inner_rtol = None  # complex64 round tolerance: by basis order (core.mixed_defaults)
         self.spmv = "auto"
         self.notes = []
         flags = []
