@@ -203,11 +203,12 @@ def batched_leg(args, log):
         with each element's entries formed once for all columns (the element
         operator in the batch).  Every right-hand side is solved and timed,
         K and M assembled once per batch.
-    (b) For comparison, the single-RHS solver (mixed precision, entity-block
-        Jacobi, element products) two right-hand sides at a time on two
-        streams, timed on --single-positions of the positions x 4 frequencies
-        (the rate of independent solves).
-    `rhs_per_s` is the batched solver's rate on all right-hand sides."""
+    (b) The single-RHS solver (mixed precision, entity-block Jacobi, element
+        products) two right-hand sides at a time on two streams
+        (core.solve_concurrently), on --single-positions of the positions x 4
+        frequencies (default: all 256 right-hand sides).
+    `rhs_per_s` is the faster of the two over the same right-hand sides
+    (`mode` names it); both rates are reported."""
     import torch
     from petgem_amd import core
     from petgem_amd.synthetic import CONFIGS, Workload
@@ -295,19 +296,24 @@ def batched_leg(args, log):
         del Ss, eops
     log(f"single-RHS solves two at a time (mixed, block Jacobi, element products): {nseq} RHS in "
         f"{seq_t:.2f} s ({nseq / seq_t:.2f} RHS/s)")
-    return {"rhs_per_s": round(ncols / t, 3), "mode": "batched_solver",
+    r_batch, r_single = ncols / t, nseq / seq_t
+    best = "batched_solver" if r_batch >= r_single else "single_rhs_two_streams"
+    return {"rhs_per_s": round(max(r_batch, r_single), 3), "mode": best,
+            "batched_solver_rhs_per_s": round(r_batch, 3),
             "s_total": round(t, 3), "nrhs": ncols, "width": width,
             "precision": args.batch_precision, "element_products": use_eop,
             "batch_iterations": its, "worst_relres": worst,
             "single_rhs_two_streams_rhs_per_s": round(nseq / seq_t, 3),
             "single_rhs_sample": int(nseq),
-            "speedup_vs_single_rhs_two_streams": round((ncols / t) / (nseq / seq_t), 3),
+            "batched_vs_single_rhs_two_streams": round((ncols / t) / (nseq / seq_t), 3),
             "workload": (f"c4: p=2, 30^3 Kuhn x 6 = {T.n_elems} tets, {T.ndof} DOFs, "
                          f"{asm.plan.nnz} nnz; {len(xs)} HED x-positions (100 m apart at "
                          f"z=-950 m) x {freqs} Hz = {ncols} RHS, rtol {args.rtol}; batches of "
                          f"{width} columns of one frequency, COCG + entity-block Jacobi per "
                          f"column ({args.batch_precision}), element products; K/M assembled "
-                         "once")}
+                         "once; single_rhs_two_streams = the single-RHS solver (mixed, "
+                         "entity-block Jacobi, element products) two at a time on two "
+                         f"streams over {nseq} of the right-hand sides")}
 
 
 def c3_leg(args, log):
@@ -567,8 +573,8 @@ def main():
                     help="batched and adjoint legs: FP64, or complex64 under FP64 refinement")
     ap.add_argument("--batch-positions", type=int, default=64,
                     help="HED positions of the batched leg (of 64; x 4 frequencies)")
-    ap.add_argument("--single-positions", type=int, default=16,
-                    help="positions (x 4 frequencies) of the single-RHS comparison")
+    ap.add_argument("--single-positions", type=int, default=64,
+                    help="positions (x 4 frequencies) of the single-RHS solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the multi-GPU sharding of the "

@@ -589,13 +589,14 @@ class ElementOperator:
 
 
 def mixed_defaults(p):
-    """(inner_rtol, stall_iters) of the complex64 rounds by basis order: at p = 3
-    the 45-DOF complex64 element products carry more rounding, so a round
-    aims for 1e-3 and is watched 800 iterations for stagnation (measured on the
-    p = 3 workload: 0.98 -> 0.81 s at 1 Hz, 2.59 -> 1.69 s at 0.25 Hz, where
-    the rounds then reach their tenfold gain and need no FP64 hand-over);
-    p <= 2 keeps 1e-4 and the solver's 400 (best at p = 2)."""
-    return (1e-3, 800) if int(p) >= 3 else (1e-4, 0)
+    """(inner_rtol, stall_iters) of the single-RHS solver's complex64 rounds by
+    basis order.  A round is watched 800 iterations for stagnation (at 400 the
+    low-frequency rounds were cut before their tenfold gain and handed over to
+    FP64: C4 at 0.1 Hz 0.505 -> 0.369 s per right-hand side, p = 3 at
+    0.25 Hz 2.59 -> 1.69 s; no change at 1 Hz).  At p = 3 the 45-DOF complex64
+    element products carry more rounding, so a round aims for 1e-3 (0.98 ->
+    0.81 s at 1 Hz); p <= 2 keeps 1e-4 (best at p = 2)."""
+    return (1e-3, 800) if int(p) >= 3 else (1e-4, 800)
 
 
 class KrylovSolver:
