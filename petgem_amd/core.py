@@ -336,14 +336,14 @@ class MultiMeshSolver:
     not depend on the other meshes of its batch, so it is bitwise the same."""
 
     def __init__(self, systems, rtol=1e-8, maxit=100000, check_every=25, use_graph=True,
-                 element=None, precision="fp64", inner_rtol=1e-4, stall_iters=0, lanes=1,
+                 element=None, precision="fp64", inner_rtol=1e-4, stall_iters=800, lanes=1,
                  stream=None):
         """systems: list of (plan, values) with values (nnz, 2) float64 on the device.
         element: optional list, one per mesh in the order of `systems`, of
         (assembler, omega, ent_bc, entity_order) of the same systems (the
         assembler after its geometry; ent_bc and entity_order may be None).
         precision "mixed" (with `element`): complex64 rounds under FP64
-        refreshes per mesh (inner_rtol, stall_iters as in KrylovSolver)."""
+        refreshes per mesh (inner_rtol, stall_iters as in KrylovSolver at p <= 2)."""
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         if precision == "mixed" and element is None:
