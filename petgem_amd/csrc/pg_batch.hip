@@ -425,16 +425,23 @@ __global__ void __launch_bounds__(kBS) k_fin_mu(const Ctl* ctl, Col* col, int R,
   }
 }
 
-// x += alpha p; r -= alpha q; z = r / diag_r; partials of rho = r^T z and |r|^2
+// z = r / diag_r, as every kernel that forms the preconditioned residual does
+template <typename VT>
+__device__ inline VT jacobi_z(VT rv, double kd, double md, double om) {
+  return from_d<VT>(zdiv(to_d(rv), make_double2(kd, -om * md)));
+}
+
+// r -= alpha q; partials of rho = r^T z and |r|^2 with z = r / diag_r.  z is
+// not stored: k_pupdate forms it again from r (the same operations, so the
+// same values), and takes this iteration's step x += alpha p there, where p is
+// read anyway (or k_bxstep takes it once the batch pauses).
 template <int CT, int TMAX, typename VT>
 __global__ void __launch_bounds__(kBS) k_update(const Ctl* ctl, const Col* col, int64_t n, int R,
                                                 const double* __restrict__ dK,
                                                 const double* __restrict__ dM,
                                                 const double* __restrict__ omu,
-                                                VT* __restrict__ X, VT* __restrict__ Rr,
-                                                const VT* __restrict__ P,
-                                                const VT* __restrict__ Q,
-                                                VT* __restrict__ Z, double* part) {
+                                                VT* __restrict__ Rr, const VT* __restrict__ Q,
+                                                double* part) {
   if (ctl->all_done) return;
   constexpr int RPB = kBS / CT;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
@@ -456,18 +463,13 @@ __global__ void __launch_bounds__(kBS) k_update(const Ctl* ctl, const Col* col, 
     for (int t = 0; t < TMAX; ++t)
       if (t < T) {
         const int64_t idx = row * R + c + t * CT;
-        const VT pv = P[idx], qv = Q[idx];
-        VT xv = X[idx], rv = Rr[idx];
-        xv.x += al[t].x * pv.x - al[t].y * pv.y;
-        xv.y += al[t].x * pv.y + al[t].y * pv.x;
+        const VT qv = Q[idx];
+        VT rv = Rr[idx];
         rv.x -= al[t].x * qv.x - al[t].y * qv.y;
         rv.y -= al[t].x * qv.y + al[t].y * qv.x;
-        X[idx] = xv;
         Rr[idx] = rv;
         const double2 rd = to_d(rv);
-        const VT zv = from_d<VT>(zdiv(rd, make_double2(kd, -om[t] * md)));
-        Z[idx] = zv;
-        const double2 zd = to_d(zv);
+        const double2 zd = to_d(jacobi_z(rv, kd, md, om[t]));
         v[t][0] += rd.x * zd.x - rd.y * zd.y;
         v[t][1] += rd.x * zd.y + rd.y * zd.x;
         v[t][2] += rd.x * rd.x + rd.y * rd.y;
@@ -523,39 +525,74 @@ __global__ void __launch_bounds__(kBS) k_fin_all(Ctl* ctl, const Col* col, int R
   if (threadIdx.x == 0) ctl->all_done = notdone ? 0 : 1;
 }
 
-// p = z + beta p (done columns keep p; their alpha is 0)
+// x += alpha p, the previous iteration's step (alpha is that iteration's; it is
+// 0 for columns finished before it and after a (re)start, where p = 0 too);
+// then z = r / diag_r and p = z + beta p
+template <typename VT>
+__device__ inline void step_and_direction(VT* X, VT* P, int64_t idx, VT al, VT be, VT zv) {
+  const VT pv = P[idx];
+  VT xv = X[idx];
+  xv.x += al.x * pv.x - al.y * pv.y;
+  xv.y += al.x * pv.y + al.y * pv.x;
+  X[idx] = xv;
+  P[idx] = VT{zv.x + be.x * pv.x - be.y * pv.y, zv.y + be.x * pv.y + be.y * pv.x};
+}
+
 template <int CT, int TMAX, typename VT>
 __global__ void __launch_bounds__(kBS) k_pupdate(const Ctl* ctl, const Col* col, int64_t n, int R,
-                                                 const VT* __restrict__ Z, VT* __restrict__ P) {
+                                                 const double* __restrict__ dK,
+                                                 const double* __restrict__ dM,
+                                                 const double* __restrict__ omu,
+                                                 VT* __restrict__ X, const VT* __restrict__ Rr,
+                                                 VT* __restrict__ P) {
   if (ctl->all_done) return;
   constexpr int RPB = kBS / CT;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
   const int T = R / CT;
   int64_t lo, hi;
   rows_of_block(n, &lo, &hi);
-  VT be[TMAX];
+  VT al[TMAX], be[TMAX];
+  double om[TMAX];
 #pragma unroll
-  for (int t = 0; t < TMAX; ++t)
+  for (int t = 0; t < TMAX; ++t) {
+    al[t] = from_d<VT>(t < T ? col[c + t * CT].alpha : make_double2(0.0, 0.0));
     be[t] = from_d<VT>(t < T ? col[c + t * CT].beta : make_double2(0.0, 0.0));
-  for (int64_t row = lo + rs; row < hi; row += RPB)
+    om[t] = t < T ? omu[c + t * CT] : 0.0;
+  }
+  for (int64_t row = lo + rs; row < hi; row += RPB) {
+    const double kd = dK[row], md = dM[row];
 #pragma unroll
     for (int t = 0; t < TMAX; ++t)
       if (t < T) {
         const int64_t idx = row * R + c + t * CT;
-        const VT pv = P[idx], zv = Z[idx];
-        P[idx] = VT{zv.x + be[t].x * pv.x - be[t].y * pv.y, zv.y + be[t].x * pv.y + be[t].y * pv.x};
+        step_and_direction(X, P, idx, al[t], be[t], jacobi_z(Rr[idx], kd, md, om[t]));
       }
+  }
 }
 
-// init: x = 0, r = b, z = r/diag, p = 0; partials of rho, |r|^2 (= |b|^2)
+// x += alpha p for every column: the step of the last iteration before the
+// batch paused (alpha is 0 for columns that finished earlier)
+template <typename VT>
+__global__ void k_bxstep(const Col* col, int64_t nr, int R, const VT* __restrict__ P,
+                         VT* __restrict__ X) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  const VT al = from_d<VT>(col[i % R].alpha);
+  const VT pv = P[i];
+  VT xv = X[i];
+  xv.x += al.x * pv.x - al.y * pv.y;
+  xv.y += al.x * pv.y + al.y * pv.x;
+  X[i] = xv;
+}
+
+// init: x = 0, r = b, p = 0; partials of rho = r^T z (z = r/diag), |r|^2 (= |b|^2)
 template <int CT, int TMAX>
 __global__ void __launch_bounds__(kBS) k_init(int64_t n, int R, const double* __restrict__ dK,
                                               const double* __restrict__ dM,
                                               const double* __restrict__ omu,
                                               const double2* __restrict__ B,
                                               double2* __restrict__ X, double2* __restrict__ Rr,
-                                              double2* __restrict__ Z, double2* __restrict__ P,
-                                              double* part) {
+                                              double2* __restrict__ P, double* part) {
   constexpr int RPB = kBS / CT;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
   const int T = R / CT;
@@ -577,8 +614,7 @@ __global__ void __launch_bounds__(kBS) k_init(int64_t n, int R, const double* __
         const double2 bv = B[idx];
         X[idx] = make_double2(0.0, 0.0);
         Rr[idx] = bv;
-        const double2 zv = zdiv(bv, make_double2(kd, -om[t] * md));
-        Z[idx] = zv;
+        const double2 zv = jacobi_z(bv, kd, md, om[t]);
         P[idx] = make_double2(0.0, 0.0);
         v[t][0] += bv.x * zv.x - bv.y * zv.y;
         v[t][1] += bv.x * zv.y + bv.y * zv.x;
@@ -654,8 +690,8 @@ __global__ void __launch_bounds__(kBS) k_fin_true(Col* col, int R, const double*
   if (threadIdx.x == 0) col[blockIdx.x].res2 = tot[0];
 }
 
-// restart from the current iterate: r = b - A x, z = r / diag, p = 0;
-// partials of rho = r^T z and |r|^2 per column
+// restart from the current iterate: r = b - A x, p = 0; partials of
+// rho = r^T z (z = r / diag) and |r|^2 per column
 template <int CT, int TMAX>
 __global__ void __launch_bounds__(kBS) k_reinit(int64_t n, int R,
                                                 const int32_t* __restrict__ rowptr,
@@ -667,7 +703,7 @@ __global__ void __launch_bounds__(kBS) k_reinit(int64_t n, int R,
                                                 const double* __restrict__ omu,
                                                 const double2* __restrict__ X,
                                                 const double2* __restrict__ B,
-                                                double2* __restrict__ Rr, double2* __restrict__ Z,
+                                                double2* __restrict__ Rr,
                                                 double2* __restrict__ P, double* part) {
   constexpr int RPB = kBS / CT;
   const int c = threadIdx.x % CT, rs = threadIdx.x / CT;
@@ -704,8 +740,7 @@ __global__ void __launch_bounds__(kBS) k_reinit(int64_t n, int R,
         const double2 bv = B[idx];
         const double2 rv = make_double2(bv.x - acc[t].x, bv.y - acc[t].y);
         Rr[idx] = rv;
-        const double2 zv = zdiv(rv, make_double2(kd, -om[t] * md));
-        Z[idx] = zv;
+        const double2 zv = jacobi_z(rv, kd, md, om[t]);
         P[idx] = make_double2(0.0, 0.0);
         v[t][0] += rv.x * zv.x - rv.y * zv.y;
         v[t][1] += rv.x * zv.y + rv.y * zv.x;
@@ -739,8 +774,8 @@ __global__ void k_ctl_stall(Ctl* ctl, int limit) {
 }
 
 // Mixed precision, refinement step: r64 = b - A x (FP64, the caller's K and M),
-// then the complex64 recursion restarts from it: r = demote(r64), z = r / diag,
-// p = 0, x32 = 0.  Partials of rho = r^T z (complex64 values) and |r64|^2.
+// then the complex64 recursion restarts from it: r = demote(r64), p = 0,
+// x32 = 0.  Partials of rho = r^T z (z = r / diag, complex64 values) and |r64|^2.
 template <int CT, int TMAX>
 __global__ void __launch_bounds__(kBS) k_refine(int64_t n, int R,
                                                 const int32_t* __restrict__ rowptr,
@@ -752,7 +787,7 @@ __global__ void __launch_bounds__(kBS) k_refine(int64_t n, int R,
                                                 const double* __restrict__ omu,
                                                 const double2* __restrict__ X,
                                                 const double2* __restrict__ B,
-                                                float2* __restrict__ Rr, float2* __restrict__ Z,
+                                                float2* __restrict__ Rr,
                                                 float2* __restrict__ P, float2* __restrict__ X32,
                                                 double* part) {
   constexpr int RPB = kBS / CT;
@@ -791,8 +826,7 @@ __global__ void __launch_bounds__(kBS) k_refine(int64_t n, int R,
         const double2 r64 = make_double2(bv.x - acc[t].x, bv.y - acc[t].y);
         const float2 rv = from_d<float2>(r64);
         Rr[idx] = rv;
-        const float2 zv = from_d<float2>(zdiv(to_d(rv), make_double2(kd, -om[t] * md)));
-        Z[idx] = zv;
+        const float2 zv = jacobi_z(rv, kd, md, om[t]);
         P[idx] = make_float2(0.f, 0.f);
         X32[idx] = make_float2(0.f, 0.f);
         const double2 rd = to_d(rv), zd = to_d(zv);
@@ -933,17 +967,27 @@ __device__ inline void ents_of_block(int64_t nent, int64_t* lo, int64_t* hi) {
   rows_of_block(nent, lo, hi);
 }
 
-// x += alpha p; r -= alpha q; z = B_e r (entity blocks); partials of r^T z, |r|^2
+// z_k = (B_e r)_k over an entity's d DOFs (B_e row-major d x d), as every
+// kernel that forms the preconditioned residual computes it
+template <typename VT>
+__device__ inline VT block_z(const double2* Bm, const double2* rl, int d, int k) {
+  double2 zk = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int u = 0; u < kBlk; ++u)
+    if (u < d) zk = zadd(zk, zmul(Bm[k * d + u], rl[u]));
+  return from_d<VT>(zk);
+}
+
+// r -= alpha q; partials of r^T z and |r|^2 with z = B_e r (entity blocks).
+// As in k_update, z is not stored and x += alpha p is left to k_pupdate_blk.
 template <int CT, int TMAX, typename VT>
 __global__ void __launch_bounds__(kBS) k_update_blk(const Ctl* ctl, const Col* col, int64_t nent,
                                                     int R, const int32_t* __restrict__ bptr,
                                                     const int32_t* __restrict__ boff, int64_t nb2,
                                                     const double2* __restrict__ binv,
                                                     const int32_t* __restrict__ fidx,
-                                                    VT* __restrict__ X, VT* __restrict__ Rr,
-                                                    const VT* __restrict__ P,
-                                                    const VT* __restrict__ Q,
-                                                    VT* __restrict__ Z, double* part) {
+                                                    VT* __restrict__ Rr, const VT* __restrict__ Q,
+                                                    double* part) {
   if (ctl->all_done) return;
   constexpr int EPB = kBS / CT;
   const int c = threadIdx.x % CT, es = threadIdx.x / CT;
@@ -970,13 +1014,10 @@ __global__ void __launch_bounds__(kBS) k_update_blk(const Ctl* ctl, const Col* c
         for (int k = 0; k < kBlk; ++k)
           if (k < d) {
             const int64_t idx = (int64_t)(i0 + k) * R + cc;
-            const VT pv = P[idx], qv = Q[idx];
-            VT xv = X[idx], rv = Rr[idx];
-            xv.x += al[t].x * pv.x - al[t].y * pv.y;
-            xv.y += al[t].x * pv.y + al[t].y * pv.x;
+            const VT qv = Q[idx];
+            VT rv = Rr[idx];
             rv.x -= al[t].x * qv.x - al[t].y * qv.y;
             rv.y -= al[t].x * qv.y + al[t].y * qv.x;
-            X[idx] = xv;
             Rr[idx] = rv;
             rl[k] = to_d(rv);
           }
@@ -984,13 +1025,7 @@ __global__ void __launch_bounds__(kBS) k_update_blk(const Ctl* ctl, const Col* c
 #pragma unroll
         for (int k = 0; k < kBlk; ++k)
           if (k < d) {
-            double2 zk = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int u = 0; u < kBlk; ++u)
-              if (u < d) zk = zadd(zk, zmul(Bm[k * d + u], rl[u]));
-            const VT zv = from_d<VT>(zk);
-            Z[(int64_t)(i0 + k) * R + cc] = zv;
-            const double2 zd = to_d(zv);
+            const double2 zd = to_d(block_z<VT>(Bm, rl, d, k));
             v[t][0] += rl[k].x * zd.x - rl[k].y * zd.y;
             v[t][1] += rl[k].x * zd.y + rl[k].y * zd.x;
             v[t][2] += rl[k].x * rl[k].x + rl[k].y * rl[k].y;
@@ -1000,16 +1035,58 @@ __global__ void __launch_bounds__(kBS) k_update_blk(const Ctl* ctl, const Col* c
   column_partials<CT, 3, TMAX>(R, v, part);
 }
 
-// z = B_e r for every entity and column (after a (re)start that formed r);
-// overwrites the r^T z partials (slots 0, 1) of the kernel before it
+// x += alpha p (the previous iteration's step), z = B_e r, p = z + beta p
+template <int CT, int TMAX, typename VT>
+__global__ void __launch_bounds__(kBS) k_pupdate_blk(const Ctl* ctl, const Col* col, int64_t nent,
+                                                     int R, const int32_t* __restrict__ bptr,
+                                                     const int32_t* __restrict__ boff, int64_t nb2,
+                                                     const double2* __restrict__ binv,
+                                                     const int32_t* __restrict__ fidx,
+                                                     VT* __restrict__ X, const VT* __restrict__ Rr,
+                                                     VT* __restrict__ P) {
+  if (ctl->all_done) return;
+  constexpr int EPB = kBS / CT;
+  const int c = threadIdx.x % CT, es = threadIdx.x / CT;
+  const int T = R / CT;
+  int64_t lo, hi;
+  ents_of_block(nent, &lo, &hi);
+  VT al[TMAX], be[TMAX];
+  int fi[TMAX];
+#pragma unroll
+  for (int t = 0; t < TMAX; ++t) {
+    al[t] = from_d<VT>(t < T ? col[c + t * CT].alpha : make_double2(0.0, 0.0));
+    be[t] = from_d<VT>(t < T ? col[c + t * CT].beta : make_double2(0.0, 0.0));
+    fi[t] = t < T ? fidx[c + t * CT] : 0;
+  }
+  for (int64_t e = lo + es; e < hi; e += EPB) {
+    const int i0 = bptr[e], d = bptr[e + 1] - i0;
+#pragma unroll
+    for (int t = 0; t < TMAX; ++t)
+      if (t < T) {
+        const int cc = c + t * CT;
+        double2 rl[kBlk];
+#pragma unroll
+        for (int k = 0; k < kBlk; ++k)
+          if (k < d) rl[k] = to_d(Rr[(int64_t)(i0 + k) * R + cc]);
+        const double2* Bm = binv + (int64_t)fi[t] * nb2 + boff[e];
+#pragma unroll
+        for (int k = 0; k < kBlk; ++k)
+          if (k < d)
+            step_and_direction(X, P, (int64_t)(i0 + k) * R + cc, al[t], be[t],
+                               block_z<VT>(Bm, rl, d, k));
+      }
+  }
+}
+
+// the r^T z partials (z = B_e r) for every entity and column after a
+// (re)start that formed r; overwrites slots 0, 1 of the kernel before it
 template <int CT, int TMAX, typename VT>
 __global__ void __launch_bounds__(kBS) k_bapply(int64_t nent, int R,
                                                 const int32_t* __restrict__ bptr,
                                                 const int32_t* __restrict__ boff, int64_t nb2,
                                                 const double2* __restrict__ binv,
                                                 const int32_t* __restrict__ fidx,
-                                                const VT* __restrict__ Rr, VT* __restrict__ Z,
-                                                double* part) {
+                                                const VT* __restrict__ Rr, double* part) {
   constexpr int EPB = kBS / CT;
   const int c = threadIdx.x % CT, es = threadIdx.x / CT;
   const int T = R / CT;
@@ -1032,13 +1109,7 @@ __global__ void __launch_bounds__(kBS) k_bapply(int64_t nent, int R,
 #pragma unroll
         for (int k = 0; k < kBlk; ++k)
           if (k < d) {
-            double2 zk = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int u = 0; u < kBlk; ++u)
-              if (u < d) zk = zadd(zk, zmul(Bm[k * d + u], rl[u]));
-            const VT zv = from_d<VT>(zk);
-            Z[(int64_t)(i0 + k) * R + cc] = zv;
-            const double2 zd = to_d(zv);
+            const double2 zd = to_d(block_z<VT>(Bm, rl, d, k));
             v[t][0] += rl[k].x * zd.x - rl[k].y * zd.y;
             v[t][1] += rl[k].x * zd.y + rl[k].y * zd.x;
           }
@@ -1090,7 +1161,7 @@ struct Batch {
   const int32_t *rowptr = nullptr, *cols = nullptr;
   const double *K = nullptr, *M = nullptr;
   double *omu = nullptr, *dK = nullptr, *dM = nullptr, *part = nullptr;
-  double2 *Rr = nullptr, *P = nullptr, *Q = nullptr, *Z = nullptr;
+  double2 *Rr = nullptr, *P = nullptr, *Q = nullptr;
   Ctl* ctl = nullptr;
   Col* col = nullptr;
   Ctl* hctl = nullptr;
@@ -1117,7 +1188,7 @@ struct Batch {
   int nfreq = 0;
   double2* binv = nullptr;
   std::vector<double> hom;  // omega * mu0 per column (host)
-  float2 *R32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *Z32 = nullptr, *X32 = nullptr;
+  float2 *R32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *X32 = nullptr;
   float *Ks32 = nullptr, *Ms32 = nullptr;
   // element-by-element products (optional, pg_batch_set_element_operator):
   // the operator, its summation schedule, the local results per precision and
@@ -1145,10 +1216,15 @@ struct Batch {
 };
 
 template <int CT, int TM, typename VT, typename MT>
-static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q, VT* Z,
-                          const MT* Ks, const MT* Ms) {
+static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q, const MT* Ks,
+                          const MT* Ms) {
   const dim3 g(B->grid), b(kBS), gr(B->R);
-  hipLaunchKernelGGL((k_pupdate<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->n, B->R, Z, P);
+  if (B->binv)
+    hipLaunchKernelGGL((k_pupdate_blk<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->nent, B->R,
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, X, Rr, P);
+  else
+    hipLaunchKernelGGL((k_pupdate<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK,
+                       B->dM, B->omu, X, Rr, P);
   int np = B->grid;
   if (B->eop) {
     const int32_t* done = &B->ctl->all_done;
@@ -1183,10 +1259,10 @@ static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q,
   hipLaunchKernelGGL(k_fin_mu, gr, b, 0, st, B->ctl, B->col, B->R, B->part, np);
   if (B->binv)
     hipLaunchKernelGGL((k_update_blk<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->nent, B->R,
-                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, X, Rr, P, Q, Z, B->part);
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, Rr, Q, B->part);
   else
     hipLaunchKernelGGL((k_update<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->n, B->R, B->dK,
-                       B->dM, B->omu, X, Rr, P, Q, Z, B->part);
+                       B->dM, B->omu, Rr, Q, B->part);
   hipLaunchKernelGGL(k_fin_update, gr, b, 0, st, B->ctl, B->col, B->R, B->part, B->grid);
   hipLaunchKernelGGL(k_fin_all, dim3(1), b, 0, st, B->ctl, B->col, B->R);
 }
@@ -1194,21 +1270,31 @@ static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q,
 template <int CT, int TM>
 static void launch_iter(Batch* B, hipStream_t st, double2* X) {
   if (B->mixed)
-    launch_iter_t<CT, TM, float2, float>(B, st, B->X32, B->R32, B->P32, B->Q32, B->Z32, B->Ks32,
+    launch_iter_t<CT, TM, float2, float>(B, st, B->X32, B->R32, B->P32, B->Q32, B->Ks32,
                                          B->Ms32);
   else
-    launch_iter_t<CT, TM, double2, double>(B, st, X, B->Rr, B->P, B->Q, B->Z, B->Ks, B->Ms);
+    launch_iter_t<CT, TM, double2, double>(B, st, X, B->Rr, B->P, B->Q, B->Ks, B->Ms);
+}
+
+// the last iteration's step x += alpha p once every column has paused
+static void launch_xstep(Batch* B, hipStream_t st, double2* X) {
+  const int64_t nr = B->n * (int64_t)B->R;
+  const dim3 g((unsigned)((nr + kBS - 1) / kBS)), b(kBS);
+  if (B->mixed)
+    hipLaunchKernelGGL(k_bxstep<float2>, g, b, 0, st, B->col, nr, B->R, B->P32, B->X32);
+  else
+    hipLaunchKernelGGL(k_bxstep<double2>, g, b, 0, st, B->col, nr, B->R, B->P, X);
 }
 
 template <int CT, int TM>
 static void launch_refine(Batch* B, hipStream_t st, const double2* Bv, const double2* X,
                           int first) {
   hipLaunchKernelGGL((k_refine<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->rowptr,
-                     B->cols, B->K, B->M, B->dK, B->dM, B->omu, X, Bv, B->R32, B->Z32, B->P32,
+                     B->cols, B->K, B->M, B->dK, B->dM, B->omu, X, Bv, B->R32, B->P32,
                      B->X32, B->part);
   if (B->binv)
     hipLaunchKernelGGL((k_bapply<CT, TM, float2>), dim3(B->grid), dim3(kBS), 0, st, B->nent, B->R,
-                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->R32, B->Z32, B->part);
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->R32, B->part);
   hipLaunchKernelGGL(k_fin_refine, dim3(B->R), dim3(kBS), 0, st, B->ctl, B->col, B->R, B->part,
                      B->grid, first, B->delta2);
   hipLaunchKernelGGL(k_ctl_resume, dim3(1), dim3(64), 0, st, B->ctl);
@@ -1218,10 +1304,10 @@ static void launch_refine(Batch* B, hipStream_t st, const double2* Bv, const dou
 template <int CT, int TM>
 static void launch_init(Batch* B, hipStream_t st, const double2* Bv, double2* X) {
   hipLaunchKernelGGL((k_init<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->dK, B->dM,
-                     B->omu, Bv, X, B->Rr, B->Z, B->P, B->part);
+                     B->omu, Bv, X, B->Rr, B->P, B->part);
   if (B->binv)
     hipLaunchKernelGGL((k_bapply<CT, TM, double2>), dim3(B->grid), dim3(kBS), 0, st, B->nent, B->R,
-                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->Rr, B->Z, B->part);
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->Rr, B->part);
   hipLaunchKernelGGL(k_fin_init, dim3(B->R), dim3(kBS), 0, st, B->col, B->R, B->part, B->grid);
 }
 
@@ -1267,11 +1353,11 @@ static int refresh_diag(Batch* B, hipStream_t st) {
 template <int CT, int TM>
 static void launch_reinit(Batch* B, hipStream_t st, const double2* Bv, const double2* X) {
   hipLaunchKernelGGL((k_reinit<CT, TM>), dim3(B->grid), dim3(kBS), 0, st, B->n, B->R, B->rowptr,
-                     B->cols, B->K, B->M, B->dK, B->dM, B->omu, X, Bv, B->Rr, B->Z, B->P,
+                     B->cols, B->K, B->M, B->dK, B->dM, B->omu, X, Bv, B->Rr, B->P,
                      B->part);
   if (B->binv)
     hipLaunchKernelGGL((k_bapply<CT, TM, double2>), dim3(B->grid), dim3(kBS), 0, st, B->nent, B->R,
-                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->Rr, B->Z, B->part);
+                       B->bptr, B->boff, B->nb2, B->binv, B->fidx, B->Rr, B->part);
   hipLaunchKernelGGL(k_fin_reinit, dim3(B->R), dim3(kBS), 0, st, B->ctl, B->col, B->R, B->part,
                      B->grid);
   hipLaunchKernelGGL(k_ctl_resume, dim3(1), dim3(64), 0, st, B->ctl);
@@ -1345,7 +1431,6 @@ int pg_batch_create(int64_t n, int64_t nnz, const int32_t* rowptr, const int32_t
   PG_BA(B->Rr, (size_t)n * R);
   PG_BA(B->P, (size_t)n * R);
   PG_BA(B->Q, (size_t)n * R);
-  PG_BA(B->Z, (size_t)n * R);
   PG_BA(B->ctl, 1);
   PG_BA(B->col, R);
   PG_BA(B->bad, 1);
@@ -1398,6 +1483,7 @@ int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations,
         PG_HIP(hipStreamSynchronize(st));
         if (B->hctl->all_done) break;
       }
+      launch_xstep(B, st, X);
       hipLaunchKernelGGL(k_accum, dim3((unsigned)(((int64_t)B->n * B->R + kBS - 1) / kBS)),
                          dim3(kBS), 0, st, (int64_t)B->n * B->R, X, B->X32);
       PG_BATCH_DISPATCH(launch_refine, B, st, Bv, X, 0);
@@ -1433,6 +1519,7 @@ int pg_batch_solve(void* batch, const double* b, double* x, int32_t* iterations,
         PG_HIP(hipStreamSynchronize(st));
         if (B->hctl->all_done) break;
       }
+      launch_xstep(B, st, X);
       PG_BATCH_DISPATCH(launch_true, B, st, Bv, X);
       PG_CHECK_LAUNCH();
       PG_HIP(hipMemcpyAsync(B->hcol, B->col, B->R * sizeof(Col), hipMemcpyDeviceToHost, st));
@@ -1541,7 +1628,7 @@ int pg_batch_set_mixed(void* batch, double inner_rtol, int stall_iters, void* st
   const size_t nr = (size_t)B->n * B->R;
   int rc;
   if ((rc = B->alloc(&B->R32, nr)) != PG_OK || (rc = B->alloc(&B->P32, nr)) != PG_OK ||
-      (rc = B->alloc(&B->Q32, nr)) != PG_OK || (rc = B->alloc(&B->Z32, nr)) != PG_OK ||
+      (rc = B->alloc(&B->Q32, nr)) != PG_OK ||
       (rc = B->alloc(&B->X32, nr)) != PG_OK || (rc = B->alloc(&B->Ks32, B->nnz)) != PG_OK ||
       (rc = B->alloc(&B->Ms32, B->nnz)) != PG_OK)
     return rc;

@@ -479,23 +479,21 @@ __global__ void __launch_bounds__(kBS) k_fin_res64(Scal* __restrict__ s, const d
 // ---------------------------------------------------------------------------
 // COCG vector kernels
 // ---------------------------------------------------------------------------
-// x += alpha p; r -= alpha q; z = Dinv r; partials of r^T z and |r|^2
+// r -= alpha q; z = Dinv r; partials of r^T z and |r|^2.  The step x += alpha p
+// of the same iteration is taken by the next k_pupdate (which reads p anyway)
+// or, once the recursion pauses, by k_xstep.
 template <typename XT>
 __global__ void __launch_bounds__(kBS) k_update(const Scal* __restrict__ s, int64_t n,
-                                                XT* __restrict__ x, XT* __restrict__ r,
-                                                const XT* __restrict__ p,
-                                                const XT* __restrict__ q,
+                                                XT* __restrict__ r, const XT* __restrict__ q,
                                                 const XT* __restrict__ dinv,
                                                 XT* __restrict__ z, double* part) {
   if (s->done) return;
   const double2 al = s->alpha;
-  const XT a = from_d<XT>(al);
   const XT na = from_d<XT>(make_double2(-al.x, -al.y));
   int64_t lo, hi;
   chunk(n, kBS, &lo, &hi);
   double red[3] = {0.0, 0.0, 0.0};
   for (int64_t i = lo + threadIdx.x; i < hi; i += kBS) {
-    x[i] = cfma(a, p[i], x[i]);
     const XT ri = cfma(na, q[i], r[i]);
     r[i] = ri;
     const XT zi = cm(dinv[i], ri);
@@ -512,12 +510,10 @@ __global__ void __launch_bounds__(kBS) k_update(const Scal* __restrict__ s, int6
 template <typename XT>
 __global__ void __launch_bounds__(kBS) k_update_blk(
     const Scal* __restrict__ s, int64_t nb, const int32_t* __restrict__ bptr,
-    const int32_t* __restrict__ boff, XT* __restrict__ x, XT* __restrict__ r,
-    const XT* __restrict__ p, const XT* __restrict__ q, const XT* __restrict__ binv,
-    XT* __restrict__ z, double* part) {
+    const int32_t* __restrict__ boff, XT* __restrict__ r, const XT* __restrict__ q,
+    const XT* __restrict__ binv, XT* __restrict__ z, double* part) {
   if (s->done) return;
   const double2 al = s->alpha;
-  const XT a = from_d<XT>(al);
   const XT na = from_d<XT>(make_double2(-al.x, -al.y));
   int64_t lo, hi;
   chunk(nb, kBS, &lo, &hi);
@@ -530,7 +526,6 @@ __global__ void __launch_bounds__(kBS) k_update_blk(
     for (int t = 0; t < kMaxBlock; ++t)
       if (t < d) {
         const int i = i0 + t;
-        x[i] = cfma(a, p[i], x[i]);
         rl[t] = cfma(na, q[i], r[i]);
         r[i] = rl[t];
       }
@@ -556,32 +551,25 @@ __global__ void __launch_bounds__(kBS) k_update_blk(
 // 32-byte (complex128) access per vector and block, and no block index reads.
 template <typename XT>
 __global__ void __launch_bounds__(kBS) k_update_blk2(const Scal* __restrict__ s, int64_t nb,
-                                                     XT* __restrict__ x, XT* __restrict__ r,
-                                                     const XT* __restrict__ p,
+                                                     XT* __restrict__ r,
                                                      const XT* __restrict__ q,
                                                      const XT* __restrict__ binv,
                                                      XT* __restrict__ z, double* part) {
   using V = typename PairOf<XT>::type;
   if (s->done) return;
   const double2 al = s->alpha;
-  const XT a = from_d<XT>(al);
   const XT na = from_d<XT>(make_double2(-al.x, -al.y));
   int64_t lo, hi;
   chunk(nb, kBS, &lo, &hi);
   double red[3] = {0.0, 0.0, 0.0};
-  V* x2 = reinterpret_cast<V*>(x);
   V* r2 = reinterpret_cast<V*>(r);
   V* z2 = reinterpret_cast<V*>(z);
-  const V* p2 = reinterpret_cast<const V*>(p);
   const V* q2 = reinterpret_cast<const V*>(q);
   const V* b2 = reinterpret_cast<const V*>(binv);
   for (int64_t k = lo + threadIdx.x; k < hi; k += kBS) {
-    const V xv = x2[k], rv = r2[k], pv = p2[k], qv = q2[k], B0 = b2[2 * k], B1 = b2[2 * k + 1];
-    const XT xa = cfma(a, XT{pv.x, pv.y}, XT{xv.x, xv.y});
-    const XT xb = cfma(a, XT{pv.z, pv.w}, XT{xv.z, xv.w});
+    const V rv = r2[k], qv = q2[k], B0 = b2[2 * k], B1 = b2[2 * k + 1];
     const XT ra = cfma(na, XT{qv.x, qv.y}, XT{rv.x, rv.y});
     const XT rb = cfma(na, XT{qv.z, qv.w}, XT{rv.z, rv.w});
-    x2[k] = V{xa.x, xa.y, xb.x, xb.y};
     r2[k] = V{ra.x, ra.y, rb.x, rb.y};
     // z = B r, B row-major 2 x 2: B0 = (B00, B01), B1 = (B10, B11)
     const XT za = cfma(XT{B0.z, B0.w}, rb, cfma(XT{B0.x, B0.y}, ra, czero<XT>()));
@@ -627,15 +615,33 @@ __global__ void __launch_bounds__(kBS) k_fin_update(Scal* __restrict__ s, const 
   }
 }
 
-// p = z + beta p
+// x += alpha p (the previous iteration's step: alpha is still that iteration's,
+// and after a (re)start p = 0), then p = z + beta p
 template <typename XT>
 __global__ void __launch_bounds__(kBS) k_pupdate(const Scal* __restrict__ s, int64_t n,
-                                                 const XT* __restrict__ z, XT* __restrict__ p) {
+                                                 const XT* __restrict__ z, XT* __restrict__ p,
+                                                 XT* __restrict__ x) {
   if (s->done) return;
+  const XT a = from_d<XT>(s->alpha);
   const XT be = from_d<XT>(s->beta);
   int64_t lo, hi;
   chunk(n, kBS, &lo, &hi);
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kBS) p[i] = cfma(be, p[i], z[i]);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBS) {
+    const XT pv = p[i];
+    x[i] = cfma(a, pv, x[i]);
+    p[i] = cfma(be, pv, z[i]);
+  }
+}
+
+// x += alpha p: the step of the last iteration before the recursion paused
+// (alpha is 0 after a breakdown)
+template <typename XT>
+__global__ void k_xstep(const Scal* __restrict__ s, int64_t n, const XT* __restrict__ p,
+                        XT* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const XT a = from_d<XT>(s->alpha);
+  x[i] = cfma(a, p[i], x[i]);
 }
 
 // (re)start: r = r64 (demoted), z = P r, p = 0; partials of r^T z, |r|^2, |b|^2
@@ -1227,24 +1233,24 @@ template <typename VT, typename XT>
 static void launch_iteration(Solver* S, hipStream_t st, const SpmvCfg& c, const VT* vals, XT* x,
                              XT* r, XT* p, XT* q, const XT* dinv, const XT* binv, XT* z,
                              bool sample) {
-  hipLaunchKernelGGL(k_pupdate<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, z, p);
+  hipLaunchKernelGGL(k_pupdate<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, z, p, x);
   // a failed record only loses this timing sample (the elapsed-time query fails)
   if (sample) (void)hipEventRecord(S->ev[0], st);
   spmv<1, VT, XT>(S, c, st, vals, p, q, nullptr, sample && c.kind == 3 ? S->ev[2] : nullptr);
   if (sample) (void)hipEventRecord(S->ev[1], st);
   hipLaunchKernelGGL(k_fin_mu, dim3(1), dim3(kBS), 0, st, S->s, S->part, c.grid);
   int np;
-  if (S->o.pc == PG_PC_BJACOBI && S->blk2 && (reinterpret_cast<uintptr_t>(x) % 32) == 0) {
-    hipLaunchKernelGGL(k_update_blk2<XT>, dim3(S->gb), dim3(kBS), 0, st, S->s, S->nblocks, x, r, p,
-                       q, binv, z, S->part);
+  if (S->o.pc == PG_PC_BJACOBI && S->blk2) {  // r, q, z: the solver's own 256-byte aligned buffers
+    hipLaunchKernelGGL(k_update_blk2<XT>, dim3(S->gb), dim3(kBS), 0, st, S->s, S->nblocks, r, q,
+                       binv, z, S->part);
     np = S->gb;
   } else if (S->o.pc == PG_PC_BJACOBI) {
     hipLaunchKernelGGL(k_update_blk<XT>, dim3(S->gb), dim3(kBS), 0, st, S->s, S->nblocks, S->bptr,
-                       S->boff, x, r, p, q, binv, z, S->part);
+                       S->boff, r, q, binv, z, S->part);
     np = S->gb;
   } else {
-    hipLaunchKernelGGL(k_update<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, x, r, p, q, dinv,
-                       z, S->part);
+    hipLaunchKernelGGL(k_update<XT>, dim3(S->gv), dim3(kBS), 0, st, S->s, S->n, r, q, dinv, z,
+                       S->part);
     np = S->gv;
   }
   hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(kBS), 0, st, S->s, S->part, np);
@@ -1329,7 +1335,13 @@ static int cocg_solve(Solver* S, const double2* b, double2* x, pg_solve_info* in
       break;
     }
     if (!hs.done) continue;
-    // paused: converged recursion, refinement point, stagnation or iteration limit
+    // paused: converged recursion, refinement point, stagnation or iteration limit;
+    // the last iteration's step x += alpha p is still to be taken
+    if (low)
+      hipLaunchKernelGGL(k_xstep<float2>, dim3(nblk(n, kBS)), dim3(kBS), 0, st, S->s, n, S->p32,
+                         S->x32);
+    else
+      hipLaunchKernelGGL(k_xstep<double2>, dim3(nblk(n, kBS)), dim3(kBS), 0, st, S->s, n, S->p, x);
     if (low) {
       hipLaunchKernelGGL(k_accumulate, dim3(nblk(n, kBS)), dim3(kBS), 0, st, n, x, S->x32);
       refinements++;
