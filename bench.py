@@ -498,7 +498,7 @@ def adjoint_leg(args, log):
     """Adjoint gradient: 8 HED sources x 2 frequencies, 101 receivers, 30^3 p=2 mesh;
     observations from the model with the resistive block, gradient at the model
     without it, w.r.t. every element's log-conductivity.  Timed two ways: forward
-    and adjoint as two batched 16-column solves (--batch-precision), and as 32
+    and adjoint as two batched 16-column solves (--adjoint-precision), and as 32
     single-RHS solves (mixed precision, entity-block Jacobi, two at a time on two
     streams); `grad_per_s` is the faster of the two."""
     import torch
@@ -514,7 +514,7 @@ def adjoint_leg(args, log):
     m0 = torch.tensor(np.log(w0.sigma[:, 0]), dtype=torch.float64, device="cuda")
     res, grads = {}, {}
     observed = floor = None
-    for mode, kw in (("batched", dict(batched=True, precision=args.batch_precision)),
+    for mode, kw in (("batched", dict(batched=True, precision=args.adjoint_precision)),
                      ("per_rhs", dict(batched=False, precision="mixed"))):
         prob = InversionProblem(T, src, [0.25, 1.0], rx, rtol=args.rtol,
                                 device=torch.device("cuda"), **kw)
@@ -546,7 +546,7 @@ def adjoint_leg(args, log):
                          "2 frequencies (0.25, 1 Hz), 101 receivers, rtol "
                          f"{args.rtol}; misfit weights 1/(|E_obs|^2 + (1e-6 max|E_obs|)^2); "
                          "forward + adjoint solves + element contraction; "
-                         f"batched = two 16-column batched solves ({args.batch_precision}, "
+                         f"batched = two 16-column batched solves ({args.adjoint_precision}, "
                          "entity-block Jacobi), per_rhs = 32 single solves (mixed, entity-block "
                          "Jacobi, element products, two at a time on two streams)")}
 
@@ -569,8 +569,11 @@ def main():
                          "(RHS/s), adjoint (grad/s)")
     ap.add_argument("--batch-width", type=int, default=64,
                     help="columns per batched solve (chunks of one frequency's positions)")
-    ap.add_argument("--batch-precision", default="mixed", choices=["fp64", "mixed"],
-                    help="batched and adjoint legs: FP64, or complex64 under FP64 refinement")
+    ap.add_argument("--batch-precision", default="fp64", choices=["fp64", "mixed"],
+                    help="batched leg: FP64, or complex64 under FP64 refinement (at C4 the "
+                         "complex64 rounds stagnate at 0.1 Hz and the FP64 batch is faster)")
+    ap.add_argument("--adjoint-precision", default="mixed", choices=["fp64", "mixed"],
+                    help="the adjoint leg's batched solves")
     ap.add_argument("--batch-positions", type=int, default=64,
                     help="HED positions of the batched leg (of 64; x 4 frequencies)")
     ap.add_argument("--single-positions", type=int, default=64,

@@ -178,6 +178,14 @@ class Plan:
             nat.check(L.pg_plan_export(h, nat.ptr(rowptr), nat.ptr(cols), nat.ptr(slot0),
                                        nat.ptr(rowlen), nat.ptr(cbeg), nat.ptr(ab),
                                        nat.ptr(contrib)))
+            # first pair per entity, when every entity fits the row-major
+            # numeric assembly (else the two-phase kernels run)
+            epair = np.empty(self.n_entities + 1, dtype=np.int32)
+            rc = L.pg_plan_entity_pairs(h, nat.ptr(epair))
+            if rc == nat.PG_ERR_UNSUPPORTED:
+                epair = None
+            else:
+                nat.check(rc)
         finally:
             L.pg_plan_destroy(h)
         if self.ndof != mesh.ndof:
@@ -191,11 +199,13 @@ class Plan:
         self.cbeg = torch.from_numpy(cbeg).to(dev)
         self.ab = torch.from_numpy(ab).to(dev)
         self.contrib = torch.from_numpy(contrib.view(np.int32)).to(dev)
+        self.entity_pair = None if epair is None else torch.from_numpy(epair).to(dev)
         self.dev_struct = nat.PlanDev(
             mesh.p, mesh.n_edges, mesh.n_faces, mesh.n_elems, self.ndof, self.nnz, self.npairs,
             self.rowptr.data_ptr(), self.cols.data_ptr(), self.slot0.data_ptr(),
             self.rowlen.data_ptr(), self.cbeg.data_ptr(), self.ab.data_ptr(),
-            self.contrib.data_ptr())
+            self.contrib.data_ptr(),
+            None if self.entity_pair is None else self.entity_pair.data_ptr())
 
 
 class Assembler:

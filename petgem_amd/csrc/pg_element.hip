@@ -21,6 +21,7 @@
 
 #include "pg_basis.h"
 #include "pg_common.h"
+#include "pg_rowasm.h"
 
 namespace pg {
 
@@ -474,6 +475,133 @@ __global__ void __launch_bounds__(256) k_asm_sums(
 }
 
 // -------------------------------------------------------------------------
+// Row-major assembly (p = 1, 2; plans within the bounds of pg_rowasm.h): one
+// single-wave workgroup per entity A with DOFs keeps A's d(A) CSR rows in LDS
+// and walks the elements containing A in ascending order (the contributors of
+// the diagonal pair (A, A)).  For each element its lanes form the entries of
+// the element's block row la(A) -- row k < d(A), every local DOF j -- from the
+// tables and the element's coefficients (one element per step: wave-uniform
+// loads) and add each into the LDS row at the column of entity(e, lb).  Every
+// slot is thus the sum of its contributors in ascending element order, bitwise
+// the two-phase result, with no block scratch; A's rows then leave as one
+// contiguous run of d(A) * rowlen values.
+// -------------------------------------------------------------------------
+template <int P>
+__global__ void __launch_bounds__(64) k_asm_rows(
+    int64_t nE, int64_t nF, const int32_t* __restrict__ epair, const double* __restrict__ T,
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ pair_slot0,
+    const int32_t* __restrict__ pair_cbeg, const int32_t* __restrict__ pair_ab,
+    const uint32_t* __restrict__ contrib, const double* __restrict__ coef,
+    const int32_t* __restrict__ orient, double omega_mu, int km,
+    const uint8_t* __restrict__ ent_bc, double2* __restrict__ values, double* __restrict__ Kout,
+    double* __restrict__ Mout) {
+  constexpr int NL = asm_nloc<P>(), MAXD = asm_maxd<P>();
+  constexpr int DE = P, DF = P * (P - 1), DV = P * (P - 1) * (P - 2) / 2;
+  constexpr int NLD = 6 * DE + 4 * DF + DV;
+  constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
+  __shared__ double2 acc[MAXD * kRowAsmMaxLen];
+  __shared__ int16_t offs[kRowAsmMaxInc * NL];  // column of entity(e_i, lb) in A's rows
+  __shared__ int32_t inc_e[kRowAsmMaxInc];
+  __shared__ int32_t inc_la[kRowAsmMaxInc];
+  __shared__ int32_t pr_cbeg[kRowAsmMaxPairs + 1];
+  __shared__ int16_t pr_off[kRowAsmMaxPairs];
+  __shared__ uint8_t colbc[kRowAsmMaxLen];
+  __shared__ int32_t s_pd;
+  const int lane = threadIdx.x;
+  const int64_t A = blockIdx.x;  // the grid covers the entities with DOFs
+  const int dA = entity_ndof_global<P>(A, nE, nF);
+  const int64_t r0 = A < nE ? A * DE
+                            : (A < nE + nF ? nE * DE + (A - nE) * DF
+                                           : nE * DE + nF * DF + (A - nE - nF) * DV);
+  const int64_t rowbase = rowptr[r0];
+  const int rowlen = (int)(rowptr[r0 + 1] - rowbase);
+  const int p0 = epair[A], npr = epair[A + 1] - p0;
+  // (the plan export admits only plans within the bounds)
+  if (dA == 0 || npr <= 0 || npr > kRowAsmMaxPairs || rowlen > kRowAsmMaxLen) return;
+  const int bcA = ent_bc ? ent_bc[A] : 0;
+  // A's pairs: contributor ranges, column offsets, PEC columns, the diagonal pair
+  for (int q = lane; q < npr; q += 64) {
+    const int pi = p0 + q;
+    const int64_t B = pair_ab[2 * pi + 1];
+    const int off = (int)(pair_slot0[pi] - rowbase);
+    pr_cbeg[q] = pair_cbeg[pi];
+    pr_off[q] = (int16_t)off;
+    const int dB = entity_ndof_global<P>(B, nE, nF);
+    const uint8_t bb = ent_bc ? ent_bc[B] : 0;
+    for (int l = 0; l < dB; ++l) colbc[off + l] = bb;
+    if (B == A) s_pd = q;
+  }
+  if (lane == 0) pr_cbeg[npr] = pair_cbeg[p0 + npr];
+  for (int q = lane; q < dA * rowlen; q += 64) acc[q] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int pd = s_pd;
+  const int ci0 = pr_cbeg[pd], ninc = pr_cbeg[pd + 1] - ci0;
+  if (ninc > kRowAsmMaxInc) return;  // uniform
+  // the elements containing A, ascending: the diagonal pair's contributors
+  for (int i = lane; i < ninc; i += 64) {
+    const uint32_t u = contrib[ci0 + i];
+    inc_e[i] = (int32_t)(u >> 8);
+    inc_la[i] = (int32_t)((u >> 4) & 15);
+  }
+  __syncthreads();
+  // column offset of entity(e, lb) for every contributor (e, la, lb) of A's pairs
+  const int c0 = pr_cbeg[0], c1 = pr_cbeg[npr];
+  for (int c = c0 + lane; c < c1; c += 64) {
+    const uint32_t u = contrib[c];
+    const int e = (int)(u >> 8), lb = (int)(u & 15);
+    int lo = 0, hi = npr - 1;  // the pair of c: last q with pr_cbeg[q] <= c
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pr_cbeg[mid] <= c) lo = mid;
+      else hi = mid - 1;
+    }
+    int a = 0, b = ninc - 1;  // the position of e in the ascending element list
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (inc_e[mid] < e) a = mid + 1;
+      else b = mid;
+    }
+    offs[a * NL + lb] = pr_off[lo];
+  }
+  __syncthreads();
+  // element block rows, ascending elements
+  for (int i = 0; i < ninc; ++i) {
+    const int64_t e = __builtin_amdgcn_readfirstlane(inc_e[i]);
+    const int la = __builtin_amdgcn_readfirstlane(inc_la[i]);
+    const int pk = orient[e];
+    const double* cf = coef + 12 * e;
+    const int va0 = entity_variant_base(la, orient_of(pk, la), P);
+    for (int q = lane; q < dA * NLD; q += 64) {
+      const int k = q / NLD, j = q - k * NLD;
+      int lb, l;
+      dof_entity<P>(j, lb, l);
+      const int vb = entity_variant_base(lb, orient_of(pk, lb), P) + l;
+      double m, kk;
+      contract(T + ((size_t)(va0 + k) * V + vb) * 12, cf, &m, &kk);
+      // separately rounded product, as the reference forms i*w*mu0*M_e before adding
+      const double im = km ? m : -__dmul_rn(omega_mu, m);
+      double2* sl = acc + k * rowlen + offs[i * NL + lb] + l;
+      sl->x += kk;
+      sl->y += im;
+    }
+    __syncthreads();
+  }
+  // PEC rows and columns (zero, unit diagonal), then A's rows in one run
+  const int dg = pr_off[pd];
+  for (int q = lane; q < dA * rowlen; q += 64) {
+    const int k = q / rowlen, col = q - k * rowlen;
+    double2 v = acc[q];
+    if (bcA | colbc[col]) v = make_double2((bcA && col - dg == k) ? 1.0 : 0.0, 0.0);
+    if (Kout) {
+      Kout[rowbase + q] = v.x;
+      Mout[rowbase + q] = v.y;
+    } else {
+      values[rowbase + q] = v;
+    }
+  }
+}
+
+// -------------------------------------------------------------------------
 // PEC boundary condition on an assembled CSR (16 lanes per row, so that the
 // column and value reads of a row are coalesced)
 // -------------------------------------------------------------------------
@@ -892,6 +1020,23 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
   const dim3 g(nblocks(threads, 256)), gb(nblocks(pl->n_elems, 256) * nloc * nloc), b(256);
   hipStream_t s = as_stream(stream);
   double2* v = reinterpret_cast<double2*>(values);
+  if (t->p <= 2 && pl->entity_pair) {
+    const int64_t nent = t->p == 1 ? pl->n_edges : pl->n_edges + pl->n_faces;
+    const int km = Kout ? 1 : 0;
+#define PG_ROWS(P)                                                                             \
+  hipLaunchKernelGGL(k_asm_rows<P>, dim3((unsigned)nent), dim3(64), 0, s, pl->n_edges,          \
+                     pl->n_faces, pl->entity_pair, t->d_T, pl->rowptr, pl->pair_slot0,          \
+                     pl->pair_cbeg, pl->pair_ab, pl->contrib, ws_coef(w),                       \
+                     ws_orient(w, pl->n_elems), om, km, ent_bc, v, Kout, Mout)
+    if (t->p == 1) {
+      PG_ROWS(1);
+    } else {
+      PG_ROWS(2);
+    }
+#undef PG_ROWS
+    PG_CHECK_LAUNCH();
+    return PG_OK;
+  }
   if (t->p == 3) {
     hipLaunchKernelGGL(k_assemble<3>, g, b, 0, s, pl->npairs, pl->n_edges, pl->n_faces, t->d_T,
                        pl->pair_slot0, pl->pair_rowlen, pl->pair_cbeg, pl->pair_ab, pl->contrib,

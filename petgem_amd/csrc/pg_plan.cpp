@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "pg_common.h"
+#include "pg_rowasm.h"
 
 namespace pg {
 
@@ -33,6 +34,10 @@ struct Plan {
   int maxd;
   std::vector<int32_t> rowptr, cols, pair_slot0, pair_rowlen, pair_cbeg, pair_ab;
   std::vector<uint32_t> contrib;
+  // first pair of every entity (nEnt + 1), and the largest row length, pair
+  // count and element count of an entity with DOFs
+  std::vector<int32_t> entity_pair;
+  int64_t max_rowlen = 0, max_pairs = 0, max_inc = 0;
 };
 
 namespace {
@@ -180,6 +185,12 @@ int pg_plan_build(int p, int64_t E, int64_t nE, int64_t nF, const int32_t* ee,
   }
   P->npairs = pbase[nEnt];
   P->nnz = zbase[nEnt];
+  for (int64_t A = 0; A < nEnt; ++A)
+    if (c.ndof(A)) {
+      P->max_rowlen = std::max(P->max_rowlen, rowlen[A]);
+      P->max_pairs = std::max(P->max_pairs, nadj[A]);
+      P->max_inc = std::max(P->max_inc, iptr[A + 1] - iptr[A]);
+    }
   P->ncontrib = cbase[nEnt];
   if (P->nnz >= (1LL << 31) || P->ncontrib >= (1LL << 31) || P->ndof >= (1LL << 31)) {
     delete P;
@@ -194,6 +205,8 @@ int pg_plan_build(int p, int64_t E, int64_t nE, int64_t nF, const int32_t* ee,
   P->pair_ab.resize(2 * P->npairs);
   P->contrib.resize(P->ncontrib);
   P->pair_cbeg[P->npairs] = (int32_t)P->ncontrib;
+  P->entity_pair.resize(nEnt + 1);
+  for (int64_t A = 0; A <= nEnt; ++A) P->entity_pair[A] = (int32_t)pbase[A];
 
   // rowptr: DOF rows in entity order (DOF numbering is entity-major)
   for (int64_t A = 0; A < nEnt; ++A) {
@@ -263,6 +276,20 @@ int pg_plan_export(const void* plan, int32_t* rowptr, int32_t* cols, int32_t* sl
   cp(cbeg, P->pair_cbeg.data(), P->pair_cbeg.size() * 4);
   cp(ab, P->pair_ab.data(), P->pair_ab.size() * 4);
   cp(contrib, P->contrib.data(), P->contrib.size() * 4);
+  return PG_OK;
+}
+
+int pg_plan_entity_pairs(const void* plan, int32_t* entity_pair) {
+  if (!plan || !entity_pair) PG_FAIL(PG_ERR_ARG, "NULL argument");
+  const Plan* P = static_cast<const Plan*>(plan);
+  if (P->max_rowlen > kRowAsmMaxLen || P->max_pairs > kRowAsmMaxPairs ||
+      P->max_inc > kRowAsmMaxInc)
+    PG_FAIL(PG_ERR_UNSUPPORTED,
+            "entity rows beyond the row-major assembly bounds (row length %lld of %d, "
+            "%lld coupled entities of %d, %lld elements of %d)",
+            (long long)P->max_rowlen, kRowAsmMaxLen, (long long)P->max_pairs, kRowAsmMaxPairs,
+            (long long)P->max_inc, kRowAsmMaxInc);
+  std::memcpy(entity_pair, P->entity_pair.data(), P->entity_pair.size() * sizeof(int32_t));
   return PG_OK;
 }
 

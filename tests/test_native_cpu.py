@@ -89,6 +89,40 @@ def test_plan_contributions_cover_every_element_entry():
 
 
 @pytest.mark.parametrize("p", [1, 2, 3])
+def test_plan_entity_pairs_index_each_entity_s_pairs(test_mesh, p):
+    """entity_pair[A] .. entity_pair[A+1] are exactly the pairs (A, B).  At p <= 2
+    the reference mesh fits the row-major assembly's bounds; at p = 3 its rows
+    (up to 303 columns) do not, and the export says so."""
+    pts, en, _ = test_mesh
+    T = Topology(pts, en, p)
+    ee = np.ascontiguousarray(T.elems_edges, np.int32)
+    ef = np.ascontiguousarray(T.elems_faces, np.int32)
+    L = nat.lib()
+    h = ctypes.c_void_p()
+    nat.check(L.pg_plan_build(T.p, T.n_elems, T.n_edges, T.n_faces, nat.ptr(ee), nat.ptr(ef),
+                              ctypes.byref(h)))
+    try:
+        info = np.zeros(6, np.int64)
+        nat.check(L.pg_plan_info(h, nat.ptr(info)))
+        ep = np.empty(int(info[4]) + 1, np.int32)
+        rc = L.pg_plan_entity_pairs(h, nat.ptr(ep))
+    finally:
+        L.pg_plan_destroy(h)
+    if p == 3:
+        assert rc == nat.PG_ERR_UNSUPPORTED and "row length 303" in L.pg_last_error().decode()
+        return
+    assert rc == 0
+    _, pl = _plan(T)
+    a = pl["ab"][0::2]
+    assert ep[0] == 0 and ep[-1] == a.size
+    assert np.array_equal(ep, np.searchsorted(a, np.arange(ep.size), side="left"))
+    # every entity with DOFs has its diagonal pair among its pairs
+    b = pl["ab"][1::2]
+    for A in np.unique(a):
+        assert (b[ep[A]:ep[A + 1]] == A).sum() == 1
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
 def test_reference_tables_match_oracle_contraction(p):
     V = B.ndof_element(p) and (12 * p + 24 * p * (p - 1) + p * (p - 1) * (p - 2) // 2)
     T = np.zeros(V * V * 12)
