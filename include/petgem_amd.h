@@ -100,17 +100,19 @@ int pg_plan_export(const void *plan, int32_t *rowptr, int32_t *cols, int32_t *pa
                    uint32_t *contrib);
 int pg_plan_destroy(void *plan);
 /* First pair of every entity: the pairs (A, B) of entity A are
- * [entity_pair[A], entity_pair[A+1]) (n_entities + 1 values, host array).
+ * [entity_pair[A], entity_pair[A+1]) (n_entities + 1 values, host array), and
+ * (max_rowlen, nullable) the longest row of an entity with DOFs.
  * PG_ERR_UNSUPPORTED when an entity's rows exceed the bounds of the row-major
  * numeric assembly (192 columns, 96 coupled entities, 24 elements); leave
  * pg_plan_dev.entity_pair NULL then.  Part of the plan export above
  * (createParallelMatrix preallocation, petgem/parallel.py:150-177). */
-int pg_plan_entity_pairs(const void *plan, int32_t *entity_pair);
+int pg_plan_entity_pairs(const void *plan, int32_t *entity_pair, int32_t *max_rowlen);
 
 /* Device view of an exported plan (all device pointers).  entity_pair
- * (nullable, from pg_plan_entity_pairs) selects the row-major numeric assembly
- * at p <= 2: one workgroup per entity sums its CSR rows in LDS and writes them
- * contiguously; without it the two-phase kernels run (same values, bitwise). */
+ * (nullable, from pg_plan_entity_pairs, with entity_row_max its max_rowlen)
+ * selects the row-major numeric assembly at p <= 2: one workgroup per entity
+ * sums its CSR rows in LDS and writes them contiguously; without it the
+ * two-phase kernels run (same values, bitwise). */
 typedef struct pg_plan_dev {
   int32_t p;
   int64_t n_edges, n_faces, n_elems;
@@ -119,6 +121,7 @@ typedef struct pg_plan_dev {
   const int32_t *pair_slot0, *pair_rowlen, *pair_cbeg, *pair_ab;
   const uint32_t *contrib;
   const int32_t *entity_pair;
+  int32_t entity_row_max;
 } pg_plan_dev;
 
 /* Numeric assembly, atomic-free: CSR values (nnz complex128) of

@@ -25,7 +25,8 @@ class PlanDev(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int32), ("n_edges", _I64), ("n_faces", _I64), ("n_elems", _I64),
                 ("ndof", _I64), ("nnz", _I64), ("npairs", _I64),
                 ("rowptr", _P), ("cols", _P), ("pair_slot0", _P), ("pair_rowlen", _P),
-                ("pair_cbeg", _P), ("pair_ab", _P), ("contrib", _P), ("entity_pair", _P)]
+                ("pair_cbeg", _P), ("pair_ab", _P), ("contrib", _P), ("entity_pair", _P),
+                ("entity_row_max", ctypes.c_int32)]
 
 
 class SolveOpts(ctypes.Structure):
@@ -55,7 +56,7 @@ SIGNATURES = {
     "pg_plan_build": (ctypes.c_int, [ctypes.c_int, _I64, _I64, _I64, _P, _P, ctypes.POINTER(_P)]),
     "pg_plan_info": (ctypes.c_int, [_P, _P]),
     "pg_plan_export": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
-    "pg_plan_entity_pairs": (ctypes.c_int, [_P, _P]),
+    "pg_plan_entity_pairs": (ctypes.c_int, [_P, _P, _P]),
     "pg_plan_destroy": (ctypes.c_int, [_P]),
     "pg_assemble": (ctypes.c_int, [_P, ctypes.POINTER(PlanDev), _P, _D, _P, _P, _P]),
     "pg_assemble_km": (ctypes.c_int, [_P, ctypes.POINTER(PlanDev), _P, _P, _P, _P, _P]),

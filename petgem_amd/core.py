@@ -181,7 +181,8 @@ class Plan:
             # first pair per entity, when every entity fits the row-major
             # numeric assembly (else the two-phase kernels run)
             epair = np.empty(self.n_entities + 1, dtype=np.int32)
-            rc = L.pg_plan_entity_pairs(h, nat.ptr(epair))
+            row_max = ctypes.c_int32(0)
+            rc = L.pg_plan_entity_pairs(h, nat.ptr(epair), ctypes.byref(row_max))
             if rc == nat.PG_ERR_UNSUPPORTED:
                 epair = None
             else:
@@ -205,7 +206,8 @@ class Plan:
             self.rowptr.data_ptr(), self.cols.data_ptr(), self.slot0.data_ptr(),
             self.rowlen.data_ptr(), self.cbeg.data_ptr(), self.ab.data_ptr(),
             self.contrib.data_ptr(),
-            None if self.entity_pair is None else self.entity_pair.data_ptr())
+            None if self.entity_pair is None else self.entity_pair.data_ptr(),
+            0 if self.entity_pair is None else row_max.value)
 
 
 class Assembler:

@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(256) k_asm_sums(
 // the two-phase result, with no block scratch; A's rows then leave as one
 // contiguous run of d(A) * rowlen values.
 // -------------------------------------------------------------------------
-template <int P>
+template <int P, int LEN>
 __global__ void __launch_bounds__(64) k_asm_rows(
     int64_t nE, int64_t nF, const int32_t* __restrict__ epair, const double* __restrict__ T,
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ pair_slot0,
@@ -499,13 +499,13 @@ __global__ void __launch_bounds__(64) k_asm_rows(
   constexpr int DE = P, DF = P * (P - 1), DV = P * (P - 1) * (P - 2) / 2;
   constexpr int NLD = 6 * DE + 4 * DF + DV;
   constexpr int V = 12 * P + 24 * P * (P - 1) + P * (P - 1) * (P - 2) / 2;
-  __shared__ double2 acc[MAXD * kRowAsmMaxLen];
+  __shared__ double2 acc[MAXD * LEN];
   __shared__ int16_t offs[kRowAsmMaxInc * NL];  // column of entity(e_i, lb) in A's rows
   __shared__ int32_t inc_e[kRowAsmMaxInc];
   __shared__ int32_t inc_la[kRowAsmMaxInc];
   __shared__ int32_t pr_cbeg[kRowAsmMaxPairs + 1];
   __shared__ int16_t pr_off[kRowAsmMaxPairs];
-  __shared__ uint8_t colbc[kRowAsmMaxLen];
+  __shared__ uint8_t colbc[LEN];
   __shared__ int32_t s_pd;
   const int lane = threadIdx.x;
   const int64_t A = blockIdx.x;  // the grid covers the entities with DOFs
@@ -517,7 +517,7 @@ __global__ void __launch_bounds__(64) k_asm_rows(
   const int rowlen = (int)(rowptr[r0 + 1] - rowbase);
   const int p0 = epair[A], npr = epair[A + 1] - p0;
   // (the plan export admits only plans within the bounds)
-  if (dA == 0 || npr <= 0 || npr > kRowAsmMaxPairs || rowlen > kRowAsmMaxLen) return;
+  if (dA == 0 || npr <= 0 || npr > kRowAsmMaxPairs || rowlen > LEN) return;
   const int bcA = ent_bc ? ent_bc[A] : 0;
   // A's pairs: contributor ranges, column offsets, PEC columns, the diagonal pair
   for (int q = lane; q < npr; q += 64) {
@@ -1023,15 +1023,21 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
   if (t->p <= 2 && pl->entity_pair) {
     const int64_t nent = t->p == 1 ? pl->n_edges : pl->n_edges + pl->n_faces;
     const int km = Kout ? 1 : 0;
-#define PG_ROWS(P)                                                                             \
-  hipLaunchKernelGGL(k_asm_rows<P>, dim3((unsigned)nent), dim3(64), 0, s, pl->n_edges,          \
+    // the short-row variant when the plan's longest row fits it (more workgroups resident)
+    const bool short_rows = pl->entity_row_max > 0 && pl->entity_row_max <= kRowAsmShortLen;
+#define PG_ROWS(P, LEN)                                                                        \
+  hipLaunchKernelGGL((k_asm_rows<P, LEN>), dim3((unsigned)nent), dim3(64), 0, s, pl->n_edges,   \
                      pl->n_faces, pl->entity_pair, t->d_T, pl->rowptr, pl->pair_slot0,          \
                      pl->pair_cbeg, pl->pair_ab, pl->contrib, ws_coef(w),                       \
                      ws_orient(w, pl->n_elems), om, km, ent_bc, v, Kout, Mout)
-    if (t->p == 1) {
-      PG_ROWS(1);
+    if (t->p == 1 && short_rows) {
+      PG_ROWS(1, kRowAsmShortLen);
+    } else if (t->p == 1) {
+      PG_ROWS(1, kRowAsmMaxLen);
+    } else if (short_rows) {
+      PG_ROWS(2, kRowAsmShortLen);
     } else {
-      PG_ROWS(2);
+      PG_ROWS(2, kRowAsmMaxLen);
     }
 #undef PG_ROWS
     PG_CHECK_LAUNCH();

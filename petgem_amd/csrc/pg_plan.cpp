@@ -279,7 +279,7 @@ int pg_plan_export(const void* plan, int32_t* rowptr, int32_t* cols, int32_t* sl
   return PG_OK;
 }
 
-int pg_plan_entity_pairs(const void* plan, int32_t* entity_pair) {
+int pg_plan_entity_pairs(const void* plan, int32_t* entity_pair, int32_t* max_rowlen) {
   if (!plan || !entity_pair) PG_FAIL(PG_ERR_ARG, "NULL argument");
   const Plan* P = static_cast<const Plan*>(plan);
   if (P->max_rowlen > kRowAsmMaxLen || P->max_pairs > kRowAsmMaxPairs ||
@@ -290,6 +290,7 @@ int pg_plan_entity_pairs(const void* plan, int32_t* entity_pair) {
             (long long)P->max_rowlen, kRowAsmMaxLen, (long long)P->max_pairs, kRowAsmMaxPairs,
             (long long)P->max_inc, kRowAsmMaxInc);
   std::memcpy(entity_pair, P->entity_pair.data(), P->entity_pair.size() * sizeof(int32_t));
+  if (max_rowlen) *max_rowlen = (int32_t)P->max_rowlen;
   return PG_OK;
 }
 

@@ -173,9 +173,10 @@ def test_assembly_vs_oracle(p, n):
 
 @pytest.mark.parametrize("p,n", [(1, 6), (2, 5), (2, 8)])
 def test_row_major_assembly_is_bitwise_the_two_phase(p, n):
-    """The row-major kernel (plans exported with entity_pair) and the two-phase
-    kernels (entity_pair NULL) give bitwise the same CSR values, with and
-    without the PEC fold, for A(omega) and for K, M."""
+    """The row-major kernel (plans exported with entity_pair; its short-row
+    variant, and the long-row one when entity_row_max is unset) and the
+    two-phase kernels (entity_pair NULL) give bitwise the same CSR values, with
+    and without the PEC fold, for A(omega) and for K, M."""
     w = Workload(n, p, noise=0.1, seed=5)
     T = w.topo
     mesh = core.DeviceMesh.from_topology(T, DEV)
@@ -185,18 +186,22 @@ def test_row_major_assembly_is_bitwise_the_two_phase(p, n):
     omega = core.omega_of(w.frequency)
     ent_bc, _ = mesh.entity_mask(T.boundary_dofs)
     res = {}
-    for mode in ("rows", "two_phase"):
-        saved = asm.plan.dev_struct.entity_pair
+    assert 0 < asm.plan.dev_struct.entity_row_max <= 96
+    for mode in ("rows", "rows_long", "two_phase"):
+        saved = (asm.plan.dev_struct.entity_pair, asm.plan.dev_struct.entity_row_max)
+        if mode == "rows_long":
+            asm.plan.dev_struct.entity_row_max = 0
         if mode == "two_phase":
             asm.plan.dev_struct.entity_pair = None
         try:
             res[mode] = (asm.assemble(omega).clone(), asm.assemble(omega, ent_bc=ent_bc).clone(),
                          [t.clone() for t in core.assemble_km(asm, ent_bc)])
         finally:
-            asm.plan.dev_struct.entity_pair = saved
-    (a0, a1, km_a), (b0, b1, km_b) = res["rows"], res["two_phase"]
-    assert torch.equal(a0, b0) and torch.equal(a1, b1)
-    assert torch.equal(km_a[0], km_b[0]) and torch.equal(km_a[1], km_b[1])
+            asm.plan.dev_struct.entity_pair, asm.plan.dev_struct.entity_row_max = saved
+    for mode in ("rows", "rows_long"):
+        (a0, a1, km_a), (b0, b1, km_b) = res[mode], res["two_phase"]
+        assert torch.equal(a0, b0) and torch.equal(a1, b1), mode
+        assert torch.equal(km_a[0], km_b[0]) and torch.equal(km_a[1], km_b[1]), mode
 
 
 def test_dirichlet_fold_equals_separate():

@@ -105,13 +105,15 @@ def test_plan_entity_pairs_index_each_entity_s_pairs(test_mesh, p):
         info = np.zeros(6, np.int64)
         nat.check(L.pg_plan_info(h, nat.ptr(info)))
         ep = np.empty(int(info[4]) + 1, np.int32)
-        rc = L.pg_plan_entity_pairs(h, nat.ptr(ep))
+        rmax = ctypes.c_int32(0)
+        rc = L.pg_plan_entity_pairs(h, nat.ptr(ep), ctypes.byref(rmax))
     finally:
         L.pg_plan_destroy(h)
     if p == 3:
         assert rc == nat.PG_ERR_UNSUPPORTED and "row length 303" in L.pg_last_error().decode()
         return
     assert rc == 0
+    assert rmax.value == np.diff(_plan(T)[1]["rowptr"]).max()
     _, pl = _plan(T)
     a = pl["ab"][0::2]
     assert ep[0] == 0 and ep[-1] == a.size
