@@ -502,7 +502,7 @@ __global__ void __launch_bounds__(64) k_asm_rows(
   __shared__ double2 acc[MAXD * LEN];
   __shared__ int16_t offs[kRowAsmMaxInc * NL];  // column of entity(e_i, lb) in A's rows
   __shared__ int32_t inc_e[kRowAsmMaxInc];
-  __shared__ int32_t inc_la[kRowAsmMaxInc];
+  __shared__ int32_t inc_o[kRowAsmMaxInc];  // la << 24 | orientation code (18 bits)
   __shared__ int32_t pr_cbeg[kRowAsmMaxPairs + 1];
   __shared__ int16_t pr_off[kRowAsmMaxPairs];
   __shared__ uint8_t colbc[LEN];
@@ -537,17 +537,27 @@ __global__ void __launch_bounds__(64) k_asm_rows(
   const int pd = s_pd;
   const int ci0 = pr_cbeg[pd], ninc = pr_cbeg[pd + 1] - ci0;
   if (ninc > kRowAsmMaxInc) return;  // uniform
-  // the elements containing A, ascending: the diagonal pair's contributors
+  // every contributor (e, la, lb) of A's pairs, loaded while the elements
+  // containing A (the diagonal pair's contributors, ascending) are listed with
+  // A's local entity and the element's orientation code
+  constexpr int CPL = (kRowAsmMaxInc * NL + 63) / 64;  // contributors per lane
+  const int c0 = pr_cbeg[0], c1 = pr_cbeg[npr];
+  uint32_t cu[CPL];
+#pragma unroll
+  for (int m = 0; m < CPL; ++m) cu[m] = c0 + lane + 64 * m < c1 ? contrib[c0 + lane + 64 * m] : 0u;
   for (int i = lane; i < ninc; i += 64) {
     const uint32_t u = contrib[ci0 + i];
-    inc_e[i] = (int32_t)(u >> 8);
-    inc_la[i] = (int32_t)((u >> 4) & 15);
+    const int e = (int)(u >> 8);
+    inc_e[i] = e;
+    inc_o[i] = (int)(((u >> 4) & 15) << 24) | orient[e];
   }
   __syncthreads();
-  // column offset of entity(e, lb) for every contributor (e, la, lb) of A's pairs
-  const int c0 = pr_cbeg[0], c1 = pr_cbeg[npr];
-  for (int c = c0 + lane; c < c1; c += 64) {
-    const uint32_t u = contrib[c];
+  // column offset of entity(e, lb) for every contributor of A's pairs
+#pragma unroll
+  for (int m = 0; m < CPL; ++m) {
+    const int c = c0 + lane + 64 * m;
+    if (c >= c1) continue;
+    const uint32_t u = cu[m];
     const int e = (int)(u >> 8), lb = (int)(u & 15);
     int lo = 0, hi = npr - 1;  // the pair of c: last q with pr_cbeg[q] <= c
     while (lo < hi) {
@@ -567,8 +577,8 @@ __global__ void __launch_bounds__(64) k_asm_rows(
   // element block rows, ascending elements
   for (int i = 0; i < ninc; ++i) {
     const int64_t e = __builtin_amdgcn_readfirstlane(inc_e[i]);
-    const int la = __builtin_amdgcn_readfirstlane(inc_la[i]);
-    const int pk = orient[e];
+    const int io = __builtin_amdgcn_readfirstlane(inc_o[i]);
+    const int la = io >> 24, pk = io & 0xffffff;
     const double* cf = coef + 12 * e;
     const int va0 = entity_variant_base(la, orient_of(pk, la), P);
     for (int q = lane; q < dA * NLD; q += 64) {
