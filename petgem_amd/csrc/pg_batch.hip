@@ -265,8 +265,10 @@ __global__ void __launch_bounds__(kBS) k_spmm_er(const Ctl* ctl, int64_t nent, i
 // q_r = A_r p_r from the element operator's local results: for every entity of
 // its schedule the sum of the entity's local results over its elements, in
 // ascending slot order (atomic-free, fixed order), PEC entities identity rows;
-// with the partials of mu_r = p_r^T q_r.  gm[i] = {r0, d, s0, s1}; the local
-// result of slot g, entity DOF t, column r is ye[(g + 64 t) R + r].
+// with the PEC rows' partials of mu_r = p_r^T q_r (p_i^2 there: the elements'
+// share is summed by the local-product kernel, so p is read only at PEC
+// entities).  gm[i] = {r0, d, s0, s1}; the local result of slot g, entity DOF
+// t, column r is ye[(g + 64 t) R + r].
 template <int CT, int TMAX, int D, typename VT>
 __global__ void __launch_bounds__(kBS) k_meop_sum(const Ctl* ctl, int64_t nent, int R,
                                                   const int4* __restrict__ gm,
@@ -340,11 +342,12 @@ __global__ void __launch_bounds__(kBS) k_meop_sum(const Ctl* ctl, int64_t nent, 
 #pragma unroll
       for (int k = 0; k < D; ++k)
         if (t < T && k < d) {
-          const int64_t idx = (int64_t)(r0 + k) * R + c + t * CT;
-          Q[idx] = acc[t][k];
-          const double2 pv = to_d(P[idx]), qv = to_d(acc[t][k]);
-          v[t][0] += pv.x * qv.x - pv.y * qv.y;
-          v[t][1] += pv.x * qv.y + pv.y * qv.x;
+          Q[(int64_t)(r0 + k) * R + c + t * CT] = acc[t][k];
+          if (s0 == s1) {  // PEC row: q = p
+            const double2 pv = to_d(acc[t][k]);
+            v[t][0] += pv.x * pv.x - pv.y * pv.y;
+            v[t][1] += 2.0 * pv.x * pv.y;
+          }
         }
   }
   column_partials<CT, 2, TMAX>(R, v, part);
@@ -401,12 +404,21 @@ __global__ void k_check_entities(int64_t nent, const int32_t* __restrict__ bptr,
       }
 }
 
-// finish mu_r and alpha_r = rho_r / mu_r (one workgroup per column)
+// finish mu_r and alpha_r = rho_r / mu_r (one workgroup per column); part2
+// (nullable): a second set of np2 partials per column added after the first
 __global__ void __launch_bounds__(kBS) k_fin_mu(const Ctl* ctl, Col* col, int R,
-                                                const double* part, int np) {
+                                                const double* part, int np,
+                                                const double* part2, int np2) {
   if (ctl->all_done) return;
   double tot[2];
   column_totals<2>(part, R, np, tot);
+  if (part2) {
+    double t2[2];
+    __syncthreads();  // column_totals' shared scratch is reused
+    column_totals<2>(part2, R, np2, t2);
+    tot[0] += t2[0];
+    tot[1] += t2[1];
+  }
   if (threadIdx.x != 0) return;
   Col& cc = col[blockIdx.x];
   const double2 mu = make_double2(tot[0], tot[1]);
@@ -1198,6 +1210,8 @@ struct Batch {
   double2* ye64 = nullptr;
   float2* ye32 = nullptr;
   int sgrid = 1;
+  double* lpart = nullptr;  // the local products' x^T y partials (nlpart per column)
+  int nlpart = 0;
   std::vector<void*> owned;
   ~Batch() {  // release errors cannot be reported from here
     for (void* v : owned) (void)hipFree(v);
@@ -1231,7 +1245,7 @@ static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q,
     VT* ye = nullptr;
     if constexpr (sizeof(VT) == sizeof(float2)) ye = B->ye32;
     else ye = B->ye64;
-    eop_multi_local(B->eop, B->R, done, B->omu, P, ye, st);
+    eop_multi_local(B->eop, B->R, done, B->omu, P, ye, B->lpart, st);
     const EopSchedule& es = B->esched;
     np = B->sgrid;
     if (es.D <= 1)
@@ -1256,7 +1270,8 @@ static void launch_iter_t(Batch* B, hipStream_t st, VT* X, VT* Rr, VT* P, VT* Q,
     hipLaunchKernelGGL((k_spmm<CT, TM>), g, b, 0, st, B->ctl, B->n, B->R, B->rowptr, B->cols,
                        B->K, B->M, B->omu, (const double2*)P, (double2*)Q, B->part);
   }
-  hipLaunchKernelGGL(k_fin_mu, gr, b, 0, st, B->ctl, B->col, B->R, B->part, np);
+  hipLaunchKernelGGL(k_fin_mu, gr, b, 0, st, B->ctl, B->col, B->R, B->part, np,
+                     B->eop ? B->lpart : nullptr, B->nlpart);
   if (B->binv)
     hipLaunchKernelGGL((k_update_blk<CT, TM, VT>), g, b, 0, st, B->ctl, B->col, B->nent, B->R,
                        B->bptr, B->boff, B->nb2, B->binv, B->fidx, Rr, Q, B->part);
@@ -1691,6 +1706,10 @@ int pg_batch_set_element_operator(void* batch, void* op, const void* workspace, 
   const size_t nye = (size_t)es.nw * kWave * es.N * B->R;
   if ((rc = B->alloc(&B->ye64, nye)) != PG_OK) return rc;
   if (B->mixed && (rc = B->alloc(&B->ye32, nye)) != PG_OK) return rc;
+  const int64_t nlp = eop_multi_parts(E, B->R);
+  if (nlp >= (1LL << 31)) PG_FAIL(PG_ERR_UNSUPPORTED, "too many element slot groups");
+  B->nlpart = (int)nlp;
+  if ((rc = B->alloc(&B->lpart, (size_t)2 * B->R * nlp)) != PG_OK) return rc;
   const int64_t epb = kBS / B->CT;
   int64_t g = std::min<int64_t>(kMaxGrid, (es.nent + epb - 1) / epb);
   if (g >= 64) g = (g / 8) * 8;

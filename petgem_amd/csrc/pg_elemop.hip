@@ -512,6 +512,9 @@ __global__ void __launch_bounds__(kBS) k_eop_sum(int64_t nent, const int32_t* __
 // and applied to rows i and j.  The entries are formed once per element for
 // all R columns.  Local results ye[((w N + i) 64 + l) R + r]: the single-RHS
 // slot numbering times R, so the per-entity summation reuses the slot lists.
+// With `dpart` each thread also sums x_e,r^T y_e,r over its elements (FP64):
+// dpart[(k R + r) gridDim + block], k = 0 real, 1 imaginary part, the element
+// share of x_r^T A_r x_r (PEC DOFs are zero in x_e and add nothing here).
 // ---------------------------------------------------------------------------
 template <int P, typename RT>
 __global__ void __launch_bounds__(256) k_meop_local(
@@ -520,7 +523,7 @@ __global__ void __launch_bounds__(256) k_meop_local(
     const int32_t* __restrict__ edofs, const double* __restrict__ coefm,
     const double* __restrict__ tab, const int2* __restrict__ pdesc,
     const double* __restrict__ omu, const typename Cx<RT>::T* __restrict__ x,
-    typename Cx<RT>::T* __restrict__ ye) {
+    typename Cx<RT>::T* __restrict__ ye, double* __restrict__ dpart) {
   using C = typename Cx<RT>::T;
   constexpr int N = P * (P + 2) * (P + 3) / 2;
   constexpr int NP = N * (N + 1) / 2;
@@ -544,6 +547,7 @@ __global__ void __launch_bounds__(256) k_meop_local(
   for (int k = threadIdx.x; k < NTS; k += blockDim.x) tl[k] = t[k];
   __syncthreads();
   const int l0 = part * (kWave / split), l1 = l0 + kWave / split;
+  double dre = 0.0, dim = 0.0;  // x^T y over this thread's elements
   for (int l = l0; l < l1; ++l) {
     int fv[NLE];
     bool any = false;
@@ -606,7 +610,16 @@ __global__ void __launch_bounds__(256) k_meop_local(
       }
     }
 #pragma unroll
-    for (int i = 0; i < N; ++i) ye[(((int64_t)w * N + i) * kWave + l) * R + r] = y[i];
+    for (int i = 0; i < N; ++i) {
+      ye[(((int64_t)w * N + i) * kWave + l) * R + r] = y[i];
+      const double xr = xv[i].x, xi = xv[i].y, yr = y[i].x, yi = y[i].y;
+      dre += xr * yr - xi * yi;
+      dim += xr * yi + xi * yr;
+    }
+  }
+  if (dpart) {
+    dpart[(int64_t)r * gridDim.x + blockIdx.x] = dre;
+    dpart[((int64_t)R + r) * gridDim.x + blockIdx.x] = dim;
   }
 }
 
@@ -839,29 +852,32 @@ const float2* eop_local_segments(const ElemOp* op, const int32_t* done, int dstr
   return op->ye32;
 }
 
+// workgroups of R threads; a slot group split over 256 / R workgroups (at
+// most 4), about 4 waves per slot group at every width
+static int multi_split(int R) { return std::max(1, std::min(4, 256 / R)); }
+int64_t eop_multi_parts(const ElemOp* op, int R) { return op->nw * multi_split(R); }
+
 template <typename RT>
 static void multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
                         const typename eop::Cx<RT>::T* x, typename eop::Cx<RT>::T* ye,
-                        hipStream_t st) {
-  // workgroups of R threads; a slot group split over 256 / R workgroups (at
-  // most 4), about 4 waves per slot group at every width
-  const int split = std::max(1, std::min(4, 256 / R));
+                        double* part, hipStream_t st) {
+  const int split = multi_split(R);
   const dim3 g((unsigned)(op->nw * split)), b(R);
   const double* tab = op->tab64;
   if (op->p == 1)
     hipLaunchKernelGGL((eop::k_meop_local<1, RT>), g, b, 0, st, op->nw, R, split, done,
-                       op->wcombo, op->edofs, op->coefm, tab, op->pdesc, omu, x, ye);
+                       op->wcombo, op->edofs, op->coefm, tab, op->pdesc, omu, x, ye, part);
   else
     hipLaunchKernelGGL((eop::k_meop_local<2, RT>), g, b, 0, st, op->nw, R, split, done,
-                       op->wcombo, op->edofs, op->coefm, tab, op->pdesc, omu, x, ye);
+                       op->wcombo, op->edofs, op->coefm, tab, op->pdesc, omu, x, ye, part);
 }
 void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
-                     const double2* x, double2* ye, hipStream_t st) {
-  multi_local<double>(op, R, done, omu, x, ye, st);
+                     const double2* x, double2* ye, double* part, hipStream_t st) {
+  multi_local<double>(op, R, done, omu, x, ye, part, st);
 }
 void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
-                     const float2* x, float2* ye, hipStream_t st) {
-  multi_local<float>(op, R, done, omu, x, ye, st);
+                     const float2* x, float2* ye, double* part, hipStream_t st) {
+  multi_local<float>(op, R, done, omu, x, ye, part, st);
 }
 
 // Operator set-up from host copies of the element DOFs (hd, E x n), element

@@ -32,7 +32,10 @@ void eop_launch(const ElemOp* op, int mode, const int32_t* done, const float2* x
 // (once; `ws` = the geometry workspace the operator was created from) keeps
 // raw coefficients that pg_eop_refresh updates from then on.
 // eop_multi_local writes the local results ye (nw * 64 * N * R values) of
-// every element and column; the batch sums them per entity with the schedule
+// every element and column and, with part non-null, per workgroup and column
+// the FP64 sums of x_e^T y_e over its elements (part[(k R + r) np + b],
+// k = 0 real, 1 imaginary, np = eop_multi_parts partials per column: the
+// elements' share of x^T A x); the batch sums them per entity with the schedule
 // below (local result of slot g, entity DOF t, column r at ye[(g + 64 t) R + r];
 // s0 == s1 marks a PEC entity, an identity row).  R = 64, 128, 192 or 256.
 struct EopSchedule {
@@ -43,10 +46,11 @@ struct EopSchedule {
 };
 int eop_multi_prepare(ElemOp* op, const void* ws, hipStream_t st);
 EopSchedule eop_schedule(const ElemOp* op);
+int64_t eop_multi_parts(const ElemOp* op, int R);
 void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
-                     const double2* x, double2* ye, hipStream_t st);
+                     const double2* x, double2* ye, double* part, hipStream_t st);
 void eop_multi_local(const ElemOp* op, int R, const int32_t* done, const double* omu,
-                     const float2* x, float2* ye, hipStream_t st);
+                     const float2* x, float2* ye, double* part, hipStream_t st);
 
 // Multi-mesh operators (pg_eop_create_multi): the meshes' systems side by side
 // (block-diagonal), every wave of 64 elements within one mesh ("segment").
