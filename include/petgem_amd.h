@@ -110,9 +110,9 @@ int pg_plan_entity_pairs(const void *plan, int32_t *entity_pair, int32_t *max_ro
 
 /* Device view of an exported plan (all device pointers).  entity_pair
  * (nullable, from pg_plan_entity_pairs, with entity_row_max its max_rowlen)
- * selects the row-major numeric assembly at p <= 2: one workgroup per entity
+ * selects the row-major numeric assembly (p = 1-3): one workgroup per entity
  * sums its CSR rows in LDS and writes them contiguously; without it the
- * two-phase kernels run (same values, bitwise). */
+ * two-phase (p <= 2) or one-pass (p = 3) kernels run (same values, bitwise). */
 typedef struct pg_plan_dev {
   int32_t p;
   int64_t n_edges, n_faces, n_elems;

@@ -257,8 +257,9 @@ __device__ inline int entity_ndof_global(int64_t g, int64_t nE, int64_t nF) {
 }
 
 // -------------------------------------------------------------------------
-// One-pass assembly (used at p = 3): one thread per (entity pair, block row)
-// loops over the pair's contributors in ascending element order, contracting
+// One-pass assembly (p = 3 plans beyond the row-major bounds): one thread per
+// (entity pair, block row)
+// that loops over the pair's contributors in ascending element order, contracting
 // each entry from the tables and the element's coefficients.  At p = 3 the
 // two-phase form below would pad every local block to 6 x 6 (a 5.8 GB scratch
 // at C3) and loses to this form.
@@ -475,7 +476,7 @@ __global__ void __launch_bounds__(256) k_asm_sums(
 }
 
 // -------------------------------------------------------------------------
-// Row-major assembly (p = 1, 2; plans within the bounds of pg_rowasm.h): one
+// Row-major assembly (p = 1-3; plans within the bounds of pg_rowasm.h): one
 // single-wave workgroup per entity A with DOFs keeps A's d(A) CSR rows in LDS
 // and walks the elements containing A in ascending order (the contributors of
 // the diagonal pair (A, A)).  For each element its lanes form the entries of
@@ -1030,8 +1031,9 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
   const dim3 g(nblocks(threads, 256)), gb(nblocks(pl->n_elems, 256) * nloc * nloc), b(256);
   hipStream_t s = as_stream(stream);
   double2* v = reinterpret_cast<double2*>(values);
-  if (t->p <= 2 && pl->entity_pair) {
-    const int64_t nent = t->p == 1 ? pl->n_edges : pl->n_edges + pl->n_faces;
+  if (pl->entity_pair) {
+    const int64_t nent = t->p == 1 ? pl->n_edges
+                                   : pl->n_edges + pl->n_faces + (t->p == 3 ? pl->n_elems : 0);
     const int km = Kout ? 1 : 0;
     // the short-row variant when the plan's longest row fits it (more workgroups resident)
     const bool short_rows = pl->entity_row_max > 0 && pl->entity_row_max <= kRowAsmShortLen;
@@ -1044,10 +1046,14 @@ static int assemble_impl(const void* tables, const pg_plan_dev* pl, const void* 
       PG_ROWS(1, kRowAsmShortLen);
     } else if (t->p == 1) {
       PG_ROWS(1, kRowAsmMaxLen);
-    } else if (short_rows) {
+    } else if (t->p == 2 && short_rows) {
       PG_ROWS(2, kRowAsmShortLen);
-    } else {
+    } else if (t->p == 2) {
       PG_ROWS(2, kRowAsmMaxLen);
+    } else if (short_rows) {
+      PG_ROWS(3, kRowAsmShortLen);
+    } else {
+      PG_ROWS(3, kRowAsmMaxLen);
     }
 #undef PG_ROWS
     PG_CHECK_LAUNCH();

@@ -171,12 +171,13 @@ def test_assembly_vs_oracle(p, n):
     assert np.array_equal(vals.real, re) and np.array_equal(vals.imag, im)
 
 
-@pytest.mark.parametrize("p,n", [(1, 6), (2, 5), (2, 8)])
+@pytest.mark.parametrize("p,n", [(1, 6), (2, 5), (2, 8), (3, 3)])
 def test_row_major_assembly_is_bitwise_the_two_phase(p, n):
     """The row-major kernel (plans exported with entity_pair; its short-row
     variant, and the long-row one when entity_row_max is unset) and the
-    two-phase kernels (entity_pair NULL) give bitwise the same CSR values, with
-    and without the PEC fold, for A(omega) and for K, M."""
+    two-phase (p <= 2) or one-pass (p = 3) kernels (entity_pair NULL) give
+    bitwise the same CSR values, with and without the PEC fold, for A(omega)
+    and for K, M."""
     w = Workload(n, p, noise=0.1, seed=5)
     T = w.topo
     mesh = core.DeviceMesh.from_topology(T, DEV)
@@ -186,7 +187,7 @@ def test_row_major_assembly_is_bitwise_the_two_phase(p, n):
     omega = core.omega_of(w.frequency)
     ent_bc, _ = mesh.entity_mask(T.boundary_dofs)
     res = {}
-    assert 0 < asm.plan.dev_struct.entity_row_max <= 96
+    assert 0 < asm.plan.dev_struct.entity_row_max <= (96 if p < 3 else 192)
     for mode in ("rows", "rows_long", "two_phase"):
         saved = (asm.plan.dev_struct.entity_pair, asm.plan.dev_struct.entity_row_max)
         if mode == "rows_long":
