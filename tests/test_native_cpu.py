@@ -124,6 +124,57 @@ def test_plan_entity_pairs_index_each_entity_s_pairs(test_mesh, p):
         assert (b[ep[A]:ep[A + 1]] == A).sum() == 1
 
 
+@pytest.mark.parametrize("p,n", [(1, 4), (2, 4), (3, 3)])
+def test_plan_invariants_of_the_row_major_assembly(p, n):
+    """What the row-major assembly kernel indexes its LDS with: every entity
+    with DOFs has exactly one diagonal pair, whose contributors are its
+    elements in ascending order; its pairs' contributors number (elements) x
+    (local entities), each (element, local entity) exactly once; every pair
+    block lies inside the entity's row, at columns ascending with the pair."""
+    T = Workload(n, p, seed=1).topo
+    ee = np.ascontiguousarray(T.elems_edges, np.int32)
+    ef = np.ascontiguousarray(T.elems_faces, np.int32)
+    L = nat.lib()
+    h = ctypes.c_void_p()
+    nat.check(L.pg_plan_build(T.p, T.n_elems, T.n_edges, T.n_faces, nat.ptr(ee), nat.ptr(ef),
+                              ctypes.byref(h)))
+    try:
+        info = np.zeros(6, np.int64)
+        nat.check(L.pg_plan_info(h, nat.ptr(info)))
+        ep = np.empty(int(info[4]) + 1, np.int32)
+        rmax = ctypes.c_int32(0)
+        nat.check(L.pg_plan_entity_pairs(h, nat.ptr(ep), ctypes.byref(rmax)))
+    finally:
+        L.pg_plan_destroy(h)
+    _, pl = _plan(T)
+    nE, nF = T.n_edges, T.n_faces
+    de, df, dv = p, p * (p - 1), p * (p - 1) * (p - 2) // 2
+    nl = 6 + (4 if df else 0) + (1 if dv else 0)
+    nd = lambda A: de if A < nE else (df if A < nE + nF else dv)  # noqa: E731
+    ab, slot0, cbeg, contrib = pl["ab"], pl["slot0"], pl["cbeg"], pl["contrib"]
+    for A in range(ep.size - 1):
+        if nd(A) == 0:
+            assert ep[A] == ep[A + 1]
+            continue
+        r0 = A * de if A < nE else (nE * de + (A - nE) * df if A < nE + nF
+                                    else nE * de + nF * df + (A - nE - nF) * dv)
+        rowbase, rowlen = pl["rowptr"][r0], pl["rowptr"][r0 + 1] - pl["rowptr"][r0]
+        assert 0 < rowlen <= rmax.value <= 192
+        pis = np.arange(ep[A], ep[A + 1])
+        assert 0 < pis.size <= 96 and (ab[2 * pis] == A).all()
+        B = ab[2 * pis + 1]
+        off = slot0[pis] - rowbase
+        widths = np.array([nd(b) for b in B])
+        assert off[0] == 0 and (np.diff(off) == widths[:-1]).all() and off[-1] + widths[-1] == rowlen
+        (pd,) = np.nonzero(B == A)[0]
+        inc = contrib[cbeg[pis[pd]]:cbeg[pis[pd] + 1]] >> 8
+        assert 0 < inc.size <= 24 and (np.diff(inc.astype(np.int64)) > 0).all()
+        c = contrib[cbeg[pis[0]]:cbeg[pis[-1] + 1]]
+        assert c.size == inc.size * nl
+        keys = (c >> 8).astype(np.int64) * 16 + (c & 15)
+        assert np.unique(keys).size == c.size and np.isin(c >> 8, inc).all()
+
+
 @pytest.mark.parametrize("p", [1, 2, 3])
 def test_reference_tables_match_oracle_contraction(p):
     V = B.ndof_element(p) and (12 * p + 24 * p * (p - 1) + p * (p - 1) * (p - 2) // 2)
