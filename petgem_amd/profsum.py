@@ -12,7 +12,7 @@ dispatch), since the counters do not see every memory channel equally.
 
 Usage::
 
-    python -m petgem_amd.profsum stats gpurun_out/kt/kb_kernel_stats.csv --top 20
+    python -m petgem_amd.profsum stats prof/kb_kernel_stats.csv --top 20
     python -m petgem_amd.profsum traffic --fetch F.csv --write W.csv --copy-bytes 1073741824
 """
 import argparse
